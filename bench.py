@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Benchmark of the direct-sum RKF7(8) hot path on one MI355X.
+
+Workload (default): star + N = 16384 planetesimals (seed 2; a from
+dN/da ~ a^-0.5 on 0.5-5 AU, e/i Rayleigh 0.01/0.005, log-uniform masses
+1e-10..1e-8 Msun, 2 g/cm^3), all-pairs (SI class), RKF7(8) adaptive with the
+reference controller at tolerance 1e-10, mixed-precision pair kernel.
+A "step" is one accepted RKF7(8) step (13 force evaluations, plus 12 per
+rejected attempt).  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line:
+  value         ordered pair interactions per second over the timed steps
+                (whole job: all stages, rejections, controller included)
+  rkf78_steps_per_s, energy error, force error vs the FP64 kernel
+  roofline      pair kernel: 20 flop per pair over its HIP-event-timed
+                launch duration, against the 157.3 TFLOP/s FP32 peak
+  cpu_baseline  the oracle's restatement of the reference CPU path
+                (symmetric FP64 loop, pp_disk.cu:822-865) on one core, timed
+                over one RKF7(8) step of the same disk
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "red.cuda_amd"))
+
+import numpy as np  # noqa: E402
+
+import redamd as R  # noqa: E402
+
+PEAK_FP32 = 157.3e12
+FLOP_PER_PAIR = 20.0
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(r, v, p, b, nbt, h0, tol):
+    """One RKF7(8) step of the oracle's reference CPU path (1 thread)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    o = O.ODisk(r, v, p, b, nbt, formula=1)
+    oi = O.OInteg(o, 4, h0, adaptive=True, tol=tol)
+    t0 = time.perf_counter()
+    oi.step()
+    el = time.perf_counter() - t0
+    tried, passed, failed = oi.counters()
+    n = len(r)
+    evals = 1 + 12 * tried
+    pairs = evals * n * (n - 1.0)
+    return {"value": pairs / el, "unit": "pair interactions/s", "cores": 1, "kind": "port",
+            "steps_per_s": passed / el,
+            "sample": f"one RKF7(8) step ({tried} attempts, {evals} force evaluations) of the same N={n - 1} disk, "
+                      f"symmetric FP64 loop, {el:.1f} s on {cpu_model()} (nproc {os.cpu_count()})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    r, v, p, b, nbt = R.make_disk(args.n, args.seed)
+    # initial step as redinit: min orbital period / 1000 (src/red.cuda.initial/main.cu:317-342)
+    a_min = np.min(np.hypot(r["x"][1:], r["y"][1:]))
+    h0 = 2 * np.pi * np.sqrt(a_min ** 3 / 1.0) / 1000.0
+    d = R.Disk(r, v, p, b, nbt)
+    d.set_precision(R.RD_PREC_MIXED if args.precision == "mixed" else R.RD_PREC_FP64)
+    d.set_tuning(use_graph=1)
+    tuning = d.autotune() if (args.precision == "mixed" and not args.no_autotune) else d.tuning()
+    d.set_tuning(use_graph=1)
+    it = R.Integrator(d, R.RD_RKF78, h0, adaptive=True, tol=args.tol)
+    print(f"# warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    it.steps(args.warmup)
+    d.sync()
+    E0 = d.integral()["E"]
+    ev0, pairs0 = it.work()
+    tried0, passed0, failed0 = it.counters()
+    t_start = d.state()[0]
+    print(f"# timed {args.steps} steps", file=sys.stderr, flush=True)
+    d.sync()
+    t0 = time.perf_counter()
+    done = 0
+    chunk = 100
+    while done < args.steps:
+        k = min(chunk, args.steps - done)
+        it.steps(k)
+        done += k
+        print(f"# {done} steps", file=sys.stderr, flush=True)
+    d.sync()
+    el = time.perf_counter() - t0
+    ev1, pairs1 = it.work()
+    tried1, passed1, failed1 = it.counters()
+    E1 = d.integral()["E"]
+    t_end, rr, vv, _, _ = d.state()
+    pairs = pairs1 - pairs0
+    value = pairs / el
+    steps_per_s = (passed1 - passed0) / el
+    rel_e = abs(E1 - E0) / abs(E0)
+
+    # pair-kernel launch time (HIP events on the engine stream) for the roofline
+    ms, pairs_launch = d.bench_force(20)
+    achieved = FLOP_PER_PAIR * pairs_launch / (ms * 1e-3)
+
+    # force error of the mixed kernel against the FP64 reference kernel at the final state
+    f_err = None
+    if args.precision == "mixed":
+        am = d.calc_dydx(1, 0, t_end, rr, vv)
+        d.set_precision(R.RD_PREC_FP64)
+        ae = d.calc_dydx(1, 0, t_end, rr, vv)
+        gm = np.stack([am["x"], am["y"], am["z"]], 1)
+        ge = np.stack([ae["x"], ae["y"], ae["z"]], 1)
+        f_err = float((np.linalg.norm(gm - ge, axis=1) / np.linalg.norm(ge, axis=1))[1:].max())
+
+    out = {
+        "metric": "pair interactions/sec + RKF78 steps/sec at rel. energy error <=1e-9, 1 GPU",
+        "value": value,
+        "unit": "pair interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 state / f32 pair force" if args.precision == "mixed" else "f64",
+        "data": "synthetic seeded disk (generated in-process)",
+        "config": {"workload": f"N={args.n} planetesimals + star, RKF7(8) tol {args.tol:g}, "
+                               f"{args.precision} pair kernel, seed {args.seed}",
+                   "n_bodies": args.n + 1, "tolerance": args.tol, "precision": args.precision,
+                   "tuning": {"sinks_per_lane": tuning.sinks_per_lane, "chunk": tuning.chunk,
+                              "blocks_per_cu": tuning.blocks_per_cu, "use_graph": 1}},
+        "rkf78_steps_per_s": steps_per_s,
+        "attempts": tried1 - tried0,
+        "rejected": failed1 - failed0,
+        "force_evals": ev1 - ev0,
+        "sim_time_days": (t_end - t_start) / R.K_GAUSS,
+        "rel_energy_error": rel_e,
+        "energy_ok": bool(rel_e <= 1e-9),
+        "max_rel_force_err_vs_fp64": f_err,
+        "roofline": {"bound": "valu-fp32", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                     "kernel": "k_force_mixed" if args.precision == "mixed" else "k_force_exact",
+                     "ms_per_launch": ms, "pairs_per_launch": pairs_launch},
+    }
+    if not args.no_cpu:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(r, v, p, b, nbt, h0, args.tol)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,231 @@
+/*
+ * red_abi.h -- C ABI of the MI355X-native direct-sum N-body engine.
+ *
+ * This is the drop-in boundary for red.cuda's GPU path: the pp_disk body
+ * arrays, the pp_disk::calc_dydx operator, the RK4 / RKF7(8) integrators and
+ * the collision / ejection / hit-centrum event hooks.  Plain C types only;
+ * no exceptions cross it; every call returns an RD_* status and
+ * rd_last_error() describes the last failure of the calling thread.
+ *
+ * Reference citations are relative to the red.cuda repository root.
+ *
+ * Threading: one HIP stream per rd_disk; calls on one handle are not
+ * thread-safe, distinct handles may be used from distinct threads.
+ */
+#ifndef RED_ABI_H
+#define RED_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RD_ABI_VERSION 1
+
+/* Status codes.  The reference throws std::string (red.type/red_macro.h:37-95,
+ * src/red.cuda/pp_disk.cu:688, 974-982); here each throw site maps to a code. */
+enum {
+    RD_OK = 0,
+    RD_EINVAL = 1,     /* bad argument or inconsistent body arrays           */
+    RD_EDEVICE = 2,    /* HIP runtime error                                  */
+    RD_ENOTIMPL = 3,   /* the reference also refuses (e.g. super-planetesimal merge, pp_disk.cu:1333-1337) */
+    RD_EOVERFLOW = 4,  /* event buffer full (the reference does not guard, pp_disk.cu:1623) */
+    RD_ESTATE = 5,     /* call not valid in the current state                */
+    RD_ENOMEM = 6      /* device or host allocation failed                   */
+};
+
+/* body_type_t, red.type/red_type.h:169-179 (arrays are sorted by it) */
+enum {
+    RD_BODY_STAR = 0, RD_BODY_GIANTPLANET, RD_BODY_ROCKYPLANET, RD_BODY_PROTOPLANET,
+    RD_BODY_SUPERPLANETESIMAL, RD_BODY_PLANETESIMAL, RD_BODY_TESTPARTICLE, RD_BODY_TYPE_N
+};
+/* threshold_t, red.type/red_type.h:128-134 */
+enum { RD_THRESHOLD_HIT_CENTRUM_DISTANCE = 0, RD_THRESHOLD_EJECTION_DISTANCE = 1,
+       RD_THRESHOLD_RADII_ENHANCE_FACTOR = 2, RD_THRESHOLD_N = 3 };
+/* event_name_t, red.type/red_type.h:145-152 */
+enum { RD_EVENT_NONE = 0, RD_EVENT_HIT_CENTRUM = 1, RD_EVENT_EJECTION = 2, RD_EVENT_COLLISION = 3 };
+/* gas_decrease_t, red.type/red_type.h:105-111 */
+enum { RD_GAS_DENSITY_CONSTANT = 0, RD_GAS_DENSITY_DECREASE_LINEAR = 1, RD_GAS_DENSITY_DECREASE_EXPONENTIAL = 2 };
+/* integrator_type_t, red.type/red_type.h:136-143 (RK4 and RKF7(8) are built) */
+enum { RD_RK4 = 2, RD_RKF78 = 4 };
+
+/* Force precision of the pair sum. */
+enum {
+    RD_PREC_FP64 = 0,   /* reference formula m_j/(d*d^2) in FP64, j ascending per sink:
+                           bit-identical to kernel_pp_disk::calc_grav_accel_naive
+                           (src/red.cuda/pp_disk.cu:374-416)                      */
+    RD_PREC_MIXED = 1   /* star (and giant-planet) sources in FP64; all other pairs use
+                           FP64 offsets rounded to FP32, FP32 rsqrt and FP32 tile sums
+                           accumulated in FP64                                      */
+};
+/* Which bodies feel the gas.  The reference drags the NSI block only
+ * (n_objects_t::get_bound_GD, red.type/red_type.h:694-700). */
+enum { RD_DRAG_NSI = 0, RD_DRAG_ALL_MASSIVE = 1 };
+
+/* ---- body arrays, byte-compatible with the reference ------------------- */
+typedef struct rd_var4 { double x, y, z, w; } rd_var4;                 /* var4_t red_type.h:206-212 */
+typedef struct rd_param { double mass, radius, density, cd; } rd_param; /* pp_disk_t::param_t red_type.h:347-353 */
+typedef struct rd_body_md {                                             /* pp_disk_t::body_metadata_t red_type.h:356-362 */
+    int32_t id;          /* > 0; negated when the body becomes inactive */
+    int32_t body_type;
+    int32_t mig_type;
+    double  mig_stop_at;
+} rd_body_md;
+typedef struct rd_integral { rd_var4 R, V, C; double E; } rd_integral;  /* pp_disk_t::integral_t red_type.h:375-381 */
+typedef struct rd_event {                                               /* pp_disk_t::event_data_t red_type.h:415-461 */
+    int32_t event_name;
+    double  t, d;
+    int32_t id0; uint32_t idx0; rd_param p0; rd_var4 r0, v0;
+    int32_t id1; uint32_t idx1; rd_param p1; rd_var4 r1, v1;
+    int32_t id2; uint32_t idx2; rd_param p2; rd_var4 r2, v2;
+    rd_param ps; rd_var4 rs, vs;
+} rd_event;
+typedef struct rd_gas_params {                                          /* analytic_gas_disk_params_t red_type.h:464-485 */
+    double rho[2], sch[2], eta[2], tau[2], mfp[2], temp[2];
+    double c_vth;
+    int32_t gas_decrease;
+    double t0, t1, e_folding_time;
+    double alpha, mean_molecular_weight, particle_diameter;
+} rd_gas_params;
+
+/* Launch shapes of the pair kernels (chosen by rd_autotune or by hand). */
+typedef struct rd_tuning {
+    uint32_t sinks_per_lane;   /* mixed kernel: 1, 2 or 4 sinks per lane          */
+    uint32_t chunk;            /* mixed kernel: sources per work item (mult. of 256) */
+    uint32_t blocks_per_cu;    /* persistent grid size = blocks_per_cu * CUs      */
+    uint32_t use_graph;        /* capture one RK attempt in a hipGraph            */
+} rd_tuning;
+
+typedef struct rd_disk rd_disk;
+typedef struct rd_integrator rd_integrator;
+
+/* ---- library ------------------------------------------------------------ */
+const char* rd_last_error(void);
+int rd_version(void);
+int rd_device_count(int* n);
+
+/* ---- pp_disk: body arrays, state, options ------------------------------- */
+/* pp_disk(n_objects_t*, gas_disk_model_t, id_dev, comp_dev) + data load
+ * (src/red.cuda/pp_disk.h:14-15, pp_disk.cu:1534-1575).  n_by_type are the
+ * n_objects_t counts (red_type.h:524-541); bodies must be sorted by
+ * body_type with those counts.  gas may be NULL (GAS_DISK_MODEL_NONE).  The
+ * thresholds are the dc_threshold values (pp_disk.cu:24, 1681). */
+int rd_disk_create(uint32_t n_total, const uint32_t n_by_type[RD_BODY_TYPE_N],
+                   const rd_var4* r, const rd_var4* v, const rd_param* p, const rd_body_md* bmd,
+                   double t, const double threshold[RD_THRESHOLD_N], const rd_gas_params* gas,
+                   int device, rd_disk** out);
+/* Several independent disks in one engine (no reference counterpart: the
+ * reference runs one disk per process).  Arrays are the per-disk arrays
+ * concatenated; n_by_type holds n_disk*RD_BODY_TYPE_N counts; t may be NULL. */
+int rd_batch_create(uint32_t n_disk, const uint32_t* n_by_type,
+                    const rd_var4* r, const rd_var4* v, const rd_param* p, const rd_body_md* bmd,
+                    const double* t, const double threshold[RD_THRESHOLD_N], const rd_gas_params* gas,
+                    int device, rd_disk** out);
+/* ~pp_disk (pp_disk.cu:1577-1587) */
+int rd_disk_destroy(rd_disk* d);
+int rd_disk_set_precision(rd_disk* d, int precision);
+int rd_disk_set_drag_mode(rd_disk* d, int mode);
+int rd_disk_set_tuning(rd_disk* d, const rd_tuning* tuning);
+int rd_disk_get_tuning(rd_disk* d, rd_tuning* tuning);
+/* pp_disk::benchmark picks n_tpb by timing (pp_disk.cu:553-632); this times
+ * the mixed-precision pair kernel over its tuning space and keeps the best. */
+int rd_autotune(rd_disk* d, rd_tuning* best);
+int rd_disk_count(rd_disk* d, uint32_t* n_disk);
+/* playing body count of disk k (n_objects_t::get_n_total_playing) */
+int rd_disk_n_total(rd_disk* d, uint32_t k, uint32_t* n);
+/* n_objects_t playing / inactive / removed (red_type.h:702-705) */
+int rd_disk_n_objects(rd_disk* d, uint32_t k, uint32_t playing[RD_BODY_TYPE_N],
+                      uint32_t inactive[RD_BODY_TYPE_N], uint32_t removed[RD_BODY_TYPE_N]);
+/* pp_disk::copy_to_host + sim_data h_y / h_p / h_body_md (pp_disk.cu:1686-1699) and pp_disk::t */
+int rd_get_state(rd_disk* d, uint32_t k, double* t, rd_var4* r, rd_var4* v, rd_param* p, rd_body_md* bmd);
+/* pp_disk::copy_to_device (pp_disk.cu:1667-1684) of the positions and velocities of disk k */
+int rd_set_state(rd_disk* d, uint32_t k, double t, const rd_var4* r, const rd_var4* v);
+int rd_sync(rd_disk* d);
+
+/* ---- the operator -------------------------------------------------------- */
+/* pp_disk::calc_dydx(i, rr, t, r, v, dy) (pp_disk.h:114, pp_disk.cu:1450-1502)
+ * on host arrays of disk k: i = 0 gives dy = v, i = 1 gravity (+ drag).
+ * Copies in and out over PCIe; the integrators never go through it. */
+int rd_calc_dydx(rd_disk* d, uint32_t k, uint32_t i, uint32_t rr, double t,
+                 const rd_var4* r, const rd_var4* v, rd_var4* dy);
+/* Time n_iter launches of the pair kernel on the current state of all disks
+ * (pp_disk::benchmark_calc_grav_accel, pp_disk.cu:613-632).  Returns the mean
+ * kernel time in ms (HIP events on the engine's stream) and the ordered
+ * sink-source pair count of one launch. */
+int rd_bench_force(rd_disk* d, uint32_t n_iter, double* ms_per_launch, double* pairs_per_launch);
+
+/* ---- integrators --------------------------------------------------------- */
+/* rungekutta4 / rungekutta8 constructors (int_rungekutta4.h:15, int_rungekutta8.h:18) */
+int rd_integrator_create(rd_disk* d, int method, double dt, int adaptive, double tol, rd_integrator** out);
+int rd_integrator_destroy(rd_integrator* it);
+/* integrator::step (integrator.h:31): one accepted step of every disk;
+ * dt_did of disk 0 is returned. */
+int rd_step(rd_integrator* it, double* dt_did);
+/* n accepted steps per disk without host event handling in between */
+int rd_steps(rd_integrator* it, uint32_t n_steps);
+/* integrator::get_dt_next / set_dt_next (integrator.h:23-24) for disk k */
+int rd_get_dt_next(rd_integrator* it, uint32_t k, double* dt);
+int rd_set_dt_next(rd_integrator* it, uint32_t k, double dt);
+/* integrator n_tried/n_passed/n_failed (integrator.h:26-28) for disk k */
+int rd_counters(rd_integrator* it, uint32_t k, uint64_t* tried, uint64_t* passed, uint64_t* failed);
+/* integrator::error_check_for_tp (integrator.h:32) */
+int rd_set_error_check_for_tp(rd_integrator* it, int on);
+/* Number of pair-kernel launches and the ordered pairs they summed so far. */
+int rd_work_counters(rd_integrator* it, uint64_t* force_evals, double* pairs);
+
+/* ---- driver loop ---------------------------------------------------------- */
+/* run_simulation's loop (src/red.cuda/main.cu:470-625): ejection/hit-centrum
+ * check and handling, step, dt clamp to output_interval, collision check and
+ * handling, rebuild after 16 events; until |elapsed| > length or at most
+ * max_steps steps per disk.  No files are written. */
+int rd_run(rd_integrator* it, double length, double output_interval, uint64_t max_steps);
+
+/* ---- events ---------------------------------------------------------------- */
+/* pp_disk::check_for_ejection_hit_centrum (pp_disk.cu:1062-1076, 284-318) and
+ * check_for_collision (1078-1092, 321-371) on disk k; n = events found.  The
+ * event list is in canonical order: (idx, kind) for ejection / hit-centrum
+ * and (lower index, higher index) for collisions -- the reference's order is
+ * that of atomicAdd (pp_disk.cu:309, 357). */
+int rd_check_ejection_hit_centrum(rd_disk* d, uint32_t k, uint32_t* n);
+int rd_check_collision(rd_disk* d, uint32_t k, uint32_t* n);
+/* copy of the pending events of disk k (pp_disk::copy_event_data_to_host, pp_disk.cu:1710-1713) */
+int rd_get_events(rd_disk* d, uint32_t k, rd_event* out, uint32_t cap, uint32_t* n);
+/* pp_disk::handle_collision (pp_disk.cu:1220-1248) and
+ * handle_ejection_hit_centrum (1290-1326) on disk k's pending events */
+int rd_handle_collision(rd_disk* d, uint32_t k);
+int rd_handle_ejection_hit_centrum(rd_disk* d, uint32_t k);
+/* pp_disk::clear_event_counter (pp_disk.cu:1715-1722) */
+int rd_clear_event_counter(rd_disk* d, uint32_t k);
+/* all events handled so far on disk k, in handling order (what print_event_data writes) */
+int rd_event_log(rd_disk* d, uint32_t k, rd_event* out, uint32_t cap, uint32_t* n);
+/* pp_disk::rebuild_vectors (pp_disk.cu:1370-1430) on disk k */
+int rd_rebuild_vectors(rd_disk* d, uint32_t k);
+
+/* ---- integrals -------------------------------------------------------------- */
+/* pp_disk::calc_integral (pp_disk.cu:1504-1516): R, V, C and E of disk k,
+ * computed on the GPU in FP64 with compensated sums. */
+int rd_calc_integral(rd_disk* d, uint32_t k, rd_integral* out);
+
+/* ---- seeded disks ------------------------------------------------------------- */
+/* Star + n planetesimals from orbital elements, barycentric, tau units
+ * (src/red.cuda.initial/main.cu:251-342, 1662-1670; samplers
+ * src/red.cuda.initial/distribution.cpp:11-162).  size_mode 0: log-uniform
+ * mass on [lo, hi] Msun; 1: log-uniform radius on [lo, hi] km. */
+typedef struct rd_disk_spec {
+    uint32_t n, seed;
+    int32_t  body_type;
+    double   a_min, a_max, a_power;
+    double   sig_e, sig_i;
+    int32_t  size_mode;
+    double   lo, hi;
+    double   density_cgs, cd;
+    double   m_star, r_star;
+} rd_disk_spec;
+int rd_make_disk(const rd_disk_spec* spec, rd_var4* r, rd_var4* v, rd_param* p, rd_body_md* bmd);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1395 @@
+// engine.hip -- host side of the engine behind include/red_abi.h.
+//
+// One rd_disk holds one or more independent planetesimal disks on one GPU.
+// An integrator "attempt" is a fixed launch sequence over all disks:
+//   force(0) + stage(0)   [disks that start a new step: k0 = f(t, y_n)]
+//   comb1                  [stage-1 positions/velocities from k0]
+//   force(r) + stage(r)    r = 1 .. last stage
+// where every stage kernel finishes k_r for its body and forms the next
+// stage's inputs (one pass over HBM per stage), and the last stage folds the
+// Fehlberg error into a per-disk max.  The host reads one double per disk,
+// applies the reference step controller (int_rungekutta8.cu:693-708) with the
+// host's libm pow, and uploads the next per-disk (t, h, flags).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/red_abi.h"
+#include "engine.hpp"
+
+using namespace rdamd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(RD_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+constexpr int kNType = RD_BODY_TYPE_N;
+
+struct HostDisk {
+    uint32_t off = 0, n = 0;
+    uint32_t playing[kNType] = {0}, inactive[kNType] = {0}, removed[kNType] = {0};
+    double t = 0.0;
+    std::vector<rd_event> pending;
+    std::vector<rd_event> log;
+    uint32_t n_event[3] = {0, 0, 0};   // TOTAL, LAST_CLEAR, LAST_STEP (red_type.h:154-160)
+
+    uint32_t n_si() const { return playing[0] + playing[1] + playing[2] + playing[3]; }
+    uint32_t n_nsi() const { return playing[4] + playing[5]; }
+    uint32_t n_massive() const { return n_si() + n_nsi(); }
+    uint32_t n_active() const
+    {
+        uint32_t a = 0;
+        for (int k = 0; k < kNType; ++k) a += playing[k] - inactive[k];
+        return a;
+    }
+};
+
+template <typename T>
+int dev_alloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) return fail(RD_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return RD_OK;
+}
+template <typename T>
+void dev_free(T*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct rd_disk {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 256;
+    uint32_t n_disk = 0, n_all = 0;
+    std::vector<HostDisk> hd;
+
+    // host copies of the per-body data that the device does not need
+    std::vector<rd_body_md> bmd;
+
+    // device state
+    double *x = nullptr, *y = nullptr, *z = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    double *xo = nullptr, *yo = nullptr, *zo = nullptr, *vxo = nullptr, *vyo = nullptr, *vzo = nullptr;
+    double* P[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    double* K = nullptr;           // [13][6][n_all]
+    double* part = nullptr;
+    size_t part_len = 0;
+    double *mass = nullptr, *radius = nullptr, *density = nullptr, *cd = nullptr;
+    int32_t *id = nullptr, *type = nullptr;
+    double* m_ex = nullptr;
+    float* m32 = nullptr;
+    uint32_t* disk_of = nullptr;
+    DiskInfo* info = nullptr;
+    DiskCtl* ctl = nullptr;
+    unsigned long long* err = nullptr;
+    double* scratch = nullptr;
+    size_t scratch_len = 0;
+    double* red_out = nullptr;     // 12 doubles for integrals
+
+    // work lists
+    WorkItem* items_force = nullptr;
+    uint32_t n_items_force = 0, grid_force = 0;
+    WorkItem* items_coll = nullptr;
+    uint32_t n_items_coll = 0;
+    bool lists_valid = false;
+    double pairs_per_eval = 0.0;
+
+    // events
+    uint32_t* ev_buf = nullptr;
+    uint32_t* ev_counter = nullptr;
+    uint32_t ev_cap = 0;
+
+    // pinned host mirrors
+    DiskCtl* h_ctl = nullptr;
+    unsigned long long* h_err = nullptr;
+
+    std::vector<DiskInfo> h_info;
+
+    int precision = RD_PREC_FP64;
+    int drag_mode = RD_DRAG_NSI;
+    rd_tuning tuning{2, 1024, 4, 0};
+    double thr[3] = {0, 0, 0};
+    GasDev gas{};
+    uint64_t generation = 0;       // bumps when buffers / lists change (invalidates graphs)
+};
+
+struct rd_integrator {
+    rd_disk* d = nullptr;
+    int method = RD_RKF78;
+    int adaptive = 1;
+    double tol = 0.0;
+    int n_stage = 13, order = 7;
+    int err_all = 0;
+    std::vector<double> dt_try, dt_did, dt_next;
+    std::vector<uint64_t> tried, passed, failed;
+    uint64_t force_evals = 0;
+    double pairs = 0.0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    uint64_t graph_gen = ~0ull;
+    int graph_kind = -1;
+};
+
+namespace {
+
+// --------------------------------------------------------------------------
+// layout helpers
+// --------------------------------------------------------------------------
+double* kv_ptr(rd_disk* d, int r, int c) { return d->K + ((size_t)r * 6 + c) * d->n_all; }
+double* ka_ptr(rd_disk* d, int r, int c) { return d->K + ((size_t)r * 6 + 3 + c) * d->n_all; }
+
+void free_buffers(rd_disk* d)
+{
+    double** dp[] = {&d->x, &d->y, &d->z, &d->vx, &d->vy, &d->vz, &d->xo, &d->yo, &d->zo, &d->vxo, &d->vyo,
+                     &d->vzo, &d->K, &d->part, &d->mass, &d->radius, &d->density, &d->cd, &d->m_ex, &d->scratch,
+                     &d->red_out};
+    for (auto p : dp) dev_free(*p);
+    for (int b = 0; b < 2; ++b)
+        for (int c = 0; c < 3; ++c) dev_free(d->P[b][c]);
+    dev_free(d->id);
+    dev_free(d->type);
+    dev_free(d->m32);
+    dev_free(d->disk_of);
+    dev_free(d->info);
+    dev_free(d->ctl);
+    dev_free(d->err);
+    dev_free(d->items_force);
+    dev_free(d->items_coll);
+    dev_free(d->ev_buf);
+    dev_free(d->ev_counter);
+    if (d->h_ctl) (void)hipHostFree(d->h_ctl);
+    if (d->h_err) (void)hipHostFree(d->h_err);
+    d->h_ctl = nullptr;
+    d->h_err = nullptr;
+    d->part_len = 0;
+    d->scratch_len = 0;
+}
+
+int alloc_buffers(rd_disk* d)
+{
+    const size_t n = d->n_all;
+    double** dp[] = {&d->x, &d->y, &d->z, &d->vx, &d->vy, &d->vz, &d->xo, &d->yo, &d->zo, &d->vxo, &d->vyo,
+                     &d->vzo, &d->mass, &d->radius, &d->density, &d->cd, &d->m_ex};
+    for (auto p : dp)
+        if (int rc = dev_alloc(p, n)) return rc;
+    for (int b = 0; b < 2; ++b)
+        for (int c = 0; c < 3; ++c)
+            if (int rc = dev_alloc(&d->P[b][c], n)) return rc;
+    if (int rc = dev_alloc(&d->K, (size_t)kMaxStage * 6 * n)) return rc;
+    if (int rc = dev_alloc(&d->id, n)) return rc;
+    if (int rc = dev_alloc(&d->type, n)) return rc;
+    if (int rc = dev_alloc(&d->m32, n)) return rc;
+    if (int rc = dev_alloc(&d->disk_of, n)) return rc;
+    if (int rc = dev_alloc(&d->info, d->n_disk)) return rc;
+    if (int rc = dev_alloc(&d->ctl, d->n_disk)) return rc;
+    if (int rc = dev_alloc(&d->err, d->n_disk)) return rc;
+    if (int rc = dev_alloc(&d->red_out, 16)) return rc;
+    d->ev_cap = std::max<uint32_t>(4096u, (uint32_t)n);
+    if (int rc = dev_alloc(&d->ev_buf, 2 * (size_t)d->ev_cap)) return rc;
+    if (int rc = dev_alloc(&d->ev_counter, 1)) return rc;
+    HIP_TRY(hipHostMalloc((void**)&d->h_ctl, sizeof(DiskCtl) * std::max<uint32_t>(1, d->n_disk)));
+    HIP_TRY(hipHostMalloc((void**)&d->h_err, sizeof(unsigned long long) * std::max<uint32_t>(1, d->n_disk)));
+    HIP_TRY(hipMemsetAsync(d->K, 0, (size_t)kMaxStage * 6 * n * sizeof(double), d->stream));
+    HIP_TRY(hipMemsetAsync(d->ctl, 0, sizeof(DiskCtl) * d->n_disk, d->stream));
+    d->generation++;
+    return RD_OK;
+}
+
+// Host AoS arrays <-> device SoA for disk range [off, off + n).
+int upload_bodies(rd_disk* d, uint32_t off, uint32_t n, const rd_var4* r, const rd_var4* v, const rd_param* p,
+                  const rd_body_md* bmd)
+{
+    if (n == 0) return RD_OK;
+    std::vector<double> buf((size_t)n);
+    auto put = [&](double* dst, auto get) -> int {
+        for (uint32_t i = 0; i < n; ++i) buf[i] = get(i);
+        HIP_TRY(hipMemcpyAsync(dst + off, buf.data(), n * sizeof(double), hipMemcpyHostToDevice, d->stream));
+        HIP_TRY(hipStreamSynchronize(d->stream));
+        return RD_OK;
+    };
+    int rc = RD_OK;
+    if (r) {
+        if ((rc = put(d->x, [&](uint32_t i) { return r[i].x; }))) return rc;
+        if ((rc = put(d->y, [&](uint32_t i) { return r[i].y; }))) return rc;
+        if ((rc = put(d->z, [&](uint32_t i) { return r[i].z; }))) return rc;
+    }
+    if (v) {
+        if ((rc = put(d->vx, [&](uint32_t i) { return v[i].x; }))) return rc;
+        if ((rc = put(d->vy, [&](uint32_t i) { return v[i].y; }))) return rc;
+        if ((rc = put(d->vz, [&](uint32_t i) { return v[i].z; }))) return rc;
+    }
+    if (p) {
+        if ((rc = put(d->mass, [&](uint32_t i) { return p[i].mass; }))) return rc;
+        if ((rc = put(d->radius, [&](uint32_t i) { return p[i].radius; }))) return rc;
+        if ((rc = put(d->density, [&](uint32_t i) { return p[i].density; }))) return rc;
+        if ((rc = put(d->cd, [&](uint32_t i) { return p[i].cd; }))) return rc;
+    }
+    if (bmd) {
+        std::vector<int32_t> ib(n), tb(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            ib[i] = bmd[i].id;
+            tb[i] = bmd[i].body_type;
+        }
+        HIP_TRY(hipMemcpyAsync(d->id + off, ib.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, d->stream));
+        HIP_TRY(hipMemcpyAsync(d->type + off, tb.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, d->stream));
+        HIP_TRY(hipStreamSynchronize(d->stream));
+    }
+    return RD_OK;
+}
+
+int download_bodies(rd_disk* d, uint32_t off, uint32_t n, rd_var4* r, rd_var4* v, rd_param* p)
+{
+    if (n == 0) return RD_OK;
+    std::vector<double> a((size_t)n), b((size_t)n), c((size_t)n), e((size_t)n);
+    auto get3 = [&](const double* s0, const double* s1, const double* s2, rd_var4* out) -> int {
+        HIP_TRY(hipMemcpyAsync(a.data(), s0 + off, n * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+        HIP_TRY(hipMemcpyAsync(b.data(), s1 + off, n * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+        HIP_TRY(hipMemcpyAsync(c.data(), s2 + off, n * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+        HIP_TRY(hipStreamSynchronize(d->stream));
+        for (uint32_t i = 0; i < n; ++i) out[i] = rd_var4{a[i], b[i], c[i], 0.0};
+        return RD_OK;
+    };
+    int rc;
+    if (r && (rc = get3(d->x, d->y, d->z, r))) return rc;
+    if (v && (rc = get3(d->vx, d->vy, d->vz, v))) return rc;
+    if (p) {
+        HIP_TRY(hipMemcpyAsync(a.data(), d->mass + off, n * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+        HIP_TRY(hipMemcpyAsync(b.data(), d->radius + off, n * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+        HIP_TRY(hipMemcpyAsync(c.data(), d->density + off, n * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+        HIP_TRY(hipMemcpyAsync(e.data(), d->cd + off, n * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+        HIP_TRY(hipStreamSynchronize(d->stream));
+        for (uint32_t i = 0; i < n; ++i) p[i] = rd_param{a[i], b[i], c[i], e[i]};
+    }
+    return RD_OK;
+}
+
+// --------------------------------------------------------------------------
+// disk info + work lists
+// --------------------------------------------------------------------------
+void fill_info(rd_disk* d)
+{
+    d->h_info.resize(d->n_disk);
+    for (uint32_t k = 0; k < d->n_disk; ++k) {
+        const HostDisk& h = d->hd[k];
+        DiskInfo& di = d->h_info[k];
+        di = DiskInfo{};
+        di.off = h.off;
+        di.n = h.n;
+        di.n_si = h.n_si();
+        di.n_massive = h.n_massive();
+        di.n_fp64 = d->precision == RD_PREC_MIXED ? h.playing[0] + h.playing[1] : 0;
+        if (d->drag_mode == RD_DRAG_NSI) {
+            di.drag_lo = h.n_si();
+            di.drag_hi = h.n_si() + h.n_nsi();
+        } else {
+            di.drag_lo = h.playing[0];
+            di.drag_hi = h.n_massive();
+        }
+        di.err_hi = h.n_massive();
+    }
+}
+
+// (sink range, source range) of the three reference classes, local indices
+// (n_objects_t::get_bound_SI/NSI/NI, red_type.h:670-692).
+void class_bounds(const HostDisk& h, int cls, uint32_t b[4])
+{
+    const uint32_t si = h.n_si(), nsi = h.n_nsi(), mas = h.n_massive(), ni = h.playing[6];
+    if (cls == 0) { b[0] = 0; b[1] = si; b[2] = 0; b[3] = mas; }
+    else if (cls == 1) { b[0] = si; b[1] = si + nsi; b[2] = 0; b[3] = si; }
+    else { b[0] = mas; b[1] = mas + ni; b[2] = 0; b[3] = mas; }
+}
+
+int build_lists(rd_disk* d)
+{
+    fill_info(d);
+    std::vector<WorkItem> items;
+    std::vector<std::pair<double, uint32_t>> order;   // (cost, disk)
+    for (uint32_t k = 0; k < d->n_disk; ++k) order.push_back({(double)d->hd[k].n * d->hd[k].n, k});
+    std::stable_sort(order.begin(), order.end(), [](auto& a, auto& b) { return a.first > b.first; });
+    size_t max_chunks = 1;
+    double pairs = 0.0;
+    const uint32_t S = d->tuning.sinks_per_lane;
+    const uint32_t C = std::max<uint32_t>(kTile, d->tuning.chunk / kTile * kTile);
+    for (auto& o : order) {
+        const uint32_t k = o.second;
+        const HostDisk& h = d->hd[k];
+        DiskInfo& di = d->h_info[k];
+        for (int cls = 0; cls < 3; ++cls) {
+            uint32_t b[4];
+            class_bounds(h, cls, b);
+            di.nchunk[cls] = 0;
+            if (b[1] <= b[0]) continue;
+            const uint32_t nsink = b[1] - b[0];
+            pairs += (double)nsink * (b[3] - b[2]) - (cls == 0 ? nsink : 0);
+            if (d->precision == RD_PREC_FP64) {
+                di.nchunk[cls] = 1;
+                for (uint32_t s = b[0]; s < b[1]; s += kBlock)
+                    items.push_back(WorkItem{k, h.off + s, h.off + std::min(b[1], s + kBlock), h.off + b[2],
+                                             h.off + b[3], 0, 0, 0});
+            } else {
+                const uint32_t lo = std::max(b[2], di.n_fp64), hi = b[3];
+                const uint32_t nsrc = hi > lo ? hi - lo : 0;
+                const uint32_t nch = (nsrc + C - 1) / C;
+                di.nchunk[cls] = nch;
+                max_chunks = std::max<size_t>(max_chunks, nch);
+                const uint32_t G = 64u * S;
+                for (uint32_t s = b[0]; s < b[1]; s += G)
+                    for (uint32_t c = 0; c < nch; ++c)
+                        items.push_back(WorkItem{k, h.off + s, h.off + std::min(b[1], s + G),
+                                                 h.off + lo + c * C, h.off + std::min(hi, lo + (c + 1) * C), c, 0, 0});
+            }
+        }
+    }
+    d->pairs_per_eval = pairs;
+    // collision work: upper triangle in (256 sinks) x (4096 sources) tiles
+    std::vector<WorkItem> coll;
+    for (auto& o : order) {
+        const HostDisk& h = d->hd[o.second];
+        for (uint32_t s = 0; s < h.n; s += kBlock)
+            for (uint32_t j = s; j < h.n; j += 4096)
+                coll.push_back(WorkItem{o.second, h.off + s, h.off + std::min(h.n, s + kBlock), h.off + j,
+                                        h.off + std::min(h.n, j + 4096), 0, 0, 0});
+    }
+    dev_free(d->items_force);
+    dev_free(d->items_coll);
+    if (int rc = dev_alloc(&d->items_force, items.size())) return rc;
+    if (int rc = dev_alloc(&d->items_coll, coll.size())) return rc;
+    d->n_items_force = (uint32_t)items.size();
+    d->n_items_coll = (uint32_t)coll.size();
+    if (!items.empty())
+        HIP_TRY(hipMemcpy(d->items_force, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    if (!coll.empty())
+        HIP_TRY(hipMemcpy(d->items_coll, coll.data(), coll.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    const uint32_t maxgrid = std::max<uint32_t>(1, d->tuning.blocks_per_cu) * (uint32_t)d->n_cu;
+    d->grid_force = std::max<uint32_t>(1, std::min<uint32_t>(d->n_items_force, maxgrid));
+    const size_t need = max_chunks * 3 * (size_t)d->n_all;
+    if (need > d->part_len) {
+        dev_free(d->part);
+        if (int rc = dev_alloc(&d->part, need)) return rc;
+        d->part_len = need;
+        HIP_TRY(hipMemsetAsync(d->part, 0, need * sizeof(double), d->stream));
+    }
+    HIP_TRY(hipMemcpy(d->info, d->h_info.data(), d->n_disk * sizeof(DiskInfo), hipMemcpyHostToDevice));
+    HIP_TRY(launch_set_sources(d->stream, d->n_all, d->mass, d->id, d->disk_of, d->info, d->m_ex, d->m32));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    d->lists_valid = true;
+    d->generation++;
+    return RD_OK;
+}
+
+int ensure_lists(rd_disk* d) { return d->lists_valid ? RD_OK : build_lists(d); }
+
+int upload_disk_of(rd_disk* d)
+{
+    std::vector<uint32_t> dof(d->n_all);
+    for (uint32_t k = 0; k < d->n_disk; ++k)
+        for (uint32_t i = 0; i < d->hd[k].n; ++i) dof[d->hd[k].off + i] = k;
+    if (d->n_all) HIP_TRY(hipMemcpy(d->disk_of, dof.data(), d->n_all * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return RD_OK;
+}
+
+// --------------------------------------------------------------------------
+// launch helpers
+// --------------------------------------------------------------------------
+StageArgs make_args(rd_disk* d, int r, int adaptive, int err_all)
+{
+    StageArgs a{};
+    a.x = d->x; a.y = d->y; a.z = d->z; a.vx = d->vx; a.vy = d->vy; a.vz = d->vz;
+    a.xo = d->xo; a.yo = d->yo; a.zo = d->zo; a.vxo = d->vxo; a.vyo = d->vyo; a.vzo = d->vzo;
+    if (r == 0) { a.fx = d->x; a.fy = d->y; a.fz = d->z; }
+    else { a.fx = d->P[r & 1][0]; a.fy = d->P[r & 1][1]; a.fz = d->P[r & 1][2]; }
+    a.px = d->P[(r + 1) & 1][0]; a.py = d->P[(r + 1) & 1][1]; a.pz = d->P[(r + 1) & 1][2];
+    for (int s = 0; s < kMaxStage; ++s)
+        for (int c = 0; c < 3; ++c) {
+            a.kv[s][c] = kv_ptr(d, s, c);
+            a.ka[s][c] = ka_ptr(d, s, c);
+        }
+    a.kv[0][0] = d->vx; a.kv[0][1] = d->vy; a.kv[0][2] = d->vz;
+    a.part = d->part;
+    a.n_all = d->n_all;
+    a.err_all = err_all;
+    a.disk_of = d->disk_of;
+    a.info = d->info;
+    a.ctl = d->ctl;
+    a.err = d->err;
+    a.mass = d->mass; a.radius = d->radius; a.density = d->density; a.cd = d->cd;
+    a.id = d->id;
+    a.gas = d->gas;
+    a.mixed = d->precision == RD_PREC_MIXED;
+    a.adaptive = adaptive;
+    return a;
+}
+
+hipError_t enqueue_force(rd_disk* d, const double* fx, const double* fy, const double* fz, uint32_t need)
+{
+    if (d->precision == RD_PREC_FP64)
+        return launch_force_exact(d->stream, d->items_force, d->n_items_force, d->grid_force, fx, fy, fz, d->m_ex,
+                                  d->ctl, need, d->part, d->n_all);
+    return launch_force_mixed(d->stream, (int)d->tuning.sinks_per_lane, d->items_force, d->n_items_force,
+                              d->grid_force, fx, fy, fz, d->m32, d->ctl, need, d->part, d->n_all);
+}
+
+int last_stage(const rd_integrator* it) { return it->n_stage - 1; }
+
+// One attempt for every disk whose ctl flags say so.
+hipError_t enqueue_attempt(rd_integrator* it)
+{
+    rd_disk* d = it->d;
+    hipError_t e;
+    {
+        StageArgs a = make_args(d, 0, it->adaptive, it->err_all);
+        if ((e = enqueue_force(d, d->x, d->y, d->z, F_NEED_K0)) != hipSuccess) return e;
+        if ((e = launch_stage(d->stream, it->method, 0, a)) != hipSuccess) return e;
+        StageArgs a1 = make_args(d, 0, it->adaptive, it->err_all);   // comb1 writes P[1]
+        if ((e = launch_comb1(d->stream, it->method, a1)) != hipSuccess) return e;
+    }
+    for (int r = 1; r <= last_stage(it); ++r) {
+        StageArgs a = make_args(d, r, it->adaptive, it->err_all);
+        if ((e = enqueue_force(d, a.fx, a.fy, a.fz, F_ACTIVE)) != hipSuccess) return e;
+        if ((e = launch_stage(d->stream, it->method, r, a)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+int run_attempt(rd_integrator* it, bool use_graph)
+{
+    rd_disk* d = it->d;
+    if (use_graph) {
+        const int kind = it->method * 16 + it->adaptive * 2 + it->err_all;
+        if (!it->gexec || it->graph_gen != d->generation || it->graph_kind != kind) {
+            if (it->gexec) (void)hipGraphExecDestroy(it->gexec);
+            if (it->graph) (void)hipGraphDestroy(it->graph);
+            it->gexec = nullptr;
+            it->graph = nullptr;
+            HIP_TRY(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = enqueue_attempt(it);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(d->stream, &g);
+            if (e != hipSuccess) return fail(RD_EDEVICE, std::string("capture: ") + hipGetErrorString(e));
+            if (e2 != hipSuccess) return fail(RD_EDEVICE, std::string("end capture: ") + hipGetErrorString(e2));
+            it->graph = g;
+            HIP_TRY(hipGraphInstantiate(&it->gexec, g, nullptr, nullptr, 0));
+            it->graph_gen = d->generation;
+            it->graph_kind = kind;
+        }
+        HIP_TRY(hipGraphLaunch(it->gexec, d->stream));
+    } else {
+        HIP_TRY(enqueue_attempt(it));
+    }
+    return RD_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+const char* rd_last_error(void) { return g_err.c_str(); }
+int rd_version(void) { return RD_ABI_VERSION; }
+
+int rd_device_count(int* n)
+{
+    if (!n) return fail(RD_EINVAL, "n is NULL");
+    HIP_TRY(hipGetDeviceCount(n));
+    return RD_OK;
+}
+
+static int validate_disk(const uint32_t* nbt, const rd_body_md* bmd, uint32_t n)
+{
+    uint32_t tot = 0;
+    for (int t = 0; t < kNType; ++t) tot += nbt[t];
+    if (tot != n) return fail(RD_EINVAL, "n_by_type does not sum to the body count");
+    uint32_t i = 0;
+    for (int t = 0; t < kNType; ++t)
+        for (uint32_t q = 0; q < nbt[t]; ++q, ++i)
+            if (bmd[i].body_type != t)
+                return fail(RD_EINVAL, "bodies must be sorted by body_type with the counts of n_by_type (body " +
+                                           std::to_string(i) + ")");
+    for (uint32_t q = 0; q < n; ++q)
+        if (bmd[q].id == 0) return fail(RD_EINVAL, "body id 0 is not allowed (ids are > 0, negative = inactive)");
+    return RD_OK;
+}
+
+int rd_batch_create(uint32_t n_disk, const uint32_t* n_by_type, const rd_var4* r, const rd_var4* v,
+                    const rd_param* p, const rd_body_md* bmd, const double* t, const double threshold[3],
+                    const rd_gas_params* gas, int device, rd_disk** out)
+{
+    if (!out || !n_by_type || !r || !v || !p || !bmd || n_disk == 0) return fail(RD_EINVAL, "NULL argument");
+    *out = nullptr;
+    rd_disk* d = new rd_disk();
+    d->n_disk = n_disk;
+    d->hd.resize(n_disk);
+    uint32_t off = 0;
+    for (uint32_t k = 0; k < n_disk; ++k) {
+        HostDisk& h = d->hd[k];
+        uint32_t n = 0;
+        for (int q = 0; q < kNType; ++q) {
+            h.playing[q] = n_by_type[k * kNType + q];
+            n += h.playing[q];
+        }
+        if (int rc = validate_disk(h.playing, bmd + off, n)) {
+            delete d;
+            return rc;
+        }
+        h.off = off;
+        h.n = n;
+        h.t = t ? t[k] : 0.0;
+        off += n;
+    }
+    d->n_all = off;
+    d->bmd.assign(bmd, bmd + off);
+    if (threshold) std::memcpy(d->thr, threshold, sizeof(d->thr));
+    if (gas) {
+        d->gas.on = 1;
+        d->gas.rho0 = gas->rho[0]; d->gas.rho1 = gas->rho[1];
+        d->gas.sch0 = gas->sch[0]; d->gas.sch1 = gas->sch[1];
+        d->gas.eta0 = gas->eta[0]; d->gas.eta1 = gas->eta[1];
+        d->gas.t0 = gas->t0; d->gas.t1 = gas->t1; d->gas.e_folding_time = gas->e_folding_time;
+        d->gas.gas_decrease = gas->gas_decrease;
+    }
+    d->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) { delete d; return fail(RD_EDEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e)); }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        d->n_cu = prop.multiProcessorCount;
+    e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete d; return fail(RD_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+    int rc = alloc_buffers(d);
+    if (!rc) rc = upload_disk_of(d);
+    for (uint32_t k = 0; k < n_disk && !rc; ++k)
+        rc = upload_bodies(d, d->hd[k].off, d->hd[k].n, r + d->hd[k].off, v + d->hd[k].off, p + d->hd[k].off,
+                           bmd + d->hd[k].off);
+    if (!rc) rc = build_lists(d);
+    if (rc) {
+        std::string keep = g_err;
+        free_buffers(d);
+        (void)hipStreamDestroy(d->stream);
+        delete d;
+        g_err = keep;
+        return rc;
+    }
+    *out = d;
+    return RD_OK;
+}
+
+int rd_disk_create(uint32_t n_total, const uint32_t n_by_type[RD_BODY_TYPE_N], const rd_var4* r, const rd_var4* v,
+                   const rd_param* p, const rd_body_md* bmd, double t, const double threshold[3],
+                   const rd_gas_params* gas, int device, rd_disk** out)
+{
+    if (!n_by_type) return fail(RD_EINVAL, "n_by_type is NULL");
+    uint32_t tot = 0;
+    for (int q = 0; q < kNType; ++q) tot += n_by_type[q];
+    if (tot != n_total) return fail(RD_EINVAL, "n_by_type does not sum to n_total");
+    return rd_batch_create(1, n_by_type, r, v, p, bmd, &t, threshold, gas, device, out);
+}
+
+int rd_disk_destroy(rd_disk* d)
+{
+    if (!d) return RD_OK;
+    (void)hipSetDevice(d->device);
+    (void)hipStreamSynchronize(d->stream);
+    free_buffers(d);
+    (void)hipStreamDestroy(d->stream);
+    delete d;
+    return RD_OK;
+}
+
+int rd_disk_set_precision(rd_disk* d, int precision)
+{
+    if (!d || (precision != RD_PREC_FP64 && precision != RD_PREC_MIXED)) return fail(RD_EINVAL, "bad precision");
+    d->precision = precision;
+    d->lists_valid = false;
+    return build_lists(d);
+}
+
+int rd_disk_set_drag_mode(rd_disk* d, int mode)
+{
+    if (!d || (mode != RD_DRAG_NSI && mode != RD_DRAG_ALL_MASSIVE)) return fail(RD_EINVAL, "bad drag mode");
+    d->drag_mode = mode;
+    d->lists_valid = false;
+    return build_lists(d);
+}
+
+int rd_disk_set_tuning(rd_disk* d, const rd_tuning* t)
+{
+    if (!d || !t) return fail(RD_EINVAL, "NULL argument");
+    if (t->sinks_per_lane != 1 && t->sinks_per_lane != 2 && t->sinks_per_lane != 4)
+        return fail(RD_EINVAL, "sinks_per_lane must be 1, 2 or 4");
+    if (t->chunk < (uint32_t)kTile || t->chunk % kTile) return fail(RD_EINVAL, "chunk must be a multiple of 256");
+    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return fail(RD_EINVAL, "blocks_per_cu must be in [1, 64]");
+    d->tuning = *t;
+    d->lists_valid = false;
+    return build_lists(d);
+}
+
+int rd_disk_get_tuning(rd_disk* d, rd_tuning* t)
+{
+    if (!d || !t) return fail(RD_EINVAL, "NULL argument");
+    *t = d->tuning;
+    return RD_OK;
+}
+
+int rd_disk_count(rd_disk* d, uint32_t* n)
+{
+    if (!d || !n) return fail(RD_EINVAL, "NULL argument");
+    *n = d->n_disk;
+    return RD_OK;
+}
+
+int rd_disk_n_total(rd_disk* d, uint32_t k, uint32_t* n)
+{
+    if (!d || !n || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    *n = d->hd[k].n;
+    return RD_OK;
+}
+
+int rd_disk_n_objects(rd_disk* d, uint32_t k, uint32_t playing[RD_BODY_TYPE_N], uint32_t inactive[RD_BODY_TYPE_N],
+                      uint32_t removed[RD_BODY_TYPE_N])
+{
+    if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    if (playing) std::memcpy(playing, d->hd[k].playing, sizeof(d->hd[k].playing));
+    if (inactive) std::memcpy(inactive, d->hd[k].inactive, sizeof(d->hd[k].inactive));
+    if (removed) std::memcpy(removed, d->hd[k].removed, sizeof(d->hd[k].removed));
+    return RD_OK;
+}
+
+int rd_sync(rd_disk* d)
+{
+    if (!d) return fail(RD_EINVAL, "NULL disk");
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    return RD_OK;
+}
+
+int rd_get_state(rd_disk* d, uint32_t k, double* t, rd_var4* r, rd_var4* v, rd_param* p, rd_body_md* bmd)
+{
+    if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    const HostDisk& h = d->hd[k];
+    if (t) *t = h.t;
+    if (int rc = download_bodies(d, h.off, h.n, r, v, p)) return rc;
+    if (bmd) std::memcpy(bmd, d->bmd.data() + h.off, h.n * sizeof(rd_body_md));
+    return RD_OK;
+}
+
+int rd_set_state(rd_disk* d, uint32_t k, double t, const rd_var4* r, const rd_var4* v)
+{
+    if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    d->hd[k].t = t;
+    return upload_bodies(d, d->hd[k].off, d->hd[k].n, r, v, nullptr, nullptr);
+}
+
+// ---- operator ---------------------------------------------------------------
+int rd_calc_dydx(rd_disk* d, uint32_t k, uint32_t i, uint32_t rr, double t, const rd_var4* r, const rd_var4* v,
+                 rd_var4* dy)
+{
+    (void)rr;
+    if (!d || k >= d->n_disk || !r || !v || !dy) return fail(RD_EINVAL, "bad argument");
+    const HostDisk& h = d->hd[k];
+    if (i == 0) {
+        std::memcpy(dy, v, h.n * sizeof(rd_var4));
+        return RD_OK;
+    }
+    if (i != 1) return fail(RD_EINVAL, "Parameter 'i' is out of range.");
+    if (int rc = ensure_lists(d)) return rc;
+    // positions into P[0], velocities into the stage-1 velocity slot
+    std::vector<double> buf(h.n);
+    double* dst[6] = {d->P[0][0], d->P[0][1], d->P[0][2], kv_ptr(d, 1, 0), kv_ptr(d, 1, 1), kv_ptr(d, 1, 2)};
+    for (int c = 0; c < 6; ++c) {
+        const rd_var4* src = c < 3 ? r : v;
+        for (uint32_t q = 0; q < h.n; ++q) buf[q] = c % 3 == 0 ? src[q].x : (c % 3 == 1 ? src[q].y : src[q].z);
+        HIP_TRY(hipMemcpyAsync(dst[c] + h.off, buf.data(), h.n * sizeof(double), hipMemcpyHostToDevice, d->stream));
+        HIP_TRY(hipStreamSynchronize(d->stream));
+    }
+    for (uint32_t q = 0; q < d->n_disk; ++q) d->h_ctl[q] = DiskCtl{d->hd[q].t, 0.0, q == k ? F_NEED_K0 : 0u, 0};
+    HIP_TRY(hipMemcpyAsync(d->ctl, d->h_ctl, d->n_disk * sizeof(DiskCtl), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(enqueue_force(d, d->P[0][0], d->P[0][1], d->P[0][2], F_NEED_K0));
+    StageArgs a = make_args(d, 2, 0, 0);   // fx = P[0]
+    HIP_TRY(launch_operator(d->stream, a, t, k, ka_ptr(d, 0, 0), ka_ptr(d, 0, 1), ka_ptr(d, 0, 2), kv_ptr(d, 1, 0),
+                            kv_ptr(d, 1, 1), kv_ptr(d, 1, 2)));
+    std::vector<double> ax(h.n), ay(h.n), az(h.n);
+    HIP_TRY(hipMemcpyAsync(ax.data(), ka_ptr(d, 0, 0) + h.off, h.n * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipMemcpyAsync(ay.data(), ka_ptr(d, 0, 1) + h.off, h.n * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipMemcpyAsync(az.data(), ka_ptr(d, 0, 2) + h.off, h.n * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    for (uint32_t q = 0; q < h.n; ++q) dy[q] = rd_var4{ax[q], ay[q], az[q], 0.0};
+    return RD_OK;
+}
+
+int rd_bench_force(rd_disk* d, uint32_t n_iter, double* ms, double* pairs)
+{
+    if (!d || n_iter == 0) return fail(RD_EINVAL, "bad argument");
+    if (int rc = ensure_lists(d)) return rc;
+    for (uint32_t q = 0; q < d->n_disk; ++q) d->h_ctl[q] = DiskCtl{d->hd[q].t, 0.0, F_NEED_K0 | F_ACTIVE, 0};
+    HIP_TRY(hipMemcpyAsync(d->ctl, d->h_ctl, d->n_disk * sizeof(DiskCtl), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(enqueue_force(d, d->x, d->y, d->z, F_NEED_K0));   // warm-up
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, d->stream));
+    for (uint32_t q = 0; q < n_iter; ++q) HIP_TRY(enqueue_force(d, d->x, d->y, d->z, F_NEED_K0));
+    HIP_TRY(hipEventRecord(e1, d->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float el = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&el, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms) *ms = (double)el / n_iter;
+    if (pairs) *pairs = d->pairs_per_eval;
+    return RD_OK;
+}
+
+int rd_autotune(rd_disk* d, rd_tuning* best)
+{
+    if (!d) return fail(RD_EINVAL, "NULL disk");
+    if (d->precision != RD_PREC_MIXED) {
+        if (best) *best = d->tuning;
+        return RD_OK;
+    }
+    const rd_tuning keep = d->tuning;
+    rd_tuning win = keep;
+    double best_ms = 1e300;
+    const uint32_t S_opts[] = {1, 2, 4};
+    const uint32_t C_opts[] = {512, 1024, 2048, 4096, 8192};
+    const uint32_t B_opts[] = {1, 2, 4, 8};
+    uint32_t max_n = 0;
+    for (auto& h : d->hd) max_n = std::max(max_n, h.n);
+    for (uint32_t S : S_opts)
+        for (uint32_t C : C_opts) {
+            if (C > 2 * std::max<uint32_t>(max_n, kTile) && C != 512) continue;
+            for (uint32_t B : B_opts) {
+                rd_tuning t = keep;
+                t.sinks_per_lane = S;
+                t.chunk = C;
+                t.blocks_per_cu = B;
+                d->tuning = t;
+                if (int rc = build_lists(d)) { d->tuning = keep; build_lists(d); return rc; }
+                double ms = 0.0;
+                if (int rc = rd_bench_force(d, 3, &ms, nullptr)) { d->tuning = keep; build_lists(d); return rc; }
+                if (ms < best_ms) { best_ms = ms; win = t; }
+            }
+        }
+    d->tuning = win;
+    if (int rc = build_lists(d)) return rc;
+    if (best) *best = win;
+    return RD_OK;
+}
+
+// ---- integrators ---------------------------------------------------------------
+int rd_integrator_create(rd_disk* d, int method, double dt, int adaptive, double tol, rd_integrator** out)
+{
+    if (!d || !out) return fail(RD_EINVAL, "NULL argument");
+    if (method != RD_RK4 && method != RD_RKF78) return fail(RD_ENOTIMPL, "only RK4 and RKF78 are built");
+    rd_integrator* it = new rd_integrator();
+    it->d = d;
+    it->method = method;
+    it->adaptive = adaptive ? 1 : 0;
+    it->tol = tol;
+    if (method == RD_RKF78) { it->n_stage = adaptive ? 13 : 11; it->order = 7; }
+    else { it->n_stage = adaptive ? 5 : 4; it->order = 4; }
+    it->dt_try.assign(d->n_disk, dt);
+    it->dt_did.assign(d->n_disk, 0.0);
+    it->dt_next.assign(d->n_disk, dt);
+    it->tried.assign(d->n_disk, 0);
+    it->passed.assign(d->n_disk, 0);
+    it->failed.assign(d->n_disk, 0);
+    *out = it;
+    return RD_OK;
+}
+
+int rd_integrator_destroy(rd_integrator* it)
+{
+    if (!it) return RD_OK;
+    if (it->gexec) (void)hipGraphExecDestroy(it->gexec);
+    if (it->graph) (void)hipGraphDestroy(it->graph);
+    delete it;
+    return RD_OK;
+}
+
+// Advance every disk in `want` by one accepted step (reference step(),
+// int_rungekutta8.cu:635-716 / int_rungekutta4.cu:173-247).
+static int step_disks(rd_integrator* it, const std::vector<char>& want)
+{
+    rd_disk* d = it->d;
+    if (int rc = ensure_lists(d)) return rc;
+    const uint32_t nd = d->n_disk;
+    std::vector<char> pending(want);
+    std::vector<int> iter(nd, 0);
+    std::vector<char> need_k0(nd, 1);
+    for (uint32_t k = 0; k < nd; ++k)
+        if (pending[k]) it->dt_try[k] = it->dt_next[k];
+    const double lambda = it->method == RD_RKF78 ? 41.0 / 840.0 : 1.0 / 10.0;
+    uint32_t commit_flags_pending = 0;
+    std::vector<uint32_t> commit(nd, 0);
+    for (;;) {
+        bool any = false;
+        for (uint32_t k = 0; k < nd; ++k) {
+            uint32_t fl = commit[k];
+            if (pending[k]) {
+                fl |= F_ACTIVE | (need_k0[k] ? F_NEED_K0 : 0u);
+                it->dt_did[k] = it->dt_try[k];
+                any = true;
+            }
+            d->h_ctl[k] = DiskCtl{d->hd[k].t, it->dt_try[k], fl, 0};
+        }
+        HIP_TRY(hipMemcpyAsync(d->ctl, d->h_ctl, nd * sizeof(DiskCtl), hipMemcpyHostToDevice, d->stream));
+        if (commit_flags_pending) {
+            StageArgs a = make_args(d, 0, it->adaptive, it->err_all);
+            HIP_TRY(launch_commit(d->stream, a, 0, d->ctl));
+            commit_flags_pending = 0;
+            std::fill(commit.begin(), commit.end(), 0u);
+        }
+        if (!any) break;
+        if (int rc = run_attempt(it, d->tuning.use_graph != 0)) return rc;
+        uint32_t n_new = 0, n_act = 0;
+        for (uint32_t k = 0; k < nd; ++k) {
+            if (!pending[k]) continue;
+            n_act++;
+            n_new += need_k0[k];
+        }
+        it->force_evals += (uint64_t)(last_stage(it)) + (n_new ? 1 : 0);
+        if (it->adaptive) {
+            HIP_TRY(hipMemcpyAsync(d->h_err, d->err, nd * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                   d->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(d->stream));
+        // pair count of this attempt: active disks' full evaluation per stage
+        for (uint32_t k = 0; k < nd; ++k) {
+            if (!pending[k]) continue;
+            const HostDisk& h = d->hd[k];
+            double per = 0.0;
+            for (int cls = 0; cls < 3; ++cls) {
+                uint32_t b[4];
+                class_bounds(h, cls, b);
+                if (b[1] > b[0]) per += (double)(b[1] - b[0]) * (b[3] - b[2]) - (cls == 0 ? (b[1] - b[0]) : 0);
+            }
+            it->pairs += per * (last_stage(it) + (need_k0[k] ? 1 : 0));
+            double max_err = 0.0;
+            if (it->adaptive) {
+                double raw;
+                std::memcpy(&raw, &d->h_err[k], sizeof(double));
+                max_err = std::fabs(lambda * it->dt_try[k] * raw);
+                if (0.0 < max_err) it->dt_next[k] = it->dt_try[k] *= 0.9 * std::pow(it->tol / max_err, 1.0 / (it->order));
+                else it->dt_next[k] = it->dt_try[k] *= 2.0;
+            }
+            iter[k]++;
+            if (it->adaptive && max_err > it->tol) {
+                need_k0[k] = 0;       // k0 is reused (int_rungekutta8.cu:648-656)
+            } else {
+                it->tried[k] += iter[k];
+                it->failed[k] += iter[k] - 1;
+                it->passed[k]++;
+                d->hd[k].t += it->dt_did[k];
+                pending[k] = 0;
+                commit[k] = F_COMMIT;
+                commit_flags_pending = 1;
+            }
+        }
+        (void)n_act;
+    }
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    return RD_OK;
+}
+
+int rd_step(rd_integrator* it, double* dt_did)
+{
+    if (!it) return fail(RD_EINVAL, "NULL integrator");
+    std::vector<char> want(it->d->n_disk, 1);
+    if (int rc = step_disks(it, want)) return rc;
+    if (dt_did) *dt_did = it->dt_did[0];
+    return RD_OK;
+}
+
+int rd_steps(rd_integrator* it, uint32_t n_steps)
+{
+    if (!it) return fail(RD_EINVAL, "NULL integrator");
+    rd_disk* d = it->d;
+    if (!it->adaptive) {
+        // Fixed step: every attempt is accepted, so attempts are chained on the
+        // device with the time advanced by the commit kernel.
+        if (int rc = ensure_lists(d)) return rc;
+        const uint32_t nd = d->n_disk;
+        for (uint32_t k = 0; k < nd; ++k) {
+            it->dt_try[k] = it->dt_next[k];
+            it->dt_did[k] = it->dt_try[k];
+            d->h_ctl[k] = DiskCtl{d->hd[k].t, it->dt_try[k], F_ACTIVE | F_NEED_K0 | F_COMMIT, 0};
+        }
+        HIP_TRY(hipMemcpyAsync(d->ctl, d->h_ctl, nd * sizeof(DiskCtl), hipMemcpyHostToDevice, d->stream));
+        StageArgs a = make_args(d, 0, 0, it->err_all);
+        for (uint32_t s = 0; s < n_steps; ++s) {
+            if (int rc = run_attempt(it, d->tuning.use_graph != 0)) return rc;
+            HIP_TRY(launch_commit(d->stream, a, 1, d->ctl));
+            it->force_evals += (uint64_t)it->n_stage;
+            if ((s & 255u) == 255u) HIP_TRY(hipStreamSynchronize(d->stream));
+        }
+        for (uint32_t k = 0; k < nd; ++k) {
+            const HostDisk& h = d->hd[k];
+            double per = 0.0;
+            for (int cls = 0; cls < 3; ++cls) {
+                uint32_t b[4];
+                class_bounds(h, cls, b);
+                if (b[1] > b[0]) per += (double)(b[1] - b[0]) * (b[3] - b[2]) - (cls == 0 ? (b[1] - b[0]) : 0);
+            }
+            it->pairs += per * it->n_stage * n_steps;
+            for (uint32_t s = 0; s < n_steps; ++s) d->hd[k].t += it->dt_did[k];
+            it->tried[k] += n_steps;
+            it->passed[k] += n_steps;
+        }
+        HIP_TRY(hipStreamSynchronize(d->stream));
+        return RD_OK;
+    }
+    std::vector<char> want(d->n_disk, 1);
+    for (uint32_t s = 0; s < n_steps; ++s)
+        if (int rc = step_disks(it, want)) return rc;
+    return RD_OK;
+}
+
+int rd_get_dt_next(rd_integrator* it, uint32_t k, double* dt)
+{
+    if (!it || !dt || k >= it->d->n_disk) return fail(RD_EINVAL, "bad argument");
+    *dt = it->dt_next[k];
+    return RD_OK;
+}
+
+int rd_set_dt_next(rd_integrator* it, uint32_t k, double dt)
+{
+    if (!it || k >= it->d->n_disk) return fail(RD_EINVAL, "bad argument");
+    it->dt_next[k] = dt;
+    return RD_OK;
+}
+
+int rd_counters(rd_integrator* it, uint32_t k, uint64_t* tried, uint64_t* passed, uint64_t* failed)
+{
+    if (!it || k >= it->d->n_disk) return fail(RD_EINVAL, "bad argument");
+    if (tried) *tried = it->tried[k];
+    if (passed) *passed = it->passed[k];
+    if (failed) *failed = it->failed[k];
+    return RD_OK;
+}
+
+int rd_set_error_check_for_tp(rd_integrator* it, int on)
+{
+    if (!it) return fail(RD_EINVAL, "NULL integrator");
+    it->err_all = on ? 1 : 0;
+    return RD_OK;
+}
+
+int rd_work_counters(rd_integrator* it, uint64_t* evals, double* pairs)
+{
+    if (!it) return fail(RD_EINVAL, "NULL integrator");
+    if (evals) *evals = it->force_evals;
+    if (pairs) *pairs = it->pairs;
+    return RD_OK;
+}
+
+}  // extern "C"
+
+// ============================================================================
+// events, integrals, driver loop
+// ============================================================================
+namespace {
+
+struct HostView {
+    std::vector<rd_var4> r, v;
+    std::vector<rd_param> p;
+};
+
+int fetch(rd_disk* d, uint32_t k, HostView& hv)
+{
+    const HostDisk& h = d->hd[k];
+    hv.r.resize(h.n);
+    hv.v.resize(h.n);
+    hv.p.resize(h.n);
+    return download_bodies(d, h.off, h.n, hv.r.data(), hv.v.data(), hv.p.data());
+}
+
+// pp_disk_utility::store_event_data (pp_disk.cu:196-253) without the printf.
+rd_event make_event(int name, double t, double dist, uint32_t idx1, uint32_t idx2, const HostView& hv,
+                    const rd_body_md* bmd)
+{
+    rd_event e;
+    std::memset(&e, 0, sizeof(e));
+    e.event_name = name;
+    e.d = dist;
+    e.t = t;
+    e.id0 = bmd[0].id; e.id1 = bmd[idx1].id; e.id2 = bmd[idx2].id;
+    e.idx0 = 0; e.idx1 = idx1; e.idx2 = idx2;
+    e.r0 = hv.r[0]; e.v0 = hv.v[0];
+    e.r1 = hv.r[idx1]; e.v1 = hv.v[idx1];
+    e.r2 = hv.r[idx2]; e.v2 = hv.v[idx2];
+    e.p0 = hv.p[0];
+    if (RD_EVENT_EJECTION == name) {
+        e.p1 = hv.p[idx1];
+        e.p2 = hv.p[idx2];
+        e.rs = e.r1;
+        e.vs = e.v1;
+        e.ps = e.p1;
+    }
+    return e;
+}
+
+double calc_radius(double m, double density)   // tools.cpp:455-460
+{
+    const double four_pi_over_three = 4.188790204786391;
+    return std::pow(1.0 / four_pi_over_three * m / density, 1.0 / 3.0);
+}
+
+// handle_collision_pair (pp_disk.cu:1328-1368) with tools.cpp:480-507
+int merge_pair(HostView& hv, const rd_body_md* bmd, rd_event& c)
+{
+    const uint32_t s = c.idx1, m = c.idx2;
+    if (RD_BODY_SUPERPLANETESIMAL == bmd[m].body_type)
+        return fail(RD_ENOTIMPL, "Collision between a massive body and a super-planetesimal is not yet implemented.");
+    c.p1 = hv.p[s];
+    c.p2 = hv.p[m];
+    const double m1 = c.p1.mass, m2 = c.p2.mass, M = m1 + m2;
+    c.rs.x = (m1 * c.r1.x + m2 * c.r2.x) / M;
+    c.rs.y = (m1 * c.r1.y + m2 * c.r2.y) / M;
+    c.rs.z = (m1 * c.r1.z + m2 * c.r2.z) / M;
+    c.rs.w = 0.0;
+    c.vs.x = (m1 * c.v1.x + m2 * c.v2.x) / M;
+    c.vs.y = (m1 * c.v1.y + m2 * c.v2.y) / M;
+    c.vs.z = (m1 * c.v1.z + m2 * c.v2.z) / M;
+    c.vs.w = 0.0;
+    hv.r[s] = c.rs;
+    hv.v[s] = c.vs;
+    const double R1 = c.p1.radius, R2 = c.p2.radius;
+    const double volume = 4.188790204786391 * (R1 * R1 * R1 + R2 * R2 * R2);
+    c.ps.mass = c.p1.mass + c.p2.mass;
+    c.ps.density = c.ps.mass / volume;
+    c.ps.radius = calc_radius(c.ps.mass, c.ps.density);
+    c.ps.cd = c.p1.cd;
+    hv.p[s] = c.ps;
+    return RD_OK;
+}
+
+int push_back_disk(rd_disk* d, uint32_t k, const HostView& hv)
+{
+    const HostDisk& h = d->hd[k];
+    if (int rc = upload_bodies(d, h.off, h.n, hv.r.data(), hv.v.data(), hv.p.data(), d->bmd.data() + h.off))
+        return rc;
+    HIP_TRY(launch_set_sources(d->stream, d->n_all, d->mass, d->id, d->disk_of, d->info, d->m_ex, d->m32));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    return RD_OK;
+}
+
+void inc_counters(HostDisk& h)
+{
+    for (int q = 0; q < 3; ++q) h.n_event[q]++;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rd_check_ejection_hit_centrum(rd_disk* d, uint32_t k, uint32_t* n)
+{
+    if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    HostDisk& h = d->hd[k];
+    const double ej = d->thr[RD_THRESHOLD_EJECTION_DISTANCE], hc = d->thr[RD_THRESHOLD_HIT_CENTRUM_DISTANCE];
+    const double ed2 = (0.0 < ej ? ej * ej : DBL_MAX);
+    const double hcd2 = hc * hc;
+    HIP_TRY(hipMemsetAsync(d->ev_counter, 0, sizeof(uint32_t), d->stream));
+    HIP_TRY(launch_ejection(d->stream, h.off, h.off + h.n, d->x, d->y, d->z, d->id, d->type, ed2, hcd2, d->ev_buf,
+                            d->ev_counter, d->ev_cap));
+    uint32_t cnt = 0;
+    HIP_TRY(hipMemcpyAsync(&cnt, d->ev_counter, sizeof(uint32_t), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    if (cnt > d->ev_cap) return fail(RD_EOVERFLOW, "event buffer overflow");
+    if (cnt) {
+        std::vector<uint32_t> buf(2 * (size_t)cnt);
+        HIP_TRY(hipMemcpy(buf.data(), d->ev_buf, buf.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        std::vector<std::pair<uint32_t, uint32_t>> ev;
+        for (uint32_t q = 0; q < cnt; ++q) ev.push_back({buf[2 * q], buf[2 * q + 1]});
+        // canonical: by body index, ejection before hit-centrum (kernel order per body)
+        std::sort(ev.begin(), ev.end(), [](auto& a, auto& b) {
+            return a.first != b.first ? a.first < b.first : a.second > b.second;
+        });
+        HostView hv;
+        if (int rc = fetch(d, k, hv)) return rc;
+        for (auto& q : ev) {
+            const uint32_t li = q.first - h.off;
+            const double r2 = hv.r[li].x * hv.r[li].x + hv.r[li].y * hv.r[li].y + hv.r[li].z * hv.r[li].z;
+            h.pending.push_back(make_event((int)q.second, h.t, std::sqrt(r2), 0, li, hv, d->bmd.data() + h.off));
+        }
+    }
+    if (n) *n = (uint32_t)h.pending.size();
+    return RD_OK;
+}
+
+int rd_check_collision(rd_disk* d, uint32_t k, uint32_t* n)
+{
+    if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    if (int rc = ensure_lists(d)) return rc;
+    HostDisk& h = d->hd[k];
+    const double f = d->thr[RD_THRESHOLD_RADII_ENHANCE_FACTOR];
+    HIP_TRY(hipMemsetAsync(d->ev_counter, 0, sizeof(uint32_t), d->stream));
+    // only this disk's tiles: the collision list is grouped by disk
+    HIP_TRY(launch_collision(d->stream, d->items_coll, d->n_items_coll,
+                             std::min<uint32_t>(d->n_items_coll, 8u * (uint32_t)d->n_cu), d->x, d->y, d->z,
+                             d->radius, d->id, f * f, d->ev_buf, d->ev_counter, d->ev_cap));
+    uint32_t cnt = 0;
+    HIP_TRY(hipMemcpyAsync(&cnt, d->ev_counter, sizeof(uint32_t), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    if (cnt > d->ev_cap) return fail(RD_EOVERFLOW, "event buffer overflow: more than " + std::to_string(d->ev_cap) +
+                                                       " colliding pairs");
+    if (cnt) {
+        std::vector<uint32_t> buf(2 * (size_t)cnt);
+        HIP_TRY(hipMemcpy(buf.data(), d->ev_buf, buf.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        std::vector<std::pair<uint32_t, uint32_t>> pr;
+        for (uint32_t q = 0; q < cnt; ++q)
+            if (buf[2 * q] >= h.off && buf[2 * q] < h.off + h.n) pr.push_back({buf[2 * q], buf[2 * q + 1]});
+        std::sort(pr.begin(), pr.end());
+        if (!pr.empty()) {
+            HostView hv;
+            if (int rc = fetch(d, k, hv)) return rc;
+            for (auto& q : pr) {
+                uint32_t s = q.first - h.off, m = q.second - h.off;
+                const double dx = hv.r[m].x - hv.r[s].x, dy = hv.r[m].y - hv.r[s].y, dz = hv.r[m].z - hv.r[s].z;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                if (hv.p[m].mass > hv.p[s].mass) std::swap(s, m);
+                h.pending.push_back(make_event(RD_EVENT_COLLISION, h.t, std::sqrt(r2), s, m, hv,
+                                               d->bmd.data() + h.off));
+            }
+        }
+    }
+    if (n) *n = (uint32_t)h.pending.size();
+    return RD_OK;
+}
+
+int rd_get_events(rd_disk* d, uint32_t k, rd_event* out, uint32_t cap, uint32_t* n)
+{
+    if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    const auto& p = d->hd[k].pending;
+    if (out) std::memcpy(out, p.data(), std::min<size_t>(cap, p.size()) * sizeof(rd_event));
+    if (n) *n = (uint32_t)p.size();
+    return RD_OK;
+}
+
+int rd_event_log(rd_disk* d, uint32_t k, rd_event* out, uint32_t cap, uint32_t* n)
+{
+    if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    const auto& p = d->hd[k].log;
+    if (out) std::memcpy(out, p.data(), std::min<size_t>(cap, p.size()) * sizeof(rd_event));
+    if (n) *n = (uint32_t)p.size();
+    return RD_OK;
+}
+
+int rd_clear_event_counter(rd_disk* d, uint32_t k)
+{
+    if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    d->hd[k].pending.clear();
+    return RD_OK;
+}
+
+int rd_handle_collision(rd_disk* d, uint32_t k)
+{
+    if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    HostDisk& h = d->hd[k];
+    if (h.pending.empty()) return RD_OK;
+    HostView hv;
+    if (int rc = fetch(d, k, hv)) return rc;
+    rd_body_md* bmd = d->bmd.data() + h.off;
+    // populate_sp_events (pp_disk.cu:1250-1288)
+    std::vector<rd_event> sp;
+    std::vector<char> done(h.pending.size(), 0);
+    for (size_t q = 0; q < h.pending.size(); ++q) {
+        if (done[q]) continue;
+        done[q] = 1;
+        rd_event e = h.pending[q];
+        for (size_t u = q + 1; u < h.pending.size(); ++u)
+            if (e.id1 == h.pending[u].id1 && e.id2 == h.pending[u].id2) {
+                done[u] = 1;
+                if (e.t > h.pending[u].t) e = h.pending[u];
+            }
+        sp.push_back(e);
+    }
+    for (auto& e : sp) {
+        const uint32_t ictv = e.idx2;
+        if (int rc = merge_pair(hv, bmd, e)) return rc;
+        inc_counters(h);
+        bmd[ictv].id *= -1;
+        h.inactive[bmd[ictv].body_type]++;
+        h.log.push_back(e);
+    }
+    return push_back_disk(d, k, hv);
+}
+
+int rd_handle_ejection_hit_centrum(rd_disk* d, uint32_t k)
+{
+    if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    HostDisk& h = d->hd[k];
+    if (h.pending.empty()) return RD_OK;
+    HostView hv;
+    if (int rc = fetch(d, k, hv)) return rc;
+    rd_body_md* bmd = d->bmd.data() + h.off;
+    for (auto e : h.pending) {
+        const uint32_t ictv = e.idx2;
+        if (RD_EVENT_EJECTION != e.event_name)
+            if (int rc = merge_pair(hv, bmd, e)) return rc;
+        inc_counters(h);
+        bmd[ictv].id *= -1;
+        h.inactive[bmd[ictv].body_type]++;
+        h.log.push_back(e);
+    }
+    return push_back_disk(d, k, hv);
+}
+
+int rd_rebuild_vectors(rd_disk* d, uint32_t k)
+{
+    if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    // download everything, compact disk k, re-lay all disks
+    std::vector<HostView> all(d->n_disk);
+    for (uint32_t q = 0; q < d->n_disk; ++q)
+        if (int rc = fetch(d, q, all[q])) return rc;
+    std::vector<rd_body_md> nb;
+    HostDisk& hk = d->hd[k];
+    HostView cv;
+    for (uint32_t i = 0; i < hk.n; ++i) {
+        if (0 < d->bmd[hk.off + i].id) {
+            cv.r.push_back(all[k].r[i]);
+            cv.v.push_back(all[k].v[i]);
+            cv.p.push_back(all[k].p[i]);
+        }
+    }
+    for (int t = 0; t < kNType; ++t) {   // n_objects_t::update (red_type.h:545-556)
+        hk.playing[t] -= hk.inactive[t];
+        hk.removed[t] += hk.inactive[t];
+        hk.inactive[t] = 0;
+    }
+    uint32_t off = 0;
+    std::vector<rd_body_md> bm;
+    for (uint32_t q = 0; q < d->n_disk; ++q) {
+        HostDisk& h = d->hd[q];
+        for (uint32_t i = 0; i < h.n; ++i)
+            if (q != k || 0 < d->bmd[h.off + i].id) bm.push_back(d->bmd[h.off + i]);
+        if (q == k) {
+            all[q] = cv;
+            h.n = (uint32_t)cv.r.size();
+        }
+        h.off = off;
+        off += h.n;
+    }
+    uint32_t chk = 0;
+    for (int t = 0; t < kNType; ++t) chk += hk.playing[t];
+    if (chk != hk.n) return fail(RD_EINVAL, "Number of copied bodies does not equal to the number of active bodies.");
+    d->bmd = bm;
+    d->n_all = off;
+    free_buffers(d);
+    if (int rc = alloc_buffers(d)) return rc;
+    if (int rc = upload_disk_of(d)) return rc;
+    for (uint32_t q = 0; q < d->n_disk; ++q)
+        if (int rc = upload_bodies(d, d->hd[q].off, d->hd[q].n, all[q].r.data(), all[q].v.data(), all[q].p.data(),
+                                   d->bmd.data() + d->hd[q].off))
+            return rc;
+    d->lists_valid = false;
+    return build_lists(d);
+}
+
+int rd_calc_integral(rd_disk* d, uint32_t k, rd_integral* out)
+{
+    if (!d || !out || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    const HostDisk& h = d->hd[k];
+    const size_t nch = (h.n + 2047) / 2048;
+    const size_t need = std::max<size_t>(11 * (size_t)h.n, nch * h.n);
+    if (need > d->scratch_len) {
+        dev_free(d->scratch);
+        if (int rc = dev_alloc(&d->scratch, need)) return rc;
+        d->scratch_len = need;
+    }
+    HIP_TRY(launch_integral(d->stream, h.off, h.off + h.n, d->x, d->y, d->z, d->vx, d->vy, d->vz, d->mass, d->id,
+                            d->scratch, d->scratch_len, d->red_out, d->red_out + 11));
+    double o[12];
+    HIP_TRY(hipMemcpyAsync(o, d->red_out, sizeof(o), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    const double M0 = o[0];
+    out->R = rd_var4{o[1] / M0, o[2] / M0, o[3] / M0, 0.0};
+    out->V = rd_var4{o[4] / M0, o[5] / M0, o[6] / M0, 0.0};
+    out->C = rd_var4{o[7], o[8], o[9], 0.0};
+    out->E = o[10] / 2.0 - o[11] / 2.0;
+    return RD_OK;
+}
+
+// run_simulation's loop (src/red.cuda/main.cu:470-625) per disk, all disks in
+// lock step: each iteration every running disk takes one accepted step.
+int rd_run(rd_integrator* it, double length, double output_interval, uint64_t max_steps)
+{
+    if (!it) return fail(RD_EINVAL, "NULL integrator");
+    rd_disk* d = it->d;
+    const uint32_t nd = d->n_disk;
+    std::vector<double> elapsed(nd, 0.0), ps(nd, 0.0);
+    std::vector<uint64_t> steps(nd, 0);
+    for (;;) {
+        std::vector<char> want(nd, 0);
+        bool any = false;
+        for (uint32_t k = 0; k < nd; ++k) {
+            if (std::fabs(elapsed[k]) <= std::fabs(length) && 1 < d->hd[k].n_active() && steps[k] < max_steps) {
+                want[k] = 1;
+                any = true;
+            }
+        }
+        if (!any) break;
+        for (uint32_t k = 0; k < nd; ++k) {
+            if (!want[k]) continue;
+            if (0.0 < d->thr[RD_THRESHOLD_EJECTION_DISTANCE] || 0.0 < d->thr[RD_THRESHOLD_HIT_CENTRUM_DISTANCE]) {
+                uint32_t ne = 0;
+                if (int rc = rd_check_ejection_hit_centrum(d, k, &ne)) return rc;
+                if (ne) {
+                    if (int rc = rd_handle_ejection_hit_centrum(d, k)) return rc;
+                    rd_clear_event_counter(d, k);
+                }
+            }
+        }
+        if (int rc = step_disks(it, want)) return rc;
+        for (uint32_t k = 0; k < nd; ++k) {
+            if (!want[k]) continue;
+            const double dt = it->dt_did[k];
+            steps[k]++;
+            elapsed[k] += dt;
+            ps[k] += dt;
+            if (std::fabs(output_interval) < std::fabs(it->dt_next[k]))
+                it->dt_next[k] = 0.0 < dt ? output_interval : -output_interval;
+            if (0.0 < d->thr[RD_THRESHOLD_RADII_ENHANCE_FACTOR]) {
+                uint32_t ne = 0;
+                if (int rc = rd_check_collision(d, k, &ne)) return rc;
+                if (ne) {
+                    if (int rc = rd_handle_collision(d, k)) return rc;
+                    rd_clear_event_counter(d, k);
+                }
+            }
+            if (output_interval <= std::fabs(ps[k])) ps[k] = 0.0;
+            if (16 <= d->hd[k].n_event[1]) {
+                d->hd[k].n_event[1] = 0;
+                if (int rc = rd_rebuild_vectors(d, k)) return rc;
+            }
+        }
+    }
+    return RD_OK;
+}
+
+}  // extern "C"

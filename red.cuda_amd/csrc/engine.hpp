@@ -1,0 +1,114 @@
+// engine.hpp -- internal types shared by the kernels and the host engine.
+//
+// Device layout (HBM), all disks of a batch concatenated:
+//   state y_n      : x, y, z, vx, vy, vz          FP64 SoA (24 B per body per half)
+//   state y_{n+1}  : xo, yo, zo, vxo, vyo, vzo    FP64 SoA
+//   stage positions: P[2][3]                      FP64 SoA, double-buffered by stage parity
+//   stage k        : kv[r][3] (= velocity at stage r, the position derivative)
+//                    ka[r][3] (= acceleration at stage r)            r < 13
+//   pair partials  : part[(c*3 + comp) * n_all + i]                  FP64
+//   sources        : m_ex (FP64 mass, -1 if inactive), m32 (FP32 mass, 0 if
+//                    inactive or summed in FP64)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rdamd {
+
+constexpr int kMaxStage = 13;
+constexpr int kBlock = 256;          // threads per block for every kernel
+constexpr int kTile = 256;           // sources staged in LDS per tile
+
+// Per-disk control flags (DiskCtl::flags).
+enum : uint32_t {
+    F_ACTIVE = 1u,   // disk takes part in this attempt
+    F_NEED_K0 = 2u,  // k0 = f(t, y_n) must be (re)computed
+    F_COMMIT = 4u,   // copy y_{n+1} -> y_n
+};
+
+// Per-disk static layout.  Indices are local to the disk except off.
+struct DiskInfo {
+    uint32_t off, n;               // global offset, playing body count
+    uint32_t n_si, n_massive;      // class bounds (red_type.h:670-700)
+    uint32_t n_fp64;               // leading sources summed in FP64 (mixed mode)
+    uint32_t drag_lo, drag_hi;     // bodies that feel the gas
+    uint32_t err_hi;               // bodies [0, err_hi) enter the error norm
+    uint32_t nchunk[3];            // partial-sum slots per class (SI, NSI, NI)
+    uint32_t pad;
+};
+
+// Per-disk dynamic control, written by the host before each attempt.
+struct DiskCtl {
+    double t;       // time at the start of the step
+    double h;       // dt_try
+    uint32_t flags;
+    uint32_t pad;
+};
+
+// One block of a pair kernel.  Global body indices.
+struct WorkItem {
+    uint32_t disk;
+    uint32_t sink_lo, sink_hi;
+    uint32_t src_lo, src_hi;
+    uint32_t slot;
+    uint32_t pad0, pad1;
+};
+
+// Analytic gas disk (analytic_gas_disk_params_t red_type.h:464-485), device copy.
+struct GasDev {
+    double rho0, rho1, sch0, sch1, eta0, eta1;
+    double t0, t1, e_folding_time;
+    int32_t gas_decrease;
+    int32_t on;
+};
+
+struct StageArgs {
+    const double *x, *y, *z, *vx, *vy, *vz;       // y_n
+    double *xo, *yo, *zo, *vxo, *vyo, *vzo;       // y_{n+1}
+    const double *fx, *fy, *fz;                   // force-input positions of this stage
+    double *px, *py, *pz;                         // next-stage positions (the other P buffer)
+    double* kv[kMaxStage][3];                     // kv[0] = state velocity
+    double* ka[kMaxStage][3];
+    const double* part;
+    uint32_t n_all;
+    uint32_t err_all;                             // error_check_for_tp
+    const uint32_t* disk_of;
+    const DiskInfo* info;
+    const DiskCtl* ctl;
+    unsigned long long* err;                      // per disk, bits of the raw max |E|
+    const double *mass, *radius, *density, *cd;
+    const int32_t* id;
+    GasDev gas;
+    int mixed;
+    int adaptive;
+};
+
+// Launchers (kernels.hip).
+hipError_t launch_force_exact(hipStream_t s, const WorkItem* items, uint32_t n_items, uint32_t grid,
+                              const double* x, const double* y, const double* z, const double* m_ex,
+                              const DiskCtl* ctl, uint32_t need_mask, double* part, uint32_t n_all);
+hipError_t launch_force_mixed(hipStream_t s, int sinks_per_lane, const WorkItem* items, uint32_t n_items,
+                              uint32_t grid, const double* x, const double* y, const double* z,
+                              const float* m32, const DiskCtl* ctl, uint32_t need_mask, double* part,
+                              uint32_t n_all);
+hipError_t launch_stage(hipStream_t s, int method, int r, const StageArgs& a);
+hipError_t launch_comb1(hipStream_t s, int method, const StageArgs& a);
+hipError_t launch_commit(hipStream_t s, const StageArgs& a, int advance_time, DiskCtl* ctl_rw);
+hipError_t launch_operator(hipStream_t s, const StageArgs& a, double t, uint32_t disk, double* ax, double* ay,
+                           double* az, const double* vx, const double* vy, const double* vz);
+hipError_t launch_collision(hipStream_t s, const WorkItem* items, uint32_t n_items, uint32_t grid,
+                            const double* x, const double* y, const double* z, const double* radius,
+                            const int32_t* id, double f2, uint32_t* pairs, uint32_t* counter, uint32_t cap);
+hipError_t launch_ejection(hipStream_t s, uint32_t lo, uint32_t hi, const double* x, const double* y,
+                           const double* z, const int32_t* id, const int32_t* type, double ed2, double hcd2,
+                           uint32_t* out, uint32_t* counter, uint32_t cap);
+hipError_t launch_integral(hipStream_t s, uint32_t lo, uint32_t hi, const double* x, const double* y,
+                           const double* z, const double* vx, const double* vy, const double* vz,
+                           const double* mass, const int32_t* id, double* scratch, size_t scratch_len,
+                           double* out /* 11 doubles: M, Rx,Ry,Rz, Vx,Vy,Vz, Cx,Cy,Cz, T */,
+                           double* out_u);
+hipError_t launch_set_sources(hipStream_t s, uint32_t n_all, const double* mass, const int32_t* id,
+                              const uint32_t* disk_of, const DiskInfo* info, double* m_ex, float* m32);
+
+}  // namespace rdamd

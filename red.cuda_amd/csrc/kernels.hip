@@ -1,0 +1,882 @@
+// kernels.hip -- HIP kernels of the direct-sum engine for gfx950 (CDNA4).
+//
+// Compiled with -ffp-contract=off: every FP64 expression that restates a
+// reference formula keeps the reference's rounding (the reference builds with
+// -fmad=false, Makefile:17).  Fused multiply-adds appear only where written
+// explicitly (fmaf in the mixed-precision pair loop).
+#include "engine.hpp"
+
+#include <float.h>
+
+namespace rdamd {
+
+#define SQR(x) ((x) * (x))
+
+// ---------------------------------------------------------------------------
+// Runge-Kutta tableaux
+// ---------------------------------------------------------------------------
+struct RowDef {
+    int n;
+    int col[9];
+    double a[9];
+};
+
+// Fehlberg 7(8), int_rungekutta8.cu:30-48; nonzero columns as used by
+// calc_ytemp_for_f1..f12 (int_rungekutta8.cu:82-246, 414-585).
+__host__ __device__ constexpr RowDef rk8_row(int r)
+{
+    switch (r) {
+    case 1: return {1, {0}, {2.0 / 27.0}};
+    case 2: return {2, {0, 1}, {1.0 / 36.0, 1.0 / 12.0}};
+    case 3: return {2, {0, 2}, {1.0 / 24.0, 1.0 / 8.0}};
+    case 4: return {3, {0, 2, 3}, {5.0 / 12.0, -25.0 / 16.0, 25.0 / 16.0}};
+    case 5: return {3, {0, 3, 4}, {1.0 / 20.0, 1.0 / 4.0, 1.0 / 5.0}};
+    case 6: return {4, {0, 3, 4, 5}, {-25.0 / 108.0, 125.0 / 108.0, -65.0 / 27.0, 125.0 / 54.0}};
+    case 7: return {4, {0, 4, 5, 6}, {31.0 / 300.0, 61.0 / 225.0, -2.0 / 9.0, 13.0 / 900.0}};
+    case 8: return {6, {0, 3, 4, 5, 6, 7}, {2.0, -53.0 / 6.0, 704.0 / 45.0, -107.0 / 9.0, 67.0 / 90.0, 3.0}};
+    case 9:
+        return {7, {0, 3, 4, 5, 6, 7, 8},
+                {-91.0 / 108.0, 23.0 / 108.0, -976.0 / 135.0, 311.0 / 54.0, -19.0 / 60.0, 17.0 / 6.0, -1.0 / 12.0}};
+    case 10:
+        return {8, {0, 3, 4, 5, 6, 7, 8, 9},
+                {2383.0 / 4100.0, -341.0 / 164.0, 4496.0 / 1025.0, -301.0 / 82.0, 2133.0 / 4100.0, 45.0 / 82.0,
+                 45.0 / 164.0, 18.0 / 41.0}};
+    case 11:
+        return {6, {0, 5, 6, 7, 8, 9},
+                {3.0 / 205.0, -6.0 / 41.0, -3.0 / 205.0, -3.0 / 41.0, 3.0 / 41.0, 6.0 / 41.0}};
+    case 12:
+        return {9, {0, 3, 4, 5, 6, 7, 8, 9, 11},
+                {-1777.0 / 4100.0, -341.0 / 164.0, 4496.0 / 1025.0, -289.0 / 82.0, 2193.0 / 4100.0, 51.0 / 82.0,
+                 33.0 / 164.0, 12.0 / 41.0, 1.0}};
+    default: return {0, {0}, {0.0}};
+    }
+}
+__host__ __device__ constexpr double rk8_c(int r)
+{
+    switch (r) {
+    case 1: return 2.0 / 27.0;  case 2: return 1.0 / 9.0;  case 3: return 1.0 / 6.0;
+    case 4: return 5.0 / 12.0;  case 5: return 1.0 / 2.0;  case 6: return 5.0 / 6.0;
+    case 7: return 1.0 / 6.0;   case 8: return 2.0 / 3.0;  case 9: return 1.0 / 3.0;
+    case 10: return 1.0;        case 11: return 0.0;       case 12: return 1.0;
+    default: return 0.0;
+    }
+}
+// Classical RK4, int_rungekutta4.cu:26-31
+__host__ __device__ constexpr double rk4_a(int r) { return r == 3 ? 1.0 : (r == 0 ? 0.0 : 1.0 / 2.0); }
+__host__ __device__ constexpr double rk4_c(int r) { return (r == 1 || r == 2) ? 1.0 / 2.0 : (r == 0 ? 0.0 : 1.0); }
+
+// ---------------------------------------------------------------------------
+// Pair kernels
+// ---------------------------------------------------------------------------
+// Reference formula (pp_disk.cu:374-416): sources j ascending, skip i == j
+// and inactive j (m_ex < 0), a += (m_j / (d * d^2)) * d_vec.  One sink per
+// lane; the j tile is staged in LDS and read by all four waves.
+__global__ __launch_bounds__(kBlock) void k_force_exact(const WorkItem* __restrict__ items, uint32_t n_items,
+                                                        const double* __restrict__ x, const double* __restrict__ y,
+                                                        const double* __restrict__ z,
+                                                        const double* __restrict__ m_ex,
+                                                        const DiskCtl* __restrict__ ctl, uint32_t need,
+                                                        double* __restrict__ part, uint32_t n_all)
+{
+    __shared__ double4 tile[kTile];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const WorkItem w = items[it];
+        if (!(ctl[w.disk].flags & need)) continue;
+        const uint32_t i = w.sink_lo + tid;
+        const bool valid = i < w.sink_hi;
+        const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0, zi = valid ? z[i] : 0.0;
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        for (uint32_t t0 = w.src_lo; t0 < w.src_hi; t0 += kTile) {
+            __syncthreads();
+            const uint32_t j = t0 + tid;
+            if (j < w.src_hi) tile[tid] = make_double4(x[j], y[j], z[j], m_ex[j]);
+            __syncthreads();
+            const uint32_t nt = min((uint32_t)kTile, w.src_hi - t0);
+            for (uint32_t k = 0; k < nt; ++k) {
+                const double4 s = tile[k];
+                if (s.w < 0.0 || t0 + k == i) continue;
+                const double dx = s.x - xi;
+                const double dy = s.y - yi;
+                const double dz = s.z - zi;
+                const double r2 = SQR(dx) + SQR(dy) + SQR(dz);
+                const double d = sqrt(r2);
+                const double sc = s.w / (d * r2);
+                ax += sc * dx;
+                ay += sc * dy;
+                az += sc * dz;
+            }
+        }
+        if (valid) {
+            part[(size_t)(w.slot * 3 + 0) * n_all + i] = ax;
+            part[(size_t)(w.slot * 3 + 1) * n_all + i] = ay;
+            part[(size_t)(w.slot * 3 + 2) * n_all + i] = az;
+        }
+        __syncthreads();
+    }
+}
+
+struct alignas(16) SrcF {
+    double x, y, z;
+    float m, pad;
+};
+
+// Softening floor that keeps the self pair (d = 0) and zero-mass padding
+// finite: r^-3 at r^2 = 1e-24 AU^2 is 1e36, inside FP32 range, and the term is
+// multiplied by d_vec = 0.  Real separations are > 1e-8 AU (two 1 km bodies
+// touching), so the floor changes r^2 by < 1e-8 ulp.
+constexpr float kEps2 = 1.0e-24f;
+
+// Mixed precision pair kernel.  A work item is 64*S sinks (S per lane) against
+// a chunk of sources; the four waves of the block split each 256-source LDS
+// tile in quarters.  Per pair: FP64 offset rounded to FP32, r^2 by FMA, one
+// v_rsq_f32, FP32 partial sums per 64-source quarter tile folded into FP64.
+// The quarter sums of the four waves are combined in fixed order, so the
+// result is deterministic.
+template <int S>
+__global__ __launch_bounds__(kBlock) void k_force_mixed(const WorkItem* __restrict__ items, uint32_t n_items,
+                                                        const double* __restrict__ x, const double* __restrict__ y,
+                                                        const double* __restrict__ z,
+                                                        const float* __restrict__ m32,
+                                                        const DiskCtl* __restrict__ ctl, uint32_t need,
+                                                        double* __restrict__ part, uint32_t n_all)
+{
+    __shared__ SrcF tile[kTile];
+    __shared__ double red[3][S][3][64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const WorkItem w = items[it];
+        if (!(ctl[w.disk].flags & need)) continue;
+        double xi[S], yi[S], zi[S], ax[S], ay[S], az[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const uint32_t i = w.sink_lo + lane + 64u * s;
+            const bool valid = i < w.sink_hi;
+            xi[s] = valid ? x[i] : 0.0;
+            yi[s] = valid ? y[i] : 0.0;
+            zi[s] = valid ? z[i] : 0.0;
+            ax[s] = ay[s] = az[s] = 0.0;
+        }
+        for (uint32_t t0 = w.src_lo; t0 < w.src_hi; t0 += kTile) {
+            __syncthreads();
+            {
+                const uint32_t j = t0 + tid;
+                SrcF r;
+                if (j < w.src_hi) {
+                    r.x = x[j]; r.y = y[j]; r.z = z[j]; r.m = m32[j];
+                } else {
+                    r.x = 0.0; r.y = 0.0; r.z = 0.0; r.m = 0.0f;
+                }
+                r.pad = 0.0f;
+                tile[tid] = r;
+            }
+            __syncthreads();
+            const uint32_t nt = min((uint32_t)kTile, w.src_hi - t0);
+            const uint32_t k0 = wave * 64u;
+            const uint32_t k1 = min(k0 + 64u, (nt + 3u) & ~3u);
+            float fx[S], fy[S], fz[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) fx[s] = fy[s] = fz[s] = 0.0f;
+            for (uint32_t k = k0; k < k1; k += 4) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const SrcF r = tile[k + u];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const float dx = (float)(r.x - xi[s]);
+                        const float dy = (float)(r.y - yi[s]);
+                        const float dz = (float)(r.z - zi[s]);
+                        float r2 = __builtin_fmaf(dz, dz, kEps2);
+                        r2 = __builtin_fmaf(dy, dy, r2);
+                        r2 = __builtin_fmaf(dx, dx, r2);
+                        const float ri = __builtin_amdgcn_rsqf(r2);
+                        const float ri2 = ri * ri;
+                        const float sc = (r.m * ri) * ri2;
+                        fx[s] = __builtin_fmaf(sc, dx, fx[s]);
+                        fy[s] = __builtin_fmaf(sc, dy, fy[s]);
+                        fz[s] = __builtin_fmaf(sc, dz, fz[s]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                ax[s] += (double)fx[s];
+                ay[s] += (double)fy[s];
+                az[s] += (double)fz[s];
+            }
+        }
+        if (wave > 0) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                red[wave - 1][s][0][lane] = ax[s];
+                red[wave - 1][s][1][lane] = ay[s];
+                red[wave - 1][s][2][lane] = az[s];
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const uint32_t i = w.sink_lo + lane + 64u * s;
+                if (i < w.sink_hi) {
+                    const double sx = ((ax[s] + red[0][s][0][lane]) + red[1][s][0][lane]) + red[2][s][0][lane];
+                    const double sy = ((ay[s] + red[0][s][1][lane]) + red[1][s][1][lane]) + red[2][s][1][lane];
+                    const double sz = ((az[s] + red[0][s][2][lane]) + red[1][s][2][lane]) + red[2][s][2][lane];
+                    part[(size_t)(w.slot * 3 + 0) * n_all + i] = sx;
+                    part[(size_t)(w.slot * 3 + 1) * n_all + i] = sy;
+                    part[(size_t)(w.slot * 3 + 2) * n_all + i] = sz;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_force_exact(hipStream_t st, const WorkItem* items, uint32_t n_items, uint32_t grid,
+                              const double* x, const double* y, const double* z, const double* m_ex,
+                              const DiskCtl* ctl, uint32_t need, double* part, uint32_t n_all)
+{
+    if (n_items == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_force_exact, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, m_ex, ctl, need,
+                       part, n_all);
+    return hipGetLastError();
+}
+
+hipError_t launch_force_mixed(hipStream_t st, int S, const WorkItem* items, uint32_t n_items, uint32_t grid,
+                              const double* x, const double* y, const double* z, const float* m32,
+                              const DiskCtl* ctl, uint32_t need, double* part, uint32_t n_all)
+{
+    if (n_items == 0) return hipSuccess;
+    switch (S) {
+    case 1: hipLaunchKernelGGL(k_force_mixed<1>, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, m32, ctl, need, part, n_all); break;
+    case 2: hipLaunchKernelGGL(k_force_mixed<2>, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, m32, ctl, need, part, n_all); break;
+    case 4: hipLaunchKernelGGL(k_force_mixed<4>, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, m32, ctl, need, part, n_all); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Per-body pieces of the stage kernel
+// ---------------------------------------------------------------------------
+// pp_disk.cu:60-84
+__device__ double reduction_factor(const GasDev& g, double t)
+{
+    switch (g.gas_decrease) {
+    case 0: return 1.0;
+    case 1:
+        if (t <= g.t0) return 1.0;
+        else if (g.t0 < t && t <= g.t1 && g.t0 != g.t1) return 1.0 - (t - g.t0) / (g.t1 - g.t0);
+        else return 0.0;
+    case 2: return exp(-(t - g.t0) / g.e_folding_time);
+    default: return 1.0;
+    }
+}
+
+// Quadratic drag a += f u with the analytic gas disk (pp_disk.cu:87-155, 486-521).
+__device__ void drag_accel(const GasDev& g, double decr, double m_star, double rx, double ry, double rz, double vx,
+                           double vy, double vz, double cd, double radius, double density, double& ax, double& ay,
+                           double& az)
+{
+    // circular_velocity (pp_disk.cu:108-142)
+    double gx = 0.0, gy = 0.0;
+    const double r = sqrt(SQR(rx) + SQR(ry));
+    const double vc = sqrt(m_star / r);
+    if (rx == 0.0 && ry == 0.0) {
+    } else if (ry == 0.0) {
+        gy = rx > 0.0 ? vc : -vc;
+    } else if (rx == 0.0) {
+        gx = ry > 0.0 ? -vc : vc;
+    } else if (rx >= ry) {
+        const double p = ry / rx;
+        gy = rx >= 0 ? vc / sqrt(1.0 + SQR(p)) : -vc / sqrt(1.0 + SQR(p));
+        gx = -gy * p;
+    } else {
+        const double p = rx / ry;
+        gx = ry >= 0 ? -vc / sqrt(1.0 + SQR(p)) : vc / sqrt(1.0 + SQR(p));
+        gy = -gx * p;
+    }
+    // get_velocity (pp_disk.cu:145-155)
+    const double fac = sqrt(1.0 - 2.0 * g.eta0 * pow(r, g.eta1));
+    gx *= fac;
+    gy *= fac;
+    const double ux = gx - vx, uy = gy - vy, uz = 0.0 - vz;
+    const double u_n = sqrt(SQR(ux) + SQR(uy) + SQR(uz));
+    // get_density (pp_disk.cu:87-105)
+    const double h = g.sch0 * pow(r, g.sch1);
+    const double arg = SQR(rz / h);
+    double dens;
+    if (0.1 < r) {
+        dens = g.rho0 * pow(r, g.rho1) * exp(-arg);
+    } else {
+        const double a = g.rho0 * pow(0.1, g.rho1 - 4.0);
+        dens = a * SQR(SQR(r)) * exp(-arg);
+    }
+    const double f = decr * (3.0 * cd * dens * u_n) / (8.0 * radius * density);
+    ax += f * ux;
+    ay += f * uy;
+    az += f * uz;
+}
+
+// Acceleration of body i at this stage: pair partials (+ FP64 prefix in mixed
+// mode), zero for inactive sinks, plus drag.
+__device__ void stage_accel(const StageArgs& a, uint32_t i, const DiskInfo& di, double t, const double* vx,
+                            const double* vy, const double* vz, double& ax, double& ay, double& az)
+{
+    const uint32_t li = i - di.off;
+    const int cls = li < di.n_si ? 0 : (li < di.n_massive ? 1 : 2);
+    const uint32_t nc = di.nchunk[cls];
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    if (nc > 0) {
+        sx = a.part[(size_t)0 * a.n_all + i];
+        sy = a.part[(size_t)1 * a.n_all + i];
+        sz = a.part[(size_t)2 * a.n_all + i];
+        for (uint32_t c = 1; c < nc; ++c) {
+            sx += a.part[(size_t)(c * 3 + 0) * a.n_all + i];
+            sy += a.part[(size_t)(c * 3 + 1) * a.n_all + i];
+            sz += a.part[(size_t)(c * 3 + 2) * a.n_all + i];
+        }
+    }
+    if (a.mixed) {
+        // FP64 prefix sources (the star block), reference formula and order.
+        const double xi = a.fx[i], yi = a.fy[i], zi = a.fz[i];
+        const uint32_t src_hi = cls == 1 ? di.n_si : di.n_massive;
+        const uint32_t hi = min(di.n_fp64, src_hi);
+        double bx = 0.0, by = 0.0, bz = 0.0;
+        for (uint32_t lj = 0; lj < hi; ++lj) {
+            const uint32_t j = di.off + lj;
+            if (j == i || a.id[j] < 0) continue;
+            const double dx = a.fx[j] - xi;
+            const double dy = a.fy[j] - yi;
+            const double dz = a.fz[j] - zi;
+            const double r2 = SQR(dx) + SQR(dy) + SQR(dz);
+            const double d = sqrt(r2);
+            const double sc = a.mass[j] / (d * r2);
+            bx += sc * dx;
+            by += sc * dy;
+            bz += sc * dz;
+        }
+        sx = bx + sx;
+        sy = by + sy;
+        sz = bz + sz;
+    }
+    if (!(a.id[i] > 0)) sx = sy = sz = 0.0;
+    if (a.gas.on && li >= di.drag_lo && li < di.drag_hi) {
+        const double decr = reduction_factor(a.gas, t);
+        if (!(1.0e-6 > decr)) {
+            drag_accel(a.gas, decr, a.mass[di.off], a.fx[i], a.fy[i], a.fz[i], vx[i], vy[i], vz[i], a.cd[i],
+                       a.radius[i], a.density[i], sx, sy, sz);
+        }
+    }
+    ax = sx;
+    ay = sy;
+    az = sz;
+}
+
+// ytemp = y + h * (sum_q a_q k_q) for the next RKF78 stage R (int_rungekutta8.cu:82-246)
+template <int R>
+__device__ void rk8_comb(const StageArgs& a, uint32_t i, double h, int pbuf_unused)
+{
+    constexpr RowDef row = rk8_row(R);
+    const double* yv[6] = {a.x, a.y, a.z, a.vx, a.vy, a.vz};
+    double* outp[6] = {a.px, a.py, a.pz, a.kv[R][0], a.kv[R][1], a.kv[R][2]};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        double* const* K = c < 3 ? a.kv[row.col[0]] : a.ka[row.col[0]];
+        double s = row.a[0] * K[c % 3][i];
+#pragma unroll
+        for (int q = 1; q < row.n; ++q) {
+            double* const* Kq = c < 3 ? a.kv[row.col[q]] : a.ka[row.col[q]];
+            s = s + row.a[q] * Kq[c % 3][i];
+        }
+        outp[c][i] = yv[c][i] + h * s;
+    }
+}
+
+// y_{n+1} = y + h (b0 k0 + b5 k5 + b6 (k6 + k7) + b8 (k8 + k9) + b10 k10) (int_rungekutta8.cu:264-274)
+__device__ void rk8_ynp1(const StageArgs& a, uint32_t i, double h)
+{
+    constexpr double b0 = 41.0 / 840.0, b5 = 34.0 / 105.0, b6 = 9.0 / 35.0, b8 = 9.0 / 280.0, b10 = 41.0 / 840.0;
+    const double* yv[6] = {a.x, a.y, a.z, a.vx, a.vy, a.vz};
+    double* yo[6] = {a.xo, a.yo, a.zo, a.vxo, a.vyo, a.vzo};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        double* const* k0 = c < 3 ? a.kv[0] : a.ka[0];
+        double* const* k5 = c < 3 ? a.kv[5] : a.ka[5];
+        double* const* k6 = c < 3 ? a.kv[6] : a.ka[6];
+        double* const* k7 = c < 3 ? a.kv[7] : a.ka[7];
+        double* const* k8 = c < 3 ? a.kv[8] : a.ka[8];
+        double* const* k9 = c < 3 ? a.kv[9] : a.ka[9];
+        double* const* k10 = c < 3 ? a.kv[10] : a.ka[10];
+        const int q = c % 3;
+        yo[c][i] = yv[c][i] + h * (b0 * k0[q][i] + b5 * k5[q][i] + b6 * (k6[q][i] + k7[q][i]) +
+                                   b8 * (k8[q][i] + k9[q][i]) + b10 * k10[q][i]);
+    }
+}
+
+// Wave-level max into the per-disk error slot; waves whose lanes span two
+// disks fall back to one atomic per lane.
+__device__ void error_reduce(const StageArgs& a, bool valid, uint32_t d, double e)
+{
+    unsigned long long bits = valid ? (unsigned long long)__double_as_longlong(e) : 0ull;
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(valid ? d : 0xffffffffu);
+    const bool same = __all(!valid || d == d0);
+    if (same) {
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(bits, off, 64);
+            bits = o > bits ? o : bits;
+        }
+        if ((threadIdx.x & 63u) == 0 && d0 != 0xffffffffu && bits) atomicMax(&a.err[d0], bits);
+    } else if (valid && bits) {
+        atomicMax(&a.err[d], bits);
+    }
+}
+
+// RKF7(8) stage kernel for stage R: finishes k_R (partials, FP64 prefix,
+// drag) and forms what the next stage needs from body i's own k values.
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_stage_rk8(StageArgs a)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const bool in = i < a.n_all;
+    uint32_t d = 0;
+    bool valid = false;
+    double e = 0.0;
+    if (in) {
+        d = a.disk_of[i];
+        const DiskCtl c = a.ctl[d];
+        const uint32_t need = R == 0 ? F_NEED_K0 : F_ACTIVE;
+        valid = (c.flags & need) != 0;
+        if (valid) {
+            const DiskInfo di = a.info[d];
+            const double h = c.h;
+            const double t = c.t + rk8_c(R) * h;
+            double ax, ay, az;
+            stage_accel(a, i, di, t, a.kv[R][0], a.kv[R][1], a.kv[R][2], ax, ay, az);
+            a.ka[R][0][i] = ax;
+            a.ka[R][1][i] = ay;
+            a.ka[R][2][i] = az;
+            if constexpr (R >= 1 && R <= 9) {
+                rk8_comb<R + 1>(a, i, h, 0);
+            } else if constexpr (R == 10) {
+                rk8_ynp1(a, i, h);
+                if (a.adaptive) rk8_comb<11>(a, i, h, 0);
+            } else if constexpr (R == 11) {
+                rk8_comb<12>(a, i, h, 0);
+            } else if constexpr (R == 12) {
+                const uint32_t li = i - di.off;
+                if (a.err_all || li < di.err_hi) {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        const double ev = fabs(a.kv[0][q][i] + a.kv[10][q][i] - a.kv[11][q][i] - a.kv[12][q][i]);
+                        const double ea = fabs(a.ka[0][q][i] + a.ka[10][q][i] - a.ka[11][q][i] - a.ka[12][q][i]);
+                        e = fmax(e, fmax(ev, ea));
+                    }
+                }
+            }
+        }
+    }
+    if constexpr (R == 12) error_reduce(a, valid, d, e);
+}
+
+// Classical RK4 stage kernel (int_rungekutta4.cu:37-73, 130-247).
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_stage_rk4(StageArgs a)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const bool in = i < a.n_all;
+    uint32_t d = 0;
+    bool valid = false;
+    double e = 0.0;
+    if (in) {
+        d = a.disk_of[i];
+        const DiskCtl c = a.ctl[d];
+        const uint32_t need = R == 0 ? F_NEED_K0 : F_ACTIVE;
+        valid = (c.flags & need) != 0;
+        if (valid) {
+            const DiskInfo di = a.info[d];
+            const double h = c.h;
+            const double t = c.t + rk4_c(R) * h;
+            double ax, ay, az;
+            stage_accel(a, i, di, t, a.kv[R][0], a.kv[R][1], a.kv[R][2], ax, ay, az);
+            a.ka[R][0][i] = ax;
+            a.ka[R][1][i] = ay;
+            a.ka[R][2][i] = az;
+            const double* yv[6] = {a.x, a.y, a.z, a.vx, a.vy, a.vz};
+            if constexpr (R == 1 || R == 2) {
+                const double fac = rk4_a(R + 1) * h;
+                double* outp[6] = {a.px, a.py, a.pz, a.kv[R + 1][0], a.kv[R + 1][1], a.kv[R + 1][2]};
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    const double k = q < 3 ? a.kv[R][q][i] : a.ka[R][q - 3][i];
+                    outp[q][i] = yv[q][i] + fac * k;
+                }
+            } else if constexpr (R == 3) {
+                const double b0 = (1.0 / 6.0) * h, b1 = (1.0 / 3.0) * h;
+                double* yo[6] = {a.xo, a.yo, a.zo, a.vxo, a.vyo, a.vzo};
+                double* p4[6] = {a.px, a.py, a.pz, a.kv[4][0], a.kv[4][1], a.kv[4][2]};
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    double* const* K = q < 3 ? a.kv[0] : a.ka[0];
+                    double* const* K1 = q < 3 ? a.kv[1] : a.ka[1];
+                    double* const* K2 = q < 3 ? a.kv[2] : a.ka[2];
+                    double* const* K3 = q < 3 ? a.kv[3] : a.ka[3];
+                    const int c3 = q % 3;
+                    const double v = yv[q][i] + b0 * (K[c3][i] + K3[c3][i]) + b1 * (K1[c3][i] + K2[c3][i]);
+                    yo[q][i] = v;
+                    if (a.adaptive) p4[q][i] = v;
+                }
+            } else if constexpr (R == 4) {
+                const uint32_t li = i - di.off;
+                if (a.err_all || li < di.err_hi) {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        e = fmax(e, fmax(fabs(a.kv[3][q][i] - a.kv[4][q][i]), fabs(a.ka[3][q][i] - a.ka[4][q][i])));
+                    }
+                }
+            }
+        }
+    }
+    if constexpr (R == 4) error_reduce(a, valid, d, e);
+}
+
+// First combination of an attempt (stage 1 from k0), and clears the error
+// slot of each active disk.
+template <int M>
+__global__ __launch_bounds__(kBlock) void k_comb1(StageArgs a)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n_all) return;
+    const uint32_t d = a.disk_of[i];
+    const DiskCtl c = a.ctl[d];
+    if (!(c.flags & F_ACTIVE)) return;
+    if (i == a.info[d].off) a.err[d] = 0ull;
+    if constexpr (M == 4) {
+        rk8_comb<1>(a, i, c.h, 0);
+    } else {
+        const double fac = rk4_a(1) * c.h;
+        const double* yv[6] = {a.x, a.y, a.z, a.vx, a.vy, a.vz};
+        double* outp[6] = {a.px, a.py, a.pz, a.kv[1][0], a.kv[1][1], a.kv[1][2]};
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const double k = q < 3 ? a.kv[0][q][i] : a.ka[0][q - 3][i];
+            outp[q][i] = yv[q][i] + fac * k;
+        }
+    }
+}
+
+// y_n <- y_{n+1} for disks flagged F_COMMIT; optionally t += h on the device
+// (fixed-step runs that never return to the host between steps).
+__global__ __launch_bounds__(kBlock) void k_commit(StageArgs a, int advance, DiskCtl* ctl)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n_all) return;
+    const uint32_t d = a.disk_of[i];
+    const uint32_t fl = ctl[d].flags;
+    if (!(fl & F_COMMIT)) return;
+    const_cast<double*>(a.x)[i] = a.xo[i];
+    const_cast<double*>(a.y)[i] = a.yo[i];
+    const_cast<double*>(a.z)[i] = a.zo[i];
+    const_cast<double*>(a.vx)[i] = a.vxo[i];
+    const_cast<double*>(a.vy)[i] = a.vyo[i];
+    const_cast<double*>(a.vz)[i] = a.vzo[i];
+    if (advance && i == a.info[d].off) ctl[d].t = ctl[d].t + ctl[d].h;
+}
+
+// pp_disk::calc_dydx(1, ...) for one disk: acceleration at time t from the
+// partials, written to (ax, ay, az).
+__global__ __launch_bounds__(kBlock) void k_operator(StageArgs a, double t, uint32_t disk, double* ax, double* ay,
+                                                     double* az, const double* vx, const double* vy,
+                                                     const double* vz)
+{
+    const DiskInfo di = a.info[disk];
+    const uint32_t li = blockIdx.x * kBlock + threadIdx.x;
+    if (li >= di.n) return;
+    const uint32_t i = di.off + li;
+    double x, y, z;
+    stage_accel(a, i, di, t, vx, vy, vz, x, y, z);
+    ax[i] = x;
+    ay[i] = y;
+    az[i] = z;
+}
+
+#define STAGE_CASE(M, R)                                                                   \
+    case R:                                                                                \
+        hipLaunchKernelGGL(k_stage_##M<R>, dim3(grid), dim3(kBlock), 0, st, a);            \
+        break;
+
+hipError_t launch_stage(hipStream_t st, int method, int r, const StageArgs& a)
+{
+    const uint32_t grid = (a.n_all + kBlock - 1) / kBlock;
+    if (grid == 0) return hipSuccess;
+    if (method == 4) {
+        switch (r) {
+            STAGE_CASE(rk8, 0) STAGE_CASE(rk8, 1) STAGE_CASE(rk8, 2) STAGE_CASE(rk8, 3) STAGE_CASE(rk8, 4)
+            STAGE_CASE(rk8, 5) STAGE_CASE(rk8, 6) STAGE_CASE(rk8, 7) STAGE_CASE(rk8, 8) STAGE_CASE(rk8, 9)
+            STAGE_CASE(rk8, 10) STAGE_CASE(rk8, 11) STAGE_CASE(rk8, 12)
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (r) {
+            STAGE_CASE(rk4, 0) STAGE_CASE(rk4, 1) STAGE_CASE(rk4, 2) STAGE_CASE(rk4, 3) STAGE_CASE(rk4, 4)
+        default: return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_comb1(hipStream_t st, int method, const StageArgs& a)
+{
+    const uint32_t grid = (a.n_all + kBlock - 1) / kBlock;
+    if (grid == 0) return hipSuccess;
+    if (method == 4) hipLaunchKernelGGL(k_comb1<4>, dim3(grid), dim3(kBlock), 0, st, a);
+    else hipLaunchKernelGGL(k_comb1<2>, dim3(grid), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_commit(hipStream_t st, const StageArgs& a, int advance, DiskCtl* ctl)
+{
+    const uint32_t grid = (a.n_all + kBlock - 1) / kBlock;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_commit, dim3(grid), dim3(kBlock), 0, st, a, advance, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_operator(hipStream_t st, const StageArgs& a, double t, uint32_t disk, double* ax, double* ay,
+                           double* az, const double* vx, const double* vy, const double* vz)
+{
+    const uint32_t grid = (a.n_all + kBlock - 1) / kBlock;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_operator, dim3(grid), dim3(kBlock), 0, st, a, t, disk, ax, ay, az, vx, vy, vz);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Sources: FP64 mass with -1 for inactive bodies, FP32 mass with 0 for
+// inactive bodies and for the FP64 prefix.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_set_sources(uint32_t n_all, const double* mass, const int32_t* id,
+                                                        const uint32_t* disk_of, const DiskInfo* info, double* m_ex,
+                                                        float* m32)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_all) return;
+    const DiskInfo di = info[disk_of[i]];
+    const bool act = id[i] > 0;
+    m_ex[i] = act ? mass[i] : -1.0;
+    m32[i] = (act && (i - di.off) >= di.n_fp64) ? (float)mass[i] : 0.0f;
+}
+
+hipError_t launch_set_sources(hipStream_t st, uint32_t n_all, const double* mass, const int32_t* id,
+                              const uint32_t* disk_of, const DiskInfo* info, double* m_ex, float* m32)
+{
+    const uint32_t grid = (n_all + kBlock - 1) / kBlock;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_set_sources, dim3(grid), dim3(kBlock), 0, st, n_all, mass, id, disk_of, info, m_ex, m32);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Events
+// ---------------------------------------------------------------------------
+// Collision test d^2 < f^2 (R_i + R_j)^2 for active i < j (pp_disk.cu:321-371;
+// the star takes part as i, as in the reference kernel).  Pairs are appended
+// in any order; the host sorts them into (i, j) order.
+__global__ __launch_bounds__(kBlock) void k_collision(const WorkItem* __restrict__ items, uint32_t n_items,
+                                                      const double* __restrict__ x, const double* __restrict__ y,
+                                                      const double* __restrict__ z,
+                                                      const double* __restrict__ radius,
+                                                      const int32_t* __restrict__ id, double f2, uint32_t* pairs,
+                                                      uint32_t* counter, uint32_t cap)
+{
+    __shared__ double4 tile[kTile];
+    __shared__ int32_t tid_ok[kTile];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const WorkItem w = items[it];
+        const uint32_t i = w.sink_lo + tid;
+        const bool valid = i < w.sink_hi && id[i] > 0;
+        const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0, zi = valid ? z[i] : 0.0;
+        const double ri = valid ? radius[i] : 0.0;
+        for (uint32_t t0 = w.src_lo; t0 < w.src_hi; t0 += kTile) {
+            __syncthreads();
+            const uint32_t j = t0 + tid;
+            if (j < w.src_hi) {
+                tile[tid] = make_double4(x[j], y[j], z[j], radius[j]);
+                tid_ok[tid] = id[j] > 0;
+            }
+            __syncthreads();
+            const uint32_t nt = min((uint32_t)kTile, w.src_hi - t0);
+            if (valid) {
+                for (uint32_t k = 0; k < nt; ++k) {
+                    const uint32_t jj = t0 + k;
+                    if (jj <= i || !tid_ok[k]) continue;
+                    const double4 s = tile[k];
+                    const double dx = s.x - xi;
+                    const double dy = s.y - yi;
+                    const double dz = s.z - zi;
+                    const double r2 = SQR(dx) + SQR(dy) + SQR(dz);
+                    if (r2 < f2 * SQR((ri + s.w))) {
+                        const uint32_t slot = atomicAdd(counter, 1u);
+                        if (slot < cap) {
+                            pairs[2 * slot] = i;
+                            pairs[2 * slot + 1] = jj;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_collision(hipStream_t st, const WorkItem* items, uint32_t n_items, uint32_t grid,
+                            const double* x, const double* y, const double* z, const double* radius,
+                            const int32_t* id, double f2, uint32_t* pairs, uint32_t* counter, uint32_t cap)
+{
+    if (n_items == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_collision, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, radius, id, f2, pairs,
+                       counter, cap);
+    return hipGetLastError();
+}
+
+// Ejection / hit-centrum test on r^2 from the origin (pp_disk.cu:284-318).
+// out[2k] = body index, out[2k+1] = event name.
+__global__ __launch_bounds__(kBlock) void k_ejection(uint32_t lo, uint32_t hi, const double* x, const double* y,
+                                                     const double* z, const int32_t* id, const int32_t* type,
+                                                     double ed2, double hcd2, uint32_t* out, uint32_t* counter,
+                                                     uint32_t cap)
+{
+    const uint32_t i = lo + blockIdx.x * kBlock + threadIdx.x;
+    if (i >= hi) return;
+    if (!(id[i] > 0 && type[i] != 0)) return;
+    const double r2 = SQR(x[i]) + SQR(y[i]) + SQR(z[i]);
+    if (ed2 < r2) {
+        const uint32_t k = atomicAdd(counter, 1u);
+        if (k < cap) { out[2 * k] = i; out[2 * k + 1] = 2u; }
+    }
+    if (hcd2 > r2) {
+        const uint32_t k = atomicAdd(counter, 1u);
+        if (k < cap) { out[2 * k] = i; out[2 * k + 1] = 1u; }
+    }
+}
+
+hipError_t launch_ejection(hipStream_t st, uint32_t lo, uint32_t hi, const double* x, const double* y,
+                           const double* z, const int32_t* id, const int32_t* type, double ed2, double hcd2,
+                           uint32_t* out, uint32_t* counter, uint32_t cap)
+{
+    if (hi <= lo) return hipSuccess;
+    const uint32_t grid = (hi - lo + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_ejection, dim3(grid), dim3(kBlock), 0, st, lo, hi, x, y, z, id, type, ed2, hcd2, out,
+                       counter, cap);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Integrals (tools.cpp:367-410, 545-700) in FP64 with Neumaier summation.
+// ---------------------------------------------------------------------------
+struct KSum {
+    double s, c;
+    __device__ void add(double v)
+    {
+        const double t = s + v;
+        if (fabs(s) >= fabs(v)) c += (s - t) + v;
+        else c += (v - t) + s;
+        s = t;
+    }
+    __device__ double value() const { return s + c; }
+};
+
+constexpr int kIntChunk = 2048;
+
+// Potential partials: one value per (sink, source chunk).
+__global__ __launch_bounds__(kBlock) void k_potential(uint32_t lo, uint32_t hi, const double* __restrict__ x,
+                                                      const double* __restrict__ y, const double* __restrict__ z,
+                                                      const double* __restrict__ mass,
+                                                      const int32_t* __restrict__ id, double* __restrict__ out)
+{
+    __shared__ double4 tile[kTile];
+    const uint32_t n = hi - lo;
+    const uint32_t li = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t c = blockIdx.y;
+    const uint32_t j0 = lo + c * kIntChunk, j1 = min(hi, j0 + kIntChunk);
+    const uint32_t i = lo + li;
+    const bool valid = li < n && id[i] > 0;
+    const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0, zi = valid ? z[i] : 0.0;
+    const double mi = valid ? mass[i] : 0.0;
+    KSum s{0.0, 0.0};
+    for (uint32_t t0 = j0; t0 < j1; t0 += kTile) {
+        __syncthreads();
+        const uint32_t j = t0 + threadIdx.x;
+        if (j < j1) tile[threadIdx.x] = make_double4(x[j], y[j], z[j], id[j] > 0 ? mass[j] : -1.0);
+        __syncthreads();
+        const uint32_t nt = min((uint32_t)kTile, j1 - t0);
+        if (valid) {
+            for (uint32_t k = 0; k < nt; ++k) {
+                const double4 q = tile[k];
+                if (q.w < 0.0 || t0 + k == i) continue;
+                const double r = sqrt(SQR(q.x - xi) + SQR(q.y - yi) + SQR(q.z - zi));
+                s.add(mi * q.w / r);
+            }
+        }
+    }
+    if (li < n) out[(size_t)c * n + li] = s.value();
+}
+
+// Per-body terms of M, R, V, C and T.
+__global__ __launch_bounds__(kBlock) void k_body_terms(uint32_t lo, uint32_t hi, const double* x, const double* y,
+                                                       const double* z, const double* vx, const double* vy,
+                                                       const double* vz, const double* mass, const int32_t* id,
+                                                       double* out)
+{
+    const uint32_t n = hi - lo;
+    const uint32_t li = blockIdx.x * kBlock + threadIdx.x;
+    if (li >= n) return;
+    const uint32_t i = lo + li;
+    const bool act = id[i] > 0;
+    const double m = act ? mass[i] : 0.0;
+    const double cx = y[i] * vz[i] - z[i] * vy[i];
+    const double cy = z[i] * vx[i] - x[i] * vz[i];
+    const double cz = x[i] * vy[i] - y[i] * vx[i];
+    const double t[11] = {m, m * x[i], m * y[i], m * z[i], m * vx[i], m * vy[i], m * vz[i],
+                          cx * m, cy * m, cz * m, m * (SQR(vx[i]) + SQR(vy[i]) + SQR(vz[i]))};
+#pragma unroll
+    for (int k = 0; k < 11; ++k) out[(size_t)k * n + li] = act ? t[k] : 0.0;
+}
+
+// Sums rows of a [rows][len] array, one block per row, fixed order.
+__global__ __launch_bounds__(kBlock) void k_row_sum(const double* in, size_t len, double* out)
+{
+    __shared__ double ss[kBlock], cs[kBlock];
+    const double* row = in + (size_t)blockIdx.x * len;
+    KSum s{0.0, 0.0};
+    for (size_t k = threadIdx.x; k < len; k += kBlock) s.add(row[k]);
+    ss[threadIdx.x] = s.s;
+    cs[threadIdx.x] = s.c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        KSum t{0.0, 0.0};
+        for (int k = 0; k < kBlock; ++k) t.add(ss[k]);
+        for (int k = 0; k < kBlock; ++k) t.add(cs[k]);
+        out[blockIdx.x] = t.value();
+    }
+}
+
+hipError_t launch_integral(hipStream_t st, uint32_t lo, uint32_t hi, const double* x, const double* y,
+                           const double* z, const double* vx, const double* vy, const double* vz,
+                           const double* mass, const int32_t* id, double* scratch, size_t scratch_len, double* out,
+                           double* out_u)
+{
+    const uint32_t n = hi - lo;
+    if (n == 0) return hipSuccess;
+    const uint32_t nchunk = (n + kIntChunk - 1) / kIntChunk;
+    if (scratch_len < (size_t)11 * n || scratch_len < (size_t)nchunk * n) return hipErrorInvalidValue;
+    const uint32_t gb = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_body_terms, dim3(gb), dim3(kBlock), 0, st, lo, hi, x, y, z, vx, vy, vz, mass, id, scratch);
+    hipLaunchKernelGGL(k_row_sum, dim3(11), dim3(kBlock), 0, st, scratch, (size_t)n, out);
+    hipLaunchKernelGGL(k_potential, dim3(gb, nchunk), dim3(kBlock), 0, st, lo, hi, x, y, z, mass, id, scratch);
+    hipLaunchKernelGGL(k_row_sum, dim3(1), dim3(kBlock), 0, st, scratch, (size_t)nchunk * n, out_u);
+    return hipGetLastError();
+}
+
+}  // namespace rdamd

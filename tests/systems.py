@@ -1,0 +1,43 @@
+"""Seeded body systems shared by the tests."""
+import numpy as np
+
+from redamd_types import VAR4, PARAM, BODY_MD
+
+
+def small_system(n, seed, types=None, inactive=(), spread=2.0):
+    """Star at index 0 plus n-1 bodies with random states, sorted by type."""
+    rng = np.random.default_rng(seed)
+    r = np.zeros(n, VAR4)
+    v = np.zeros(n, VAR4)
+    p = np.zeros(n, PARAM)
+    b = np.zeros(n, BODY_MD)
+    r["x"], r["y"], r["z"] = rng.uniform(-spread, spread, n), rng.uniform(-spread, spread, n), rng.uniform(-0.1, 0.1, n)
+    v["x"], v["y"], v["z"] = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n), rng.uniform(-0.1, 0.1, n)
+    p["mass"] = 10 ** rng.uniform(-9, -5, n)
+    p["mass"][0] = 1.0
+    p["radius"] = 1e-5
+    p["density"] = 1e6
+    p["cd"] = 1.0
+    types = list(types) if types is not None else [0] + [3] * (n - 1)
+    b["body_type"] = types
+    b["id"] = np.arange(1, n + 1)
+    for i in inactive:
+        b["id"][i] *= -1
+    nbt = np.bincount(np.asarray(types), minlength=7).astype(np.uint32)
+    return r, v, p, b, nbt
+
+
+def mixed_types(n):
+    """Every body class: star, giants, rocky, protoplanets, super-pl, pl, test particles."""
+    k = max(1, n // 10)
+    t = [0] + [1] * 2 + [2] * k + [3] * k + [4] * k + [5] * (n - 3 - 4 * k) + [6] * k
+    return t[:n]
+
+
+def bits_equal(a, b):
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def as3(a):
+    return np.stack([a["x"], a["y"], a["z"]], 1)

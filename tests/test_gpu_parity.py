@@ -1,0 +1,220 @@
+"""HIP engine vs the CPU oracle, through the C ABI (include/red_abi.h).
+
+FP64 mode restates the reference GPU kernel (pp_disk.cu:374-416) in the same
+operation order, so forces and whole RK4 / RKF78 trajectories must be
+bit-identical to the oracle.  Mixed precision is checked against the FP64
+oracle with the tolerance written in each test.  Drag uses device pow/exp,
+so it is compared to a relative 1e-13."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import redamd as R
+from systems import small_system, mixed_types, bits_equal, as3
+
+pytestmark = pytest.mark.gpu
+
+KDAY = R.K_GAUSS     # 1 day in tau units
+
+
+def pair(r, v, p, b, nbt, thr=(0.0, 0.0, 0.0), gas=None, t=0.0):
+    d = R.Disk(r, v, p, b, nbt, t=t, threshold=thr, gas=gas)
+    o = O.ODisk(r, v, p, b, nbt, t=t, thr=thr, gas=gas)
+    return d, o
+
+
+@pytest.mark.parametrize("case", ["disk256", "inactive1000", "classes700", "ragged1"])
+def test_force_fp64_bitexact(case):
+    if case == "disk256":
+        r, v, p, b, nbt = R.make_disk(255, 11)
+    elif case == "inactive1000":
+        r, v, p, b, nbt = small_system(1000, 12, inactive=(1, 2, 17, 256, 511, 999))
+    elif case == "classes700":
+        r, v, p, b, nbt = small_system(700, 13, mixed_types(700), inactive=(5, 400))
+    else:
+        r, v, p, b, nbt = small_system(2, 14)
+    d, o = pair(r, v, p, b, nbt)
+    dy = d.calc_dydx(1, 0, 0.0, r, v)
+    rc, ody = o.calc_dydx(1, 0, 0.0, r, v)
+    assert rc == 0
+    assert bits_equal(dy, ody)
+    dv = d.calc_dydx(0, 0, 0.0, r, v)
+    assert bits_equal(dv, v)
+
+
+@pytest.mark.parametrize("spl", [1, 2, 4])
+def test_force_mixed_vs_fp64(spl):
+    r, v, p, b, nbt = R.make_disk(3000, 21)
+    d, o = pair(r, v, p, b, nbt)
+    d.set_precision(R.RD_PREC_MIXED)
+    d.set_tuning(sinks_per_lane=spl, chunk=512)
+    dy = as3(d.calc_dydx(1, 0, 0.0, r, v))
+    rc, ody = o.calc_dydx(1, 0, 0.0, r, v)
+    ref = as3(ody)
+    # mutual part alone (star term removed) for a relative check of the FP32 path
+    r1, v1, p1, b1, _ = r.copy(), v, p.copy(), b.copy(), nbt
+    p1["mass"][0] = 0.0
+    o2 = O.ODisk(r1, v1, p1, b1, nbt)
+    _, mut = o2.calc_dydx(1, 0, 0.0, r1, v1)
+    mut = as3(mut)
+    err = np.linalg.norm(dy - ref, axis=1)
+    # tolerance: FP32 pair terms (rel. 2^-22 each incl. rsqrt) summed over <= 3000 sources
+    assert np.all(err[1:] <= 2e-5 * np.linalg.norm(mut[1:], axis=1) + 1e-18), err.max()
+    # whole acceleration of the disk bodies (the star's is the FP32 mutual sum alone)
+    assert np.all(err[1:] <= 1e-9 * np.linalg.norm(ref[1:], axis=1))
+
+
+def test_force_mixed_classes_and_inactive():
+    r, v, p, b, nbt = small_system(900, 22, mixed_types(900), inactive=(7, 300, 650))
+    d, o = pair(r, v, p, b, nbt)
+    d.set_precision(R.RD_PREC_MIXED)
+    dy = as3(d.calc_dydx(1, 0, 0.0, r, v))
+    _, ody = o.calc_dydx(1, 0, 0.0, r, v)
+    ref = as3(ody)
+    err = np.linalg.norm(dy - ref, axis=1)
+    assert np.all(err <= 1e-6 * np.linalg.norm(ref, axis=1) + 1e-20)
+    for i in (7, 300, 650):
+        assert np.all(dy[i] == 0.0)
+
+
+@pytest.mark.parametrize("mode", [R.RD_DRAG_NSI, R.RD_DRAG_ALL_MASSIVE])
+def test_drag_vs_oracle(mode):
+    r, v, p, b, nbt = small_system(400, 31, mixed_types(400))
+    g = R.gas_params()
+    d, o = pair(r, v, p, b, nbt, gas=g, t=3.0)
+    d.set_drag_mode(mode)
+    O.lib.or_disk_set_drag_mode(o.h, mode)
+    dy = as3(d.calc_dydx(1, 2, 3.0, r, v))
+    _, ody = o.calc_dydx(1, 2, 3.0, r, v)
+    ref = as3(ody)
+    assert np.all(np.abs(dy - ref) <= 1e-13 * np.abs(ref).max(axis=1, keepdims=True))
+
+
+def test_rk4_fixed_bitexact():
+    r, v, p, b, nbt = R.make_disk(256, 1)
+    d, o = pair(r, v, p, b, nbt)
+    h = 0.1 * KDAY
+    it = R.Integrator(d, R.RD_RK4, h, adaptive=False)
+    oi = O.OInteg(o, 2, h, adaptive=False)
+    it.steps(25)
+    for _ in range(25):
+        oi.step()
+    t, rr, vv, _, _ = d.state()
+    ot, orr, ovv, _, _ = o.state()
+    assert t == ot
+    assert bits_equal(rr, orr) and bits_equal(vv, ovv)
+    for _ in range(5):
+        assert it.step() == oi.step()
+    t, rr, vv, _, _ = d.state()
+    ot, orr, ovv, _, _ = o.state()
+    assert t == ot and bits_equal(rr, orr) and bits_equal(vv, ovv)
+
+
+@pytest.mark.parametrize("method", [R.RD_RKF78, R.RD_RK4])
+def test_adaptive_bitexact_with_rejections(method):
+    r, v, p, b, nbt = R.make_disk(64, 2, a_min=0.8, a_max=1.2)
+    d, o = pair(r, v, p, b, nbt)
+    h0 = 60.0 * KDAY                    # too large: the first attempts are rejected
+    tol = 1e-12
+    it = R.Integrator(d, method, h0, adaptive=True, tol=tol)
+    oi = O.OInteg(o, method, h0, adaptive=True, tol=tol)
+    for _ in range(12):
+        assert it.step() == oi.step()
+        assert it.dt_next() == oi.dt_next()
+    assert it.counters() == oi.counters()
+    assert it.counters()[2] > 0
+    t, rr, vv, _, _ = d.state()
+    ot, orr, ovv, _, _ = o.state()
+    assert t == ot and bits_equal(rr, orr) and bits_equal(vv, ovv)
+
+
+def test_graph_matches_direct_launch():
+    out = []
+    for g in (0, 1):
+        r, v, p, b, nbt = R.make_disk(500, 3)
+        d = R.Disk(r, v, p, b, nbt)
+        d.set_precision(R.RD_PREC_MIXED)
+        d.set_tuning(use_graph=g)
+        it = R.Integrator(d, R.RD_RKF78, 5 * KDAY, adaptive=True, tol=1e-10)
+        for _ in range(4):
+            it.step()
+        it.steps(3)
+        out.append(d.state())
+    assert out[0][0] == out[1][0]
+    assert bits_equal(out[0][1], out[1][1]) and bits_equal(out[0][2], out[1][2])
+
+
+def test_integral_vs_oracle():
+    r, v, p, b, nbt = small_system(3000, 41, inactive=(3, 9))
+    d, o = pair(r, v, p, b, nbt)
+    gi, oi = d.integral(), o.integral()
+    assert abs(gi["E"] - oi["E"]) <= 1e-13 * abs(oi["E"])
+    for f in ("R", "V", "C"):
+        for c in "xyz":
+            assert abs(gi[f][c] - oi[f][c]) <= 1e-14 * max(1e-300, np.abs([oi[f][q] for q in "xyz"]).max())
+
+
+def crowded(n, seed):
+    """A thin ring of inflated bodies that collides within a few steps."""
+    r, v, p, b, nbt = R.make_disk(n, seed, a_min=0.99, a_max=1.01, sig_e=0.02, sig_i=0.001, r_star=1e-5)
+    return r, v, p, b, nbt
+
+
+def test_driver_loop_events_match_oracle():
+    r, v, p, b, nbt = crowded(96, 5)
+    thr = (0.2, 50.0, 1000.0)       # hit-centrum 0.2 AU, ejection 50 AU, radius factor 1000
+    d, o = pair(r, v, p, b, nbt, thr=thr)
+    h = 1.0 * KDAY
+    it = R.Integrator(d, R.RD_RKF78, h, adaptive=True, tol=1e-10)
+    oi = O.OInteg(o, 4, h, adaptive=True, tol=1e-10)
+    it.run(40.0, 2.0, 60)
+    rc, steps = oi.run(40.0, 2.0, 60)
+    assert rc == 0
+    ev, oev = d.event_log(), o.event_log()
+    assert len(ev) == len(oev) and len(ev) > 16          # enough to trigger a rebuild
+    assert bits_equal(ev, oev)
+    t, rr, vv, pp, bb = d.state()
+    ot, orr, ovv, opp, obb = o.state()
+    assert t == ot
+    assert bits_equal(rr, orr) and bits_equal(vv, ovv) and bits_equal(pp, opp) and bits_equal(bb, obb)
+    assert all(np.array_equal(a, b_) for a, b_ in zip(d.n_objects(), o.counts()))
+
+
+def test_ejection_and_hit_centrum():
+    r, v, p, b, nbt = small_system(50, 51)
+    r["x"][10], r["y"][10], r["z"][10] = 0.05, 0.0, 0.0     # inside 0.1 AU
+    r["x"][20] = 80.0                                        # beyond 60 AU
+    thr = (0.1, 60.0, 0.0)
+    d, o = pair(r, v, p, b, nbt, thr=thr)
+    n = d.check_ejection_hit_centrum()
+    on = O.lib.or_check_ejection_hit_centrum(o.h)
+    assert n == on and n >= 2
+    d.handle_ejection_hit_centrum()
+    assert O.lib.or_handle_ejection_hit_centrum(o.h) == 0
+    assert bits_equal(d.event_log(), o.event_log())
+    _, rr, vv, pp, bb = d.state()
+    _, orr, ovv, opp, obb = o.state()
+    assert bits_equal(rr, orr) and bits_equal(vv, ovv) and bits_equal(pp, opp) and bits_equal(bb, obb)
+
+
+@pytest.mark.parametrize("prec", [R.RD_PREC_FP64, R.RD_PREC_MIXED])
+def test_batch_matches_single_disks(prec):
+    sizes = [64, 300, 1000]
+    parts = [R.make_disk(n - 1, 100 + k) for k, n in enumerate(sizes)]
+    cat = [np.concatenate([q[i] for q in parts]) for i in range(4)]
+    nbt = np.stack([q[4] for q in parts])
+    bd = R.Disk(*cat, nbt, batch=True)
+    bd.set_precision(prec)
+    bit = R.Integrator(bd, R.RD_RKF78, 3 * KDAY, adaptive=True, tol=1e-11)
+    bit.steps(4)
+    for k, q in enumerate(parts):
+        sd = R.Disk(*q)
+        sd.set_precision(prec)
+        sit = R.Integrator(sd, R.RD_RKF78, 3 * KDAY, adaptive=True, tol=1e-11)
+        sit.steps(4)
+        a, b_ = bd.state(k), sd.state()
+        assert a[0] == b_[0]
+        assert bits_equal(a[1], b_[1]) and bits_equal(a[2], b_[2])
+        assert bit.counters(k) == sit.counters()
