@@ -77,7 +77,15 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default="", help="S,CHUNK,BLOCKS_PER_CU for the mixed kernel (skips autotune)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run the byte-counter calibration copy of this size and exit")
     args = ap.parse_args()
+    if args.copy_bytes:
+        ms = R.bench_copy(args.copy_bytes, 5)
+        print(json.dumps({"copy_bytes": args.copy_bytes, "ms_per_launch": ms,
+                          "GBps": 2 * args.copy_bytes / (ms * 1e-3) / 1e9}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one GPU)",
               file=sys.stderr)
@@ -90,7 +98,12 @@ def main():
     d = R.Disk(r, v, p, b, nbt)
     d.set_precision(R.RD_PREC_MIXED if args.precision == "mixed" else R.RD_PREC_FP64)
     d.set_tuning(use_graph=1)
-    tuning = d.autotune() if (args.precision == "mixed" and not args.no_autotune) else d.tuning()
+    if args.tuning:
+        S, Cs, B = (int(x) for x in args.tuning.split(","))
+        d.set_tuning(sinks_per_lane=S, chunk=Cs, blocks_per_cu=B)
+        tuning = d.tuning()
+    else:
+        tuning = d.autotune() if (args.precision == "mixed" and not args.no_autotune) else d.tuning()
     d.set_tuning(use_graph=1)
     it = R.Integrator(d, R.RD_RKF78, h0, adaptive=True, tol=args.tol)
     print(f"# warmup {args.warmup} steps", file=sys.stderr, flush=True)

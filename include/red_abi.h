@@ -156,6 +156,15 @@ int rd_calc_dydx(rd_disk* d, uint32_t k, uint32_t i, uint32_t rr, double t,
  * sink-source pair count of one launch. */
 int rd_bench_force(rd_disk* d, uint32_t n_iter, double* ms_per_launch, double* pairs_per_launch);
 
+/* Measurement helper without a reference counterpart: time n_iter launches
+ * of a plain device copy of `bytes` bytes (read + write), used to scale
+ * hardware byte counters. */
+int rd_bench_copy(int device, size_t bytes, uint32_t n_iter, double* ms_per_launch);
+/* Measurement helper without a reference counterpart: mean time of one pair
+ * kernel launch plus the stage kernel that folds its partial sums (what
+ * rd_autotune minimises). */
+int rd_bench_force_stage(rd_disk* d, uint32_t n_iter, double* ms);
+
 /* ---- integrators --------------------------------------------------------- */
 /* rungekutta4 / rungekutta8 constructors (int_rungekutta4.h:15, int_rungekutta8.h:18) */
 int rd_integrator_create(rd_disk* d, int method, double dt, int adaptive, double tol, rd_integrator** out);

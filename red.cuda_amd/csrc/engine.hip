@@ -418,17 +418,14 @@ int upload_disk_of(rd_disk* d)
 StageArgs make_args(rd_disk* d, int r, int adaptive, int err_all)
 {
     StageArgs a{};
-    a.x = d->x; a.y = d->y; a.z = d->z; a.vx = d->vx; a.vy = d->vy; a.vz = d->vz;
-    a.xo = d->xo; a.yo = d->yo; a.zo = d->zo; a.vxo = d->vxo; a.vyo = d->vyo; a.vzo = d->vzo;
-    if (r == 0) { a.fx = d->x; a.fy = d->y; a.fz = d->z; }
-    else { a.fx = d->P[r & 1][0]; a.fy = d->P[r & 1][1]; a.fz = d->P[r & 1][2]; }
-    a.px = d->P[(r + 1) & 1][0]; a.py = d->P[(r + 1) & 1][1]; a.pz = d->P[(r + 1) & 1][2];
-    for (int s = 0; s < kMaxStage; ++s)
-        for (int c = 0; c < 3; ++c) {
-            a.kv[s][c] = kv_ptr(d, s, c);
-            a.ka[s][c] = ka_ptr(d, s, c);
-        }
-    a.kv[0][0] = d->vx; a.kv[0][1] = d->vy; a.kv[0][2] = d->vz;
+    a.pos[0] = d->x; a.pos[1] = d->y; a.pos[2] = d->z;
+    a.vel[0] = d->vx; a.vel[1] = d->vy; a.vel[2] = d->vz;
+    a.out[0] = d->xo; a.out[1] = d->yo; a.out[2] = d->zo; a.out[3] = d->vxo; a.out[4] = d->vyo; a.out[5] = d->vzo;
+    for (int c = 0; c < 3; ++c) {
+        a.fpos[c] = r == 0 ? a.pos[c] : d->P[r & 1][c];
+        a.npos[c] = d->P[(r + 1) & 1][c];
+    }
+    a.K = d->K;
     a.part = d->part;
     a.n_all = d->n_all;
     a.err_all = err_all;
@@ -469,7 +466,7 @@ hipError_t enqueue_attempt(rd_integrator* it)
     }
     for (int r = 1; r <= last_stage(it); ++r) {
         StageArgs a = make_args(d, r, it->adaptive, it->err_all);
-        if ((e = enqueue_force(d, a.fx, a.fy, a.fz, F_ACTIVE)) != hipSuccess) return e;
+        if ((e = enqueue_force(d, a.fpos[0], a.fpos[1], a.fpos[2], F_ACTIVE)) != hipSuccess) return e;
         if ((e = launch_stage(d->stream, it->method, r, a)) != hipSuccess) return e;
     }
     return hipSuccess;
@@ -764,6 +761,71 @@ int rd_bench_force(rd_disk* d, uint32_t n_iter, double* ms, double* pairs)
     return RD_OK;
 }
 
+// Mean time of one pair-kernel launch followed by the stage kernel that folds
+// its partial sums (stage 1 of the current method's tableau, h = 0 so the
+// state is untouched).
+static int time_force_stage(rd_disk* d, uint32_t n_iter, double* ms)
+{
+    for (uint32_t q = 0; q < d->n_disk; ++q) d->h_ctl[q] = DiskCtl{d->hd[q].t, 0.0, F_NEED_K0 | F_ACTIVE, 0};
+    HIP_TRY(hipMemcpyAsync(d->ctl, d->h_ctl, d->n_disk * sizeof(DiskCtl), hipMemcpyHostToDevice, d->stream));
+    StageArgs a = make_args(d, 1, 0, 0);
+    a.fpos[0] = d->x; a.fpos[1] = d->y; a.fpos[2] = d->z;
+    HIP_TRY(enqueue_force(d, d->x, d->y, d->z, F_NEED_K0));
+    HIP_TRY(launch_stage(d->stream, RD_RKF78, 1, a));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, d->stream));
+    for (uint32_t q = 0; q < n_iter; ++q) {
+        HIP_TRY(enqueue_force(d, d->x, d->y, d->z, F_NEED_K0));
+        HIP_TRY(launch_stage(d->stream, RD_RKF78, 1, a));
+    }
+    HIP_TRY(hipEventRecord(e1, d->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float el = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&el, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)el / n_iter;
+    return RD_OK;
+}
+
+int rd_bench_copy(int device, size_t bytes, uint32_t n_iter, double* ms)
+{
+    if (bytes < 32 || n_iter == 0) return fail(RD_EINVAL, "bad argument");
+    HIP_TRY(hipSetDevice(device));
+    hipStream_t st;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    void *a = nullptr, *b = nullptr;
+    HIP_TRY(hipMalloc(&a, bytes));
+    HIP_TRY(hipMalloc(&b, bytes));
+    HIP_TRY(hipMemsetAsync(a, 0, bytes, st));
+    HIP_TRY(launch_copy(st, a, b, bytes));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, st));
+    for (uint32_t q = 0; q < n_iter; ++q) HIP_TRY(launch_copy(st, a, b, bytes));
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float el = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&el, e0, e1));
+    if (ms) *ms = (double)el / n_iter;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    (void)hipStreamDestroy(st);
+    return RD_OK;
+}
+
+int rd_bench_force_stage(rd_disk* d, uint32_t n_iter, double* ms)
+{
+    if (!d || n_iter == 0 || !ms) return fail(RD_EINVAL, "bad argument");
+    if (int rc = ensure_lists(d)) return rc;
+    return time_force_stage(d, n_iter, ms);
+}
+
 int rd_autotune(rd_disk* d, rd_tuning* best)
 {
     if (!d) return fail(RD_EINVAL, "NULL disk");
@@ -790,7 +852,7 @@ int rd_autotune(rd_disk* d, rd_tuning* best)
                 d->tuning = t;
                 if (int rc = build_lists(d)) { d->tuning = keep; build_lists(d); return rc; }
                 double ms = 0.0;
-                if (int rc = rd_bench_force(d, 3, &ms, nullptr)) { d->tuning = keep; build_lists(d); return rc; }
+                if (int rc = time_force_stage(d, 3, &ms)) { d->tuning = keep; build_lists(d); return rc; }
                 if (ms < best_ms) { best_ms = ms; win = t; }
             }
         }

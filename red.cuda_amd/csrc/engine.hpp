@@ -64,19 +64,19 @@ struct GasDev {
 };
 
 struct StageArgs {
-    const double *x, *y, *z, *vx, *vy, *vz;       // y_n
-    double *xo, *yo, *zo, *vxo, *vyo, *vzo;       // y_{n+1}
-    const double *fx, *fy, *fz;                   // force-input positions of this stage
-    double *px, *py, *pz;                         // next-stage positions (the other P buffer)
-    double* kv[kMaxStage][3];                     // kv[0] = state velocity
-    double* ka[kMaxStage][3];
+    const double* pos[3];        // y_n positions
+    const double* vel[3];        // y_n velocities (= stage-0 position derivative)
+    double* out[6];              // y_{n+1}: x, y, z, vx, vy, vz
+    const double* fpos[3];       // force-input positions of this stage
+    double* npos[3];             // next-stage positions (the other P buffer)
+    double* K;                   // [13][6][n_all]: velocity (0-2) and acceleration (3-5) per stage
     const double* part;
     uint32_t n_all;
-    uint32_t err_all;                             // error_check_for_tp
+    uint32_t err_all;            // error_check_for_tp
     const uint32_t* disk_of;
     const DiskInfo* info;
     const DiskCtl* ctl;
-    unsigned long long* err;                      // per disk, bits of the raw max |E|
+    unsigned long long* err;     // per disk, bits of the raw max |E|
     const double *mass, *radius, *density, *cd;
     const int32_t* id;
     GasDev gas;
@@ -108,6 +108,7 @@ hipError_t launch_integral(hipStream_t s, uint32_t lo, uint32_t hi, const double
                            const double* mass, const int32_t* id, double* scratch, size_t scratch_len,
                            double* out /* 11 doubles: M, Rx,Ry,Rz, Vx,Vy,Vz, Cx,Cy,Cz, T */,
                            double* out_u);
+hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes);
 hipError_t launch_set_sources(hipStream_t s, uint32_t n_all, const double* mass, const int32_t* id,
                               const uint32_t* disk_of, const DiskInfo* info, double* m_ex, float* m32);
 

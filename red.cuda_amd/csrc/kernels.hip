@@ -8,6 +8,8 @@
 
 #include <float.h>
 
+#include <algorithm>
+
 namespace rdamd {
 
 #define SQR(x) ((x) * (x))
@@ -127,6 +129,10 @@ struct alignas(16) SrcF {
 // touching), so the floor changes r^2 by < 1e-8 ulp.
 constexpr float kEps2 = 1.0e-24f;
 
+// Per-body kernels run one wave per block so that small N still spreads over
+// all CUs.
+constexpr int kBodyBlock = 64;
+
 // Mixed precision pair kernel.  A work item is 64*S sinks (S per lane) against
 // a chunk of sources; the four waves of the block split each 256-source LDS
 // tile in quarters.  Per pair: FP64 offset rounded to FP32, r^2 by FMA, one
@@ -143,7 +149,8 @@ __global__ __launch_bounds__(kBlock) void k_force_mixed(const WorkItem* __restri
 {
     __shared__ SrcF tile[kTile];
     __shared__ double red[3][S][3][64];
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         const WorkItem w = items[it];
         if (!(ctl[w.disk].flags & need)) continue;
@@ -257,7 +264,10 @@ hipError_t launch_force_mixed(hipStream_t st, int S, const WorkItem* items, uint
 }
 
 // ---------------------------------------------------------------------------
-// Per-body pieces of the stage kernel
+// Per-body stage kernels.  One thread per (body, component): every RK
+// combination is separable by component, so the grid is 3 x ceil(n/64) and a
+// wave holds 64 consecutive bodies of one component (coalesced).  All loads
+// are issued before the first store.
 // ---------------------------------------------------------------------------
 // pp_disk.cu:60-84
 __device__ double reduction_factor(const GasDev& g, double t)
@@ -273,10 +283,10 @@ __device__ double reduction_factor(const GasDev& g, double t)
     }
 }
 
-// Quadratic drag a += f u with the analytic gas disk (pp_disk.cu:87-155, 486-521).
-__device__ void drag_accel(const GasDev& g, double decr, double m_star, double rx, double ry, double rz, double vx,
-                           double vy, double vz, double cd, double radius, double density, double& ax, double& ay,
-                           double& az)
+// Quadratic drag a += f u with the analytic gas disk (pp_disk.cu:87-155,
+// 486-521); returns component q of f u.
+__device__ double drag_component(const GasDev& g, double decr, double m_star, double rx, double ry, double rz,
+                                 double vx, double vy, double vz, double cd, double radius, double density, int q)
 {
     // circular_velocity (pp_disk.cu:108-142)
     double gx = 0.0, gy = 0.0;
@@ -313,105 +323,127 @@ __device__ void drag_accel(const GasDev& g, double decr, double m_star, double r
         dens = a * SQR(SQR(r)) * exp(-arg);
     }
     const double f = decr * (3.0 * cd * dens * u_n) / (8.0 * radius * density);
-    ax += f * ux;
-    ay += f * uy;
-    az += f * uz;
+    return f * (q == 0 ? ux : (q == 1 ? uy : uz));
 }
 
-// Acceleration of body i at this stage: pair partials (+ FP64 prefix in mixed
-// mode), zero for inactive sinks, plus drag.
-__device__ void stage_accel(const StageArgs& a, uint32_t i, const DiskInfo& di, double t, const double* vx,
-                            const double* vy, const double* vz, double& ax, double& ay, double& az)
+__device__ inline const double* kv_of(const StageArgs& a, int r, int q)
+{
+    return r == 0 ? a.vel[q] : a.K + ((size_t)r * 6 + q) * a.n_all;
+}
+__device__ inline const double* ka_of(const StageArgs& a, int r, int q)
+{
+    return a.K + ((size_t)r * 6 + 3 + q) * a.n_all;
+}
+__device__ inline double* kv_w(const StageArgs& a, int r, int q) { return a.K + ((size_t)r * 6 + q) * a.n_all; }
+__device__ inline double* ka_w(const StageArgs& a, int r, int q) { return a.K + ((size_t)r * 6 + 3 + q) * a.n_all; }
+
+// Sum of the pair partials of body i, component q, in slot order.
+__device__ double partial_sum(const StageArgs& a, uint32_t i, int q, uint32_t nc)
+{
+    const double* __restrict__ p = a.part + (size_t)q * a.n_all + i;
+    const size_t st = (size_t)3 * a.n_all;
+    double s = 0.0;
+    uint32_t c = 0;
+    for (; c + 16 <= nc; c += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = p[(size_t)(c + u) * st];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) s = (c + u == 0) ? v[u] : s + v[u];
+    }
+    if (c < nc) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = (c + u < nc) ? p[(size_t)(c + u) * st] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (c + u < nc) s = (c + u == 0) ? v[u] : s + v[u];
+    }
+    return s;
+}
+
+// Component q of the acceleration of body i at this stage from the summed
+// partials s: FP64 prefix in mixed mode, zero for inactive sinks, plus drag.
+// pos: force-input positions; vel: velocities of this stage.
+__device__ double stage_accel(const StageArgs& a, uint32_t i, int q, const DiskInfo& di, double t, double s,
+                              const double* __restrict__ const* pos, const double* __restrict__ const* vel)
 {
     const uint32_t li = i - di.off;
     const int cls = li < di.n_si ? 0 : (li < di.n_massive ? 1 : 2);
-    const uint32_t nc = di.nchunk[cls];
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    if (nc > 0) {
-        sx = a.part[(size_t)0 * a.n_all + i];
-        sy = a.part[(size_t)1 * a.n_all + i];
-        sz = a.part[(size_t)2 * a.n_all + i];
-        for (uint32_t c = 1; c < nc; ++c) {
-            sx += a.part[(size_t)(c * 3 + 0) * a.n_all + i];
-            sy += a.part[(size_t)(c * 3 + 1) * a.n_all + i];
-            sz += a.part[(size_t)(c * 3 + 2) * a.n_all + i];
-        }
-    }
     if (a.mixed) {
         // FP64 prefix sources (the star block), reference formula and order.
-        const double xi = a.fx[i], yi = a.fy[i], zi = a.fz[i];
+        const double xi = pos[0][i], yi = pos[1][i], zi = pos[2][i];
         const uint32_t src_hi = cls == 1 ? di.n_si : di.n_massive;
         const uint32_t hi = min(di.n_fp64, src_hi);
-        double bx = 0.0, by = 0.0, bz = 0.0;
+        double b = 0.0;
         for (uint32_t lj = 0; lj < hi; ++lj) {
             const uint32_t j = di.off + lj;
             if (j == i || a.id[j] < 0) continue;
-            const double dx = a.fx[j] - xi;
-            const double dy = a.fy[j] - yi;
-            const double dz = a.fz[j] - zi;
+            const double dx = pos[0][j] - xi;
+            const double dy = pos[1][j] - yi;
+            const double dz = pos[2][j] - zi;
             const double r2 = SQR(dx) + SQR(dy) + SQR(dz);
             const double d = sqrt(r2);
             const double sc = a.mass[j] / (d * r2);
-            bx += sc * dx;
-            by += sc * dy;
-            bz += sc * dz;
+            b += sc * (q == 0 ? dx : (q == 1 ? dy : dz));
         }
-        sx = bx + sx;
-        sy = by + sy;
-        sz = bz + sz;
+        s = b + s;
     }
-    if (!(a.id[i] > 0)) sx = sy = sz = 0.0;
+    if (!(a.id[i] > 0)) s = 0.0;
     if (a.gas.on && li >= di.drag_lo && li < di.drag_hi) {
         const double decr = reduction_factor(a.gas, t);
-        if (!(1.0e-6 > decr)) {
-            drag_accel(a.gas, decr, a.mass[di.off], a.fx[i], a.fy[i], a.fz[i], vx[i], vy[i], vz[i], a.cd[i],
-                       a.radius[i], a.density[i], sx, sy, sz);
-        }
+        if (!(1.0e-6 > decr))
+            s += drag_component(a.gas, decr, a.mass[di.off], pos[0][i], pos[1][i], pos[2][i], vel[0][i], vel[1][i],
+                                vel[2][i], a.cd[i], a.radius[i], a.density[i], q);
     }
-    ax = sx;
-    ay = sy;
-    az = sz;
+    return s;
 }
 
-// ytemp = y + h * (sum_q a_q k_q) for the next RKF78 stage R (int_rungekutta8.cu:82-246)
-template <int R>
-__device__ void rk8_comb(const StageArgs& a, uint32_t i, double h, int pbuf_unused)
+// Pins already-issued loads above the per-disk control branch (the compiler
+// would otherwise sink them into it and serialise them behind the control
+// loads).  The asm is empty: it only marks the values as used here.
+__device__ inline void pin(double& v) { asm volatile("" : "+v"(v)); }
+
+__device__ inline uint32_t nchunk_of(const DiskInfo& di, uint32_t i)
+{
+    const uint32_t li = i - di.off;
+    return di.nchunk[li < di.n_si ? 0 : (li < di.n_massive ? 1 : 2)];
+}
+
+// Column values of stage row R (int_rungekutta8.cu:82-246) for body i,
+// component q; the column R_DONE (just computed) is filled in later.
+template <int R, int R_DONE>
+__device__ void rk8_load_row(const StageArgs& a, uint32_t i, int q, double* kv, double* ka)
 {
     constexpr RowDef row = rk8_row(R);
-    const double* yv[6] = {a.x, a.y, a.z, a.vx, a.vy, a.vz};
-    double* outp[6] = {a.px, a.py, a.pz, a.kv[R][0], a.kv[R][1], a.kv[R][2]};
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
-        double* const* K = c < 3 ? a.kv[row.col[0]] : a.ka[row.col[0]];
-        double s = row.a[0] * K[c % 3][i];
-#pragma unroll
-        for (int q = 1; q < row.n; ++q) {
-            double* const* Kq = c < 3 ? a.kv[row.col[q]] : a.ka[row.col[q]];
-            s = s + row.a[q] * Kq[c % 3][i];
+    for (int c = 0; c < row.n; ++c) {
+        if (row.col[c] != R_DONE) {
+            kv[c] = kv_of(a, row.col[c], q)[i];
+            ka[c] = ka_of(a, row.col[c], q)[i];
         }
-        outp[c][i] = yv[c][i] + h * s;
     }
 }
-
-// y_{n+1} = y + h (b0 k0 + b5 k5 + b6 (k6 + k7) + b8 (k8 + k9) + b10 k10) (int_rungekutta8.cu:264-274)
-__device__ void rk8_ynp1(const StageArgs& a, uint32_t i, double h)
+// y + h (sum_c a_c k_c), left to right over the nonzero columns.
+template <int R, int R_DONE>
+__device__ void rk8_comb_q(double y, double v, double h, double* kv, double* ka, double kRv, double kRa,
+                           double& pos, double& vel)
 {
-    constexpr double b0 = 41.0 / 840.0, b5 = 34.0 / 105.0, b6 = 9.0 / 35.0, b8 = 9.0 / 280.0, b10 = 41.0 / 840.0;
-    const double* yv[6] = {a.x, a.y, a.z, a.vx, a.vy, a.vz};
-    double* yo[6] = {a.xo, a.yo, a.zo, a.vxo, a.vyo, a.vzo};
+    constexpr RowDef row = rk8_row(R);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
-        double* const* k0 = c < 3 ? a.kv[0] : a.ka[0];
-        double* const* k5 = c < 3 ? a.kv[5] : a.ka[5];
-        double* const* k6 = c < 3 ? a.kv[6] : a.ka[6];
-        double* const* k7 = c < 3 ? a.kv[7] : a.ka[7];
-        double* const* k8 = c < 3 ? a.kv[8] : a.ka[8];
-        double* const* k9 = c < 3 ? a.kv[9] : a.ka[9];
-        double* const* k10 = c < 3 ? a.kv[10] : a.ka[10];
-        const int q = c % 3;
-        yo[c][i] = yv[c][i] + h * (b0 * k0[q][i] + b5 * k5[q][i] + b6 * (k6[q][i] + k7[q][i]) +
-                                   b8 * (k8[q][i] + k9[q][i]) + b10 * k10[q][i]);
+    for (int c = 0; c < row.n; ++c)
+        if (row.col[c] == R_DONE) {
+            kv[c] = kRv;
+            ka[c] = kRa;
+        }
+    double sv = row.a[0] * kv[0], sa = row.a[0] * ka[0];
+#pragma unroll
+    for (int c = 1; c < row.n; ++c) {
+        sv = sv + row.a[c] * kv[c];
+        sa = sa + row.a[c] * ka[c];
     }
+    pos = y + h * sv;
+    vel = v + h * sa;
 }
 
 // Wave-level max into the per-disk error slot; waves whose lanes span two
@@ -434,46 +466,94 @@ __device__ void error_reduce(const StageArgs& a, bool valid, uint32_t d, double 
 
 // RKF7(8) stage kernel for stage R: finishes k_R (partials, FP64 prefix,
 // drag) and forms what the next stage needs from body i's own k values.
+// Loads whose address depends only on i are issued before the per-disk
+// control is known.
 template <int R>
-__global__ __launch_bounds__(kBlock) void k_stage_rk8(StageArgs a)
+__global__ __launch_bounds__(kBodyBlock) void k_stage_rk8(StageArgs a)
 {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const bool in = i < a.n_all;
-    uint32_t d = 0;
-    bool valid = false;
-    double e = 0.0;
-    if (in) {
-        d = a.disk_of[i];
-        const DiskCtl c = a.ctl[d];
-        const uint32_t need = R == 0 ? F_NEED_K0 : F_ACTIVE;
-        valid = (c.flags & need) != 0;
-        if (valid) {
-            const DiskInfo di = a.info[d];
-            const double h = c.h;
-            const double t = c.t + rk8_c(R) * h;
-            double ax, ay, az;
-            stage_accel(a, i, di, t, a.kv[R][0], a.kv[R][1], a.kv[R][2], ax, ay, az);
-            a.ka[R][0][i] = ax;
-            a.ka[R][1][i] = ay;
-            a.ka[R][2][i] = az;
-            if constexpr (R >= 1 && R <= 9) {
-                rk8_comb<R + 1>(a, i, h, 0);
-            } else if constexpr (R == 10) {
-                rk8_ynp1(a, i, h);
-                if (a.adaptive) rk8_comb<11>(a, i, h, 0);
-            } else if constexpr (R == 11) {
-                rk8_comb<12>(a, i, h, 0);
-            } else if constexpr (R == 12) {
-                const uint32_t li = i - di.off;
-                if (a.err_all || li < di.err_hi) {
+    const uint32_t i0 = blockIdx.x * kBodyBlock + threadIdx.x;
+    const bool in = i0 < a.n_all;
+    const uint32_t i = in ? i0 : a.n_all - 1;
+    const int q = blockIdx.y;
+    // round 1
+    const uint32_t d = a.disk_of[i];
+    const double y = a.pos[q][i], v = a.vel[q][i];
+    const double kRv = kv_of(a, R, q)[i];
+    constexpr int NR = (R >= 1 && R <= 9) ? R + 1 : (R == 10 ? 11 : (R == 11 ? 12 : 1));
+    double kv[9], ka[9];
+    if constexpr (R >= 1 && R <= 11) rk8_load_row<NR, R>(a, i, q, kv, ka);
+    double y10v[6], y10a[6];
+    if constexpr (R == 10) {
+        const int cols[6] = {0, 5, 6, 7, 8, 9};
 #pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        const double ev = fabs(a.kv[0][q][i] + a.kv[10][q][i] - a.kv[11][q][i] - a.kv[12][q][i]);
-                        const double ea = fabs(a.ka[0][q][i] + a.ka[10][q][i] - a.ka[11][q][i] - a.ka[12][q][i]);
-                        e = fmax(e, fmax(ev, ea));
-                    }
-                }
+        for (int u = 0; u < 6; ++u) {
+            y10v[u] = kv_of(a, cols[u], q)[i];
+            y10a[u] = ka_of(a, cols[u], q)[i];
+        }
+    }
+    double e0v = 0, e10v = 0, e11v = 0, e0a = 0, e10a = 0, e11a = 0;
+    if constexpr (R == 12) {
+        e0v = kv_of(a, 0, q)[i]; e10v = kv_of(a, 10, q)[i]; e11v = kv_of(a, 11, q)[i];
+        e0a = ka_of(a, 0, q)[i]; e10a = ka_of(a, 10, q)[i]; e11a = ka_of(a, 11, q)[i];
+        pin(e0v); pin(e10v); pin(e11v); pin(e0a); pin(e10a); pin(e11a);
+    }
+    if constexpr (R >= 1 && R <= 11) {
+        constexpr RowDef row = rk8_row(NR);
+#pragma unroll
+        for (int c = 0; c < row.n; ++c)
+            if (row.col[c] != R) {
+                pin(kv[c]);
+                pin(ka[c]);
             }
+    }
+    if constexpr (R == 10) {
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+            pin(y10v[u]);
+            pin(y10a[u]);
+        }
+    }
+    double yy = y, vv = v, kk = kRv;
+    pin(yy); pin(vv); pin(kk);
+    // round 2
+    const DiskCtl c = a.ctl[d];
+    const DiskInfo di = a.info[d];
+    const bool valid = in && (c.flags & (R == 0 ? F_NEED_K0 : F_ACTIVE)) != 0;
+    double e = 0.0;
+    if (valid) {
+        const double h = c.h;
+        const double t = c.t + rk8_c(R) * h;
+        const double* vR[3] = {kv_of(a, R, 0), kv_of(a, R, 1), kv_of(a, R, 2)};
+        const double kRa = stage_accel(a, i, q, di, t, partial_sum(a, i, q, nchunk_of(di, i)), a.fpos, vR);
+        ka_w(a, R, q)[i] = kRa;
+        if constexpr (R >= 1 && R <= 9) {
+            double p, w;
+            rk8_comb_q<NR, R>(yy, vv, h, kv, ka, kk, kRa, p, w);
+            a.npos[q][i] = p;
+            kv_w(a, NR, q)[i] = w;
+        } else if constexpr (R == 10) {
+            // y_{n+1} = y + h (b0 k0 + b5 k5 + b6 (k6 + k7) + b8 (k8 + k9) + b10 k10) (int_rungekutta8.cu:264-274)
+            constexpr double b0 = 41.0 / 840.0, b5 = 34.0 / 105.0, b6 = 9.0 / 35.0, b8 = 9.0 / 280.0,
+                             b10 = 41.0 / 840.0;
+            a.out[q][i] = yy + h * (b0 * y10v[0] + b5 * y10v[1] + b6 * (y10v[2] + y10v[3]) +
+                                   b8 * (y10v[4] + y10v[5]) + b10 * kk);
+            a.out[3 + q][i] = vv + h * (b0 * y10a[0] + b5 * y10a[1] + b6 * (y10a[2] + y10a[3]) +
+                                       b8 * (y10a[4] + y10a[5]) + b10 * kRa);
+            if (a.adaptive) {
+                double p, w;
+                rk8_comb_q<11, R>(yy, vv, h, kv, ka, kk, kRa, p, w);
+                a.npos[q][i] = p;
+                kv_w(a, 11, q)[i] = w;
+            }
+        } else if constexpr (R == 11) {
+            double p, w;
+            rk8_comb_q<12, R>(yy, vv, h, kv, ka, kk, kRa, p, w);
+            a.npos[q][i] = p;
+            kv_w(a, 12, q)[i] = w;
+        } else if constexpr (R == 12) {
+            const uint32_t li = i - di.off;
+            // |f0 + f10 - f11 - f12| (int_rungekutta8.cu:250-260)
+            if (a.err_all || li < di.err_hi) e = fmax(fabs(e0v + e10v - e11v - kk), fabs(e0a + e10a - e11a - kRa));
         }
     }
     if constexpr (R == 12) error_reduce(a, valid, d, e);
@@ -481,60 +561,55 @@ __global__ __launch_bounds__(kBlock) void k_stage_rk8(StageArgs a)
 
 // Classical RK4 stage kernel (int_rungekutta4.cu:37-73, 130-247).
 template <int R>
-__global__ __launch_bounds__(kBlock) void k_stage_rk4(StageArgs a)
+__global__ __launch_bounds__(kBodyBlock) void k_stage_rk4(StageArgs a)
 {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const bool in = i < a.n_all;
-    uint32_t d = 0;
-    bool valid = false;
+    const uint32_t i0 = blockIdx.x * kBodyBlock + threadIdx.x;
+    const bool in = i0 < a.n_all;
+    const uint32_t i = in ? i0 : a.n_all - 1;
+    const int q = blockIdx.y;
+    const uint32_t d = a.disk_of[i];
+    const double y = a.pos[q][i], v = a.vel[q][i];
+    const double kRv = kv_of(a, R, q)[i];
+    double k0v = 0, k1v = 0, k2v = 0, k0a = 0, k1a = 0, k2a = 0, k3v = 0, k3a = 0;
+    if constexpr (R == 3) {
+        k0v = kv_of(a, 0, q)[i]; k1v = kv_of(a, 1, q)[i]; k2v = kv_of(a, 2, q)[i];
+        k0a = ka_of(a, 0, q)[i]; k1a = ka_of(a, 1, q)[i]; k2a = ka_of(a, 2, q)[i];
+    }
+    if constexpr (R == 4) {
+        k3v = kv_of(a, 3, q)[i];
+        k3a = ka_of(a, 3, q)[i];
+    }
+    pin(k0v); pin(k1v); pin(k2v); pin(k0a); pin(k1a); pin(k2a); pin(k3v); pin(k3a);
+    double yy = y, vv = v, kk = kRv;
+    pin(yy); pin(vv); pin(kk);
+    const DiskCtl c = a.ctl[d];
+    const DiskInfo di = a.info[d];
+    const bool valid = in && (c.flags & (R == 0 ? F_NEED_K0 : F_ACTIVE)) != 0;
     double e = 0.0;
-    if (in) {
-        d = a.disk_of[i];
-        const DiskCtl c = a.ctl[d];
-        const uint32_t need = R == 0 ? F_NEED_K0 : F_ACTIVE;
-        valid = (c.flags & need) != 0;
-        if (valid) {
-            const DiskInfo di = a.info[d];
-            const double h = c.h;
-            const double t = c.t + rk4_c(R) * h;
-            double ax, ay, az;
-            stage_accel(a, i, di, t, a.kv[R][0], a.kv[R][1], a.kv[R][2], ax, ay, az);
-            a.ka[R][0][i] = ax;
-            a.ka[R][1][i] = ay;
-            a.ka[R][2][i] = az;
-            const double* yv[6] = {a.x, a.y, a.z, a.vx, a.vy, a.vz};
-            if constexpr (R == 1 || R == 2) {
-                const double fac = rk4_a(R + 1) * h;
-                double* outp[6] = {a.px, a.py, a.pz, a.kv[R + 1][0], a.kv[R + 1][1], a.kv[R + 1][2]};
-#pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    const double k = q < 3 ? a.kv[R][q][i] : a.ka[R][q - 3][i];
-                    outp[q][i] = yv[q][i] + fac * k;
-                }
-            } else if constexpr (R == 3) {
-                const double b0 = (1.0 / 6.0) * h, b1 = (1.0 / 3.0) * h;
-                double* yo[6] = {a.xo, a.yo, a.zo, a.vxo, a.vyo, a.vzo};
-                double* p4[6] = {a.px, a.py, a.pz, a.kv[4][0], a.kv[4][1], a.kv[4][2]};
-#pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    double* const* K = q < 3 ? a.kv[0] : a.ka[0];
-                    double* const* K1 = q < 3 ? a.kv[1] : a.ka[1];
-                    double* const* K2 = q < 3 ? a.kv[2] : a.ka[2];
-                    double* const* K3 = q < 3 ? a.kv[3] : a.ka[3];
-                    const int c3 = q % 3;
-                    const double v = yv[q][i] + b0 * (K[c3][i] + K3[c3][i]) + b1 * (K1[c3][i] + K2[c3][i]);
-                    yo[q][i] = v;
-                    if (a.adaptive) p4[q][i] = v;
-                }
-            } else if constexpr (R == 4) {
-                const uint32_t li = i - di.off;
-                if (a.err_all || li < di.err_hi) {
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        e = fmax(e, fmax(fabs(a.kv[3][q][i] - a.kv[4][q][i]), fabs(a.ka[3][q][i] - a.ka[4][q][i])));
-                    }
-                }
+    if (valid) {
+        const double h = c.h;
+        const double t = c.t + rk4_c(R) * h;
+        const double* vR[3] = {kv_of(a, R, 0), kv_of(a, R, 1), kv_of(a, R, 2)};
+        const double kRa = stage_accel(a, i, q, di, t, partial_sum(a, i, q, nchunk_of(di, i)), a.fpos, vR);
+        ka_w(a, R, q)[i] = kRa;
+        if constexpr (R == 1 || R == 2) {
+            const double fac = rk4_a(R + 1) * h;
+            a.npos[q][i] = yy + fac * kk;
+            kv_w(a, R + 1, q)[i] = vv + fac * kRa;
+        } else if constexpr (R == 3) {
+            // y + b0 (f1 + f4) + b1 (f2 + f3), b pre-multiplied by h (int_rungekutta4.cu:50-60, 150-171)
+            const double b0 = (1.0 / 6.0) * h, b1 = (1.0 / 3.0) * h;
+            const double yo = yy + b0 * (k0v + kk) + b1 * (k1v + k2v);
+            const double vo = vv + b0 * (k0a + kRa) + b1 * (k1a + k2a);
+            a.out[q][i] = yo;
+            a.out[3 + q][i] = vo;
+            if (a.adaptive) {
+                a.npos[q][i] = yo;
+                kv_w(a, 4, q)[i] = vo;
             }
+        } else if constexpr (R == 4) {
+            const uint32_t li = i - di.off;
+            if (a.err_all || li < di.err_hi) e = fmax(fabs(k3v - kk), fabs(k3a - kRa));   // |f4 - f5|
         }
     }
     if constexpr (R == 4) error_reduce(a, valid, d, e);
@@ -543,72 +618,70 @@ __global__ __launch_bounds__(kBlock) void k_stage_rk4(StageArgs a)
 // First combination of an attempt (stage 1 from k0), and clears the error
 // slot of each active disk.
 template <int M>
-__global__ __launch_bounds__(kBlock) void k_comb1(StageArgs a)
+__global__ __launch_bounds__(kBodyBlock) void k_comb1(StageArgs a)
 {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = blockIdx.x * kBodyBlock + threadIdx.x;
+    const int q = blockIdx.y;
     if (i >= a.n_all) return;
     const uint32_t d = a.disk_of[i];
+    const double y = a.pos[q][i], v = a.vel[q][i];
+    const double k0v = a.vel[q][i], k0a = ka_of(a, 0, q)[i];
     const DiskCtl c = a.ctl[d];
     if (!(c.flags & F_ACTIVE)) return;
-    if (i == a.info[d].off) a.err[d] = 0ull;
+    if (q == 0 && i == a.info[d].off) a.err[d] = 0ull;
     if constexpr (M == 4) {
-        rk8_comb<1>(a, i, c.h, 0);
+        double kv[9], ka[9], p, w;
+        rk8_comb_q<1, 0>(y, v, c.h, kv, ka, k0v, k0a, p, w);
+        a.npos[q][i] = p;
+        kv_w(a, 1, q)[i] = w;
     } else {
         const double fac = rk4_a(1) * c.h;
-        const double* yv[6] = {a.x, a.y, a.z, a.vx, a.vy, a.vz};
-        double* outp[6] = {a.px, a.py, a.pz, a.kv[1][0], a.kv[1][1], a.kv[1][2]};
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const double k = q < 3 ? a.kv[0][q][i] : a.ka[0][q - 3][i];
-            outp[q][i] = yv[q][i] + fac * k;
-        }
+        a.npos[q][i] = y + fac * k0v;
+        kv_w(a, 1, q)[i] = v + fac * k0a;
     }
 }
 
 // y_n <- y_{n+1} for disks flagged F_COMMIT; optionally t += h on the device
 // (fixed-step runs that never return to the host between steps).
-__global__ __launch_bounds__(kBlock) void k_commit(StageArgs a, int advance, DiskCtl* ctl)
+__global__ __launch_bounds__(kBodyBlock) void k_commit(StageArgs a, int advance, DiskCtl* ctl)
 {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t i = blockIdx.x * kBodyBlock + threadIdx.x;
+    const int q = blockIdx.y;
     if (i >= a.n_all) return;
     const uint32_t d = a.disk_of[i];
-    const uint32_t fl = ctl[d].flags;
-    if (!(fl & F_COMMIT)) return;
-    const_cast<double*>(a.x)[i] = a.xo[i];
-    const_cast<double*>(a.y)[i] = a.yo[i];
-    const_cast<double*>(a.z)[i] = a.zo[i];
-    const_cast<double*>(a.vx)[i] = a.vxo[i];
-    const_cast<double*>(a.vy)[i] = a.vyo[i];
-    const_cast<double*>(a.vz)[i] = a.vzo[i];
-    if (advance && i == a.info[d].off) ctl[d].t = ctl[d].t + ctl[d].h;
+    if (!(ctl[d].flags & F_COMMIT)) return;
+    const_cast<double*>(a.pos[q])[i] = a.out[q][i];
+    const_cast<double*>(a.vel[q])[i] = a.out[3 + q][i];
+    if (advance && q == 0 && i == a.info[d].off) ctl[d].t = ctl[d].t + ctl[d].h;
 }
 
 // pp_disk::calc_dydx(1, ...) for one disk: acceleration at time t from the
-// partials, written to (ax, ay, az).
-__global__ __launch_bounds__(kBlock) void k_operator(StageArgs a, double t, uint32_t disk, double* ax, double* ay,
-                                                     double* az, const double* vx, const double* vy,
-                                                     const double* vz)
+// partials, written to acc[q].
+__global__ __launch_bounds__(kBodyBlock) void k_operator(StageArgs a, double t, uint32_t disk, double* ax,
+                                                         double* ay, double* az, const double* vx, const double* vy,
+                                                         const double* vz)
 {
     const DiskInfo di = a.info[disk];
-    const uint32_t li = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t li = blockIdx.x * kBodyBlock + threadIdx.x;
+    const int q = blockIdx.y;
     if (li >= di.n) return;
     const uint32_t i = di.off + li;
-    double x, y, z;
-    stage_accel(a, i, di, t, vx, vy, vz, x, y, z);
-    ax[i] = x;
-    ay[i] = y;
-    az[i] = z;
+    const double* vel[3] = {vx, vy, vz};
+    const double s = stage_accel(a, i, q, di, t, partial_sum(a, i, q, nchunk_of(di, i)), a.fpos, vel);
+    (q == 0 ? ax : (q == 1 ? ay : az))[i] = s;
 }
 
 #define STAGE_CASE(M, R)                                                                   \
     case R:                                                                                \
-        hipLaunchKernelGGL(k_stage_##M<R>, dim3(grid), dim3(kBlock), 0, st, a);            \
+        hipLaunchKernelGGL(k_stage_##M<R>, grid, dim3(kBodyBlock), 0, st, a);              \
         break;
+
+static dim3 body_grid(uint32_t n) { return dim3((n + kBodyBlock - 1) / kBodyBlock, 3); }
 
 hipError_t launch_stage(hipStream_t st, int method, int r, const StageArgs& a)
 {
-    const uint32_t grid = (a.n_all + kBlock - 1) / kBlock;
-    if (grid == 0) return hipSuccess;
+    if (a.n_all == 0) return hipSuccess;
+    const dim3 grid = body_grid(a.n_all);
     if (method == 4) {
         switch (r) {
             STAGE_CASE(rk8, 0) STAGE_CASE(rk8, 1) STAGE_CASE(rk8, 2) STAGE_CASE(rk8, 3) STAGE_CASE(rk8, 4)
@@ -627,27 +700,24 @@ hipError_t launch_stage(hipStream_t st, int method, int r, const StageArgs& a)
 
 hipError_t launch_comb1(hipStream_t st, int method, const StageArgs& a)
 {
-    const uint32_t grid = (a.n_all + kBlock - 1) / kBlock;
-    if (grid == 0) return hipSuccess;
-    if (method == 4) hipLaunchKernelGGL(k_comb1<4>, dim3(grid), dim3(kBlock), 0, st, a);
-    else hipLaunchKernelGGL(k_comb1<2>, dim3(grid), dim3(kBlock), 0, st, a);
+    if (a.n_all == 0) return hipSuccess;
+    if (method == 4) hipLaunchKernelGGL(k_comb1<4>, body_grid(a.n_all), dim3(kBodyBlock), 0, st, a);
+    else hipLaunchKernelGGL(k_comb1<2>, body_grid(a.n_all), dim3(kBodyBlock), 0, st, a);
     return hipGetLastError();
 }
 
 hipError_t launch_commit(hipStream_t st, const StageArgs& a, int advance, DiskCtl* ctl)
 {
-    const uint32_t grid = (a.n_all + kBlock - 1) / kBlock;
-    if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_commit, dim3(grid), dim3(kBlock), 0, st, a, advance, ctl);
+    if (a.n_all == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_commit, body_grid(a.n_all), dim3(kBodyBlock), 0, st, a, advance, ctl);
     return hipGetLastError();
 }
 
 hipError_t launch_operator(hipStream_t st, const StageArgs& a, double t, uint32_t disk, double* ax, double* ay,
                            double* az, const double* vx, const double* vy, const double* vz)
 {
-    const uint32_t grid = (a.n_all + kBlock - 1) / kBlock;
-    if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_operator, dim3(grid), dim3(kBlock), 0, st, a, t, disk, ax, ay, az, vx, vy, vz);
+    if (a.n_all == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_operator, body_grid(a.n_all), dim3(kBodyBlock), 0, st, a, t, disk, ax, ay, az, vx, vy, vz);
     return hipGetLastError();
 }
 
@@ -876,6 +946,22 @@ hipError_t launch_integral(hipStream_t st, uint32_t lo, uint32_t hi, const doubl
     hipLaunchKernelGGL(k_row_sum, dim3(11), dim3(kBlock), 0, st, scratch, (size_t)n, out);
     hipLaunchKernelGGL(k_potential, dim3(gb, nchunk), dim3(kBlock), 0, st, lo, hi, x, y, z, mass, id, scratch);
     hipLaunchKernelGGL(k_row_sum, dim3(1), dim3(kBlock), 0, st, scratch, (size_t)nchunk * n, out_u);
+    return hipGetLastError();
+}
+
+// Plain device copy used to calibrate hardware byte counters (FETCH_SIZE,
+// WRITE_SIZE) against a known traffic volume.
+__global__ __launch_bounds__(kBlock) void k_copy(const double4* __restrict__ src, double4* __restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) dst[i] = src[i];
+}
+
+hipError_t launch_copy(hipStream_t st, const void* src, void* dst, size_t bytes)
+{
+    const size_t n = bytes / sizeof(double4);
+    if (n == 0) return hipSuccess;
+    const size_t grid = std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(k_copy, dim3((uint32_t)grid), dim3(kBlock), 0, st, (const double4*)src, (double4*)dst, n);
     return hipGetLastError();
 }
 

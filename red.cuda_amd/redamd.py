@@ -61,6 +61,8 @@ _SIGS = {
     "rd_sync": (C.c_int, [_P]),
     "rd_calc_dydx": (C.c_int, [_P, _u32, _u32, _u32, _f64, _P, _P, _P]),
     "rd_bench_force": (C.c_int, [_P, _u32, C.POINTER(_f64), C.POINTER(_f64)]),
+    "rd_bench_copy": (C.c_int, [C.c_int, C.c_size_t, _u32, C.POINTER(_f64)]),
+    "rd_bench_force_stage": (C.c_int, [_P, _u32, C.POINTER(_f64)]),
     "rd_integrator_create": (C.c_int, [_P, C.c_int, _f64, C.c_int, _f64, C.POINTER(_P)]),
     "rd_integrator_destroy": (C.c_int, [_P]),
     "rd_step": (C.c_int, [_P, C.POINTER(_f64)]),
@@ -113,6 +115,13 @@ def device_count():
     n = C.c_int(0)
     check(lib.rd_device_count(C.byref(n)))
     return n.value
+
+
+def bench_copy(nbytes, n_iter=10, device=0):
+    """Mean time [ms] of a plain device copy of nbytes (calibrates byte counters)."""
+    ms = _f64()
+    check(lib.rd_bench_copy(device, nbytes, n_iter, C.byref(ms)))
+    return ms.value
 
 
 def make_disk(n, seed, body_type=BODY_PROTO, a_min=0.5, a_max=5.0, a_power=-0.5, sig_e=0.01, sig_i=0.005,
@@ -243,6 +252,11 @@ class Disk:
         ms, pairs = _f64(), _f64()
         check(lib.rd_bench_force(self.h, n_iter, C.byref(ms), C.byref(pairs)))
         return ms.value, pairs.value
+
+    def bench_force_stage(self, n_iter=10):
+        ms = _f64()
+        check(lib.rd_bench_force_stage(self.h, n_iter, C.byref(ms)))
+        return ms.value
 
     def check_ejection_hit_centrum(self, k=0):
         n = _u32()

@@ -35,8 +35,16 @@ def mixed_types(n):
 
 
 def bits_equal(a, b):
+    """Bit-for-bit equality of every field (padding bytes are not data)."""
+    a, b = np.asarray(a), np.asarray(b)
+    if a.shape != b.shape:
+        return False
+    if a.dtype.names:
+        if a.dtype.names != b.dtype.names:
+            return False
+        return all(bits_equal(a[f], b[f]) for f in a.dtype.names)
     a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
-    return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    return a.dtype.itemsize == b.dtype.itemsize and np.array_equal(a.view(np.uint8), b.view(np.uint8))
 
 
 def as3(a):
