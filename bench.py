@@ -33,6 +33,7 @@ import numpy as np  # noqa: E402
 import redamd as R  # noqa: E402
 
 PEAK_FP32 = 157.3e12
+PEAK_FP64 = 78.6e12
 FLOP_PER_PAIR = 20.0
 
 
@@ -66,6 +67,136 @@ def cpu_baseline(r, v, p, b, nbt, h0, tol):
                       f"symmetric FP64 loop, {el:.1f} s on {cpu_model()} (nproc {os.cpu_count()})"}
 
 
+def roofline(d, precision):
+    ms, pairs_launch = d.bench_force(20)
+    achieved = FLOP_PER_PAIR * pairs_launch / (ms * 1e-3)
+    return {"bound": "valu-fp32" if precision == "mixed" else "valu-fp64", "achieved": achieved / 1e12,
+            "peak": (PEAK_FP32 if precision == "mixed" else PEAK_FP64) / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / (PEAK_FP32 if precision == "mixed" else PEAK_FP64), "traffic": None,
+            "kernel": "k_force_mixed" if precision == "mixed" else "k_force_exact",
+            "ms_per_launch": ms, "pairs_per_launch": pairs_launch}
+
+
+def emit(out):
+    print(json.dumps(out))
+
+
+def bench_config1(args):
+    """Config 1: star + 256, RK4 fixed h = 0.1 d, 10^4 steps, FP64 (bit-exact path)."""
+    steps = args.steps if args.steps != 1000 else 10000
+    r, v, p, b, nbt = R.make_disk(256, 1)
+    d = R.Disk(r, v, p, b, nbt)
+    d.set_tuning(use_graph=1)
+    it = R.Integrator(d, R.RD_RK4, 0.1 * R.K_GAUSS, adaptive=False)
+    it.steps(args.warmup)
+    E0 = d.integral()["E"]
+    ev0, p0 = it.work()
+    d.sync()
+    t0 = time.perf_counter()
+    it.steps(steps)
+    d.sync()
+    el = time.perf_counter() - t0
+    ev1, p1 = it.work()
+    rel_e = abs(d.integral()["E"] - E0) / abs(E0)
+    out = {"metric": "pair interactions/sec + RK4 steps/sec (config 1)", "value": (p1 - p0) / el,
+           "unit": "pair interactions/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+           "ms_per_step": el * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic seeded disk",
+           "config": {"workload": "star + 256 planetesimals, RK4 fixed h = 0.1 d, FP64, seed 1"},
+           "rk4_steps_per_s": steps / el, "rel_energy_error": rel_e, "roofline": roofline(d, "fp64")}
+    emit(out)
+
+
+def bench_config3(args):
+    """Config 3: N = 131072 in a gas disk, drag on all massive bodies, perfect merging
+    (radius factor 1), hit-centrum 0.1 AU, mixed precision, RKF7(8)."""
+    n = 131072 if args.n == 16384 else args.n
+    steps = args.steps if args.steps != 1000 else 40
+    r, v, p, b, nbt = R.make_disk(n, 3, size_mode=1, lo=1.0, hi=100.0, cd=1.0)
+    g = R.gas_params()
+    d = R.Disk(r, v, p, b, nbt, threshold=(0.1, 0.0, 1.0), gas=g)
+    d.set_drag_mode(R.RD_DRAG_ALL_MASSIVE)
+    d.set_precision(R.RD_PREC_MIXED)
+    tuning = d.autotune() if not args.no_autotune else d.tuning()
+    d.set_tuning(use_graph=1)
+    a_min = np.min(np.hypot(r["x"][1:], r["y"][1:]))
+    h0 = 2 * np.pi * np.sqrt(a_min ** 3) / 1000.0
+    it = R.Integrator(d, R.RD_RKF78, h0, adaptive=True, tol=args.tol)
+    it.run(1e30, 1e30, args.warmup)
+    E0 = d.integral()["E"]
+    ev0, p0 = it.work()
+    c0 = it.counters()
+    d.sync()
+    t0 = time.perf_counter()
+    it.run(1e30, 1e30, steps)
+    d.sync()
+    el = time.perf_counter() - t0
+    ev1, p1 = it.work()
+    c1 = it.counters()
+    rel_e = abs(d.integral()["E"] - E0) / abs(E0)
+    out = {"metric": "pair interactions/sec + RKF78 steps/sec (config 3)", "value": (p1 - p0) / el,
+           "unit": "pair interactions/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+           "ms_per_step": el * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64 state / f32 pair force", "data": "synthetic seeded disk",
+           "config": {"workload": f"N={n} in a gas disk (Sigma=1700 r^-1.5, H/r=0.05 r^0.25, eta=0.002), drag on "
+                                  "all massive bodies, R 1-100 km, merging f=1, hit-centrum 0.1 AU, mixed, RKF7(8) "
+                                  f"tol {args.tol:g}",
+                      "tuning": {"sinks_per_lane": tuning.sinks_per_lane, "chunk": tuning.chunk,
+                                 "blocks_per_cu": tuning.blocks_per_cu}},
+           "rkf78_steps_per_s": steps / el, "attempts": c1[0] - c0[0], "rejected": c1[2] - c0[2],
+           "events": int(len(d.event_log())), "rel_energy_error": rel_e,
+           "note": "energy is not conserved with gas drag; the error column is the drag dissipation",
+           "roofline": roofline(d, "mixed")}
+    emit(out)
+
+
+def batch_sizes(n_disk=512, lo=64, hi=4096, seed=4):
+    """N_d uniform on [lo, hi] from MT19937(seed) raw outputs, as red_random::uniform(int, int)
+    (src/red.cuda.initial/distribution.cpp:43-46)."""
+    rs = np.random.RandomState(seed)
+    out = []
+    for _ in range(n_disk):
+        u = int.from_bytes(rs.bytes(4), "little")
+        out.append(lo + int(u / 4294967296.0 * (hi - lo + 1)))
+    return out
+
+
+def bench_config4(args):
+    """Config 4: 512 independent disks, N_d ~ U[64, 4096], disk d seeded 4000 + d,
+    one persistent launch per stage over all disks, attempts replayed from a hipGraph."""
+    sizes = batch_sizes()
+    parts = [R.make_disk(n, 4000 + k) for k, n in enumerate(sizes)]
+    cat = [np.concatenate([q[i] for q in parts]) for i in range(4)]
+    nbt = np.stack([q[4] for q in parts])
+    d = R.Disk(*cat, nbt, batch=True)
+    d.set_precision(R.RD_PREC_MIXED if args.precision == "mixed" else R.RD_PREC_FP64)
+    tuning = d.autotune() if (args.precision == "mixed" and not args.no_autotune) else d.tuning()
+    d.set_tuning(use_graph=1)
+    steps = args.steps if args.steps != 1000 else 100
+    it = R.Integrator(d, R.RD_RKF78, 0.1 * R.K_GAUSS, adaptive=True, tol=args.tol)
+    it.steps(args.warmup)
+    ev0, p0 = it.work()
+    d.sync()
+    t0 = time.perf_counter()
+    it.steps(steps)
+    d.sync()
+    el = time.perf_counter() - t0
+    ev1, p1 = it.work()
+    tried = sum(it.counters(k)[0] for k in range(len(sizes)))
+    out = {"metric": "pair interactions/sec + RKF78 steps/sec (config 4, batched)", "value": (p1 - p0) / el,
+           "unit": "pair interactions/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+           "ms_per_step": el * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64 state / f32 pair force" if args.precision == "mixed" else "f64",
+           "data": "synthetic seeded disks",
+           "config": {"workload": f"512 disks, N_d ~ U[64, 4096] (total {sum(sizes) + len(sizes)} bodies), "
+                                  f"RKF7(8) tol {args.tol:g}, {args.precision}",
+                      "tuning": {"sinks_per_lane": tuning.sinks_per_lane, "chunk": tuning.chunk,
+                                 "blocks_per_cu": tuning.blocks_per_cu}},
+           "disk_steps_per_s": steps * len(sizes) / el, "attempts_all_disks": tried,
+           "roofline": roofline(d, args.precision)}
+    emit(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +209,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", default="", help="S,CHUNK,BLOCKS_PER_CU for the mixed kernel (skips autotune)")
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4],
+                    help="BASELINE config: 1 RK4 N=256, 2 RKF78 N=16384 (default), 3 gas+collisions N=131072, "
+                         "4 batch of 512 disks")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run the byte-counter calibration copy of this size and exit")
     args = ap.parse_args()
@@ -90,6 +224,12 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one GPU)",
               file=sys.stderr)
         sys.exit(2)
+    if args.config == 1:
+        return bench_config1(args)
+    if args.config == 3:
+        return bench_config3(args)
+    if args.config == 4:
+        return bench_config4(args)
 
     r, v, p, b, nbt = R.make_disk(args.n, args.seed)
     # initial step as redinit: min orbital period / 1000 (src/red.cuda.initial/main.cu:317-342)

@@ -580,9 +580,7 @@ int rd_batch_create(uint32_t n_disk, const uint32_t* n_by_type, const rd_var4* r
     if (e != hipSuccess) { delete d; return fail(RD_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
     int rc = alloc_buffers(d);
     if (!rc) rc = upload_disk_of(d);
-    for (uint32_t k = 0; k < n_disk && !rc; ++k)
-        rc = upload_bodies(d, d->hd[k].off, d->hd[k].n, r + d->hd[k].off, v + d->hd[k].off, p + d->hd[k].off,
-                           bmd + d->hd[k].off);
+    if (!rc) rc = upload_bodies(d, 0, d->n_all, r, v, p, bmd);
     if (!rc) rc = build_lists(d);
     if (rc) {
         std::string keep = g_err;
@@ -1169,78 +1167,133 @@ void inc_counters(HostDisk& h)
 
 extern "C" {
 
-int rd_check_ejection_hit_centrum(rd_disk* d, uint32_t k, uint32_t* n)
+}  // extern "C"
+
+namespace {
+
+// Uploads ctl with F_CHECK on the disks in `want` (t and h kept).
+int flag_check(rd_disk* d, const std::vector<char>& want)
 {
-    if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
-    HostDisk& h = d->hd[k];
-    const double ej = d->thr[RD_THRESHOLD_EJECTION_DISTANCE], hc = d->thr[RD_THRESHOLD_HIT_CENTRUM_DISTANCE];
-    const double ed2 = (0.0 < ej ? ej * ej : DBL_MAX);
-    const double hcd2 = hc * hc;
+    for (uint32_t q = 0; q < d->n_disk; ++q) {
+        d->h_ctl[q].t = d->hd[q].t;
+        d->h_ctl[q].flags = want[q] ? F_CHECK : 0u;
+    }
+    HIP_TRY(hipMemcpyAsync(d->ctl, d->h_ctl, d->n_disk * sizeof(DiskCtl), hipMemcpyHostToDevice, d->stream));
     HIP_TRY(hipMemsetAsync(d->ev_counter, 0, sizeof(uint32_t), d->stream));
-    HIP_TRY(launch_ejection(d->stream, h.off, h.off + h.n, d->x, d->y, d->z, d->id, d->type, ed2, hcd2, d->ev_buf,
-                            d->ev_counter, d->ev_cap));
+    return RD_OK;
+}
+
+int read_pairs(rd_disk* d, std::vector<std::pair<uint32_t, uint32_t>>& pr)
+{
     uint32_t cnt = 0;
     HIP_TRY(hipMemcpyAsync(&cnt, d->ev_counter, sizeof(uint32_t), hipMemcpyDeviceToHost, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
-    if (cnt > d->ev_cap) return fail(RD_EOVERFLOW, "event buffer overflow");
+    if (cnt > d->ev_cap)
+        return fail(RD_EOVERFLOW, "event buffer overflow: more than " + std::to_string(d->ev_cap) + " events");
+    pr.clear();
     if (cnt) {
         std::vector<uint32_t> buf(2 * (size_t)cnt);
         HIP_TRY(hipMemcpy(buf.data(), d->ev_buf, buf.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-        std::vector<std::pair<uint32_t, uint32_t>> ev;
-        for (uint32_t q = 0; q < cnt; ++q) ev.push_back({buf[2 * q], buf[2 * q + 1]});
-        // canonical: by body index, ejection before hit-centrum (kernel order per body)
-        std::sort(ev.begin(), ev.end(), [](auto& a, auto& b) {
-            return a.first != b.first ? a.first < b.first : a.second > b.second;
-        });
-        HostView hv;
-        if (int rc = fetch(d, k, hv)) return rc;
-        for (auto& q : ev) {
-            const uint32_t li = q.first - h.off;
-            const double r2 = hv.r[li].x * hv.r[li].x + hv.r[li].y * hv.r[li].y + hv.r[li].z * hv.r[li].z;
-            h.pending.push_back(make_event((int)q.second, h.t, std::sqrt(r2), 0, li, hv, d->bmd.data() + h.off));
-        }
+        for (uint32_t q = 0; q < cnt; ++q) pr.push_back({buf[2 * q], buf[2 * q + 1]});
     }
-    if (n) *n = (uint32_t)h.pending.size();
+    return RD_OK;
+}
+
+uint32_t disk_of_body(const rd_disk* d, uint32_t i)
+{
+    uint32_t lo = 0, hi = d->n_disk;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (d->hd[mid].off <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// pp_disk::check_for_ejection_hit_centrum on every disk in `want`
+// (pp_disk.cu:284-318, 1186-1201); events are appended to each disk's pending
+// list in canonical order (body index, ejection before hit-centrum).
+int detect_ejections(rd_disk* d, const std::vector<char>& want)
+{
+    const double ej = d->thr[RD_THRESHOLD_EJECTION_DISTANCE], hc = d->thr[RD_THRESHOLD_HIT_CENTRUM_DISTANCE];
+    const double ed2 = (0.0 < ej ? ej * ej : DBL_MAX);
+    const double hcd2 = hc * hc;
+    if (int rc = flag_check(d, want)) return rc;
+    HIP_TRY(launch_ejection(d->stream, d->n_all, d->x, d->y, d->z, d->id, d->type, d->disk_of, d->ctl, ed2, hcd2,
+                            d->ev_buf, d->ev_counter, d->ev_cap));
+    std::vector<std::pair<uint32_t, uint32_t>> ev;
+    if (int rc = read_pairs(d, ev)) return rc;
+    std::sort(ev.begin(), ev.end(), [](auto& a, auto& b) {
+        return a.first != b.first ? a.first < b.first : a.second > b.second;
+    });
+    uint32_t cur = ~0u;
+    HostView hv;
+    for (auto& q : ev) {
+        const uint32_t k = disk_of_body(d, q.first);
+        HostDisk& h = d->hd[k];
+        if (k != cur) {
+            if (int rc = fetch(d, k, hv)) return rc;
+            cur = k;
+        }
+        const uint32_t li = q.first - h.off;
+        const double r2 = hv.r[li].x * hv.r[li].x + hv.r[li].y * hv.r[li].y + hv.r[li].z * hv.r[li].z;
+        h.pending.push_back(make_event((int)q.second, h.t, std::sqrt(r2), 0, li, hv, d->bmd.data() + h.off));
+    }
+    return RD_OK;
+}
+
+// pp_disk::check_for_collision on every disk in `want` (pp_disk.cu:321-371,
+// 1203-1218); events per disk in (i, j) order.
+int detect_collisions(rd_disk* d, const std::vector<char>& want)
+{
+    if (int rc = ensure_lists(d)) return rc;
+    const double f = d->thr[RD_THRESHOLD_RADII_ENHANCE_FACTOR];
+    if (int rc = flag_check(d, want)) return rc;
+    HIP_TRY(launch_collision(d->stream, d->items_coll, d->n_items_coll,
+                             std::min<uint32_t>(d->n_items_coll, 8u * (uint32_t)d->n_cu), d->x, d->y, d->z,
+                             d->radius, d->id, d->ctl, f * f, d->ev_buf, d->ev_counter, d->ev_cap));
+    std::vector<std::pair<uint32_t, uint32_t>> pr;
+    if (int rc = read_pairs(d, pr)) return rc;
+    std::sort(pr.begin(), pr.end());
+    uint32_t cur = ~0u;
+    HostView hv;
+    for (auto& q : pr) {
+        const uint32_t k = disk_of_body(d, q.first);
+        HostDisk& h = d->hd[k];
+        if (k != cur) {
+            if (int rc = fetch(d, k, hv)) return rc;
+            cur = k;
+        }
+        uint32_t s = q.first - h.off, m = q.second - h.off;
+        const double dx = hv.r[m].x - hv.r[s].x, dy = hv.r[m].y - hv.r[s].y, dz = hv.r[m].z - hv.r[s].z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (hv.p[m].mass > hv.p[s].mass) std::swap(s, m);
+        h.pending.push_back(make_event(RD_EVENT_COLLISION, h.t, std::sqrt(r2), s, m, hv, d->bmd.data() + h.off));
+    }
+    return RD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rd_check_ejection_hit_centrum(rd_disk* d, uint32_t k, uint32_t* n)
+{
+    if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    std::vector<char> want(d->n_disk, 0);
+    want[k] = 1;
+    if (int rc = detect_ejections(d, want)) return rc;
+    if (n) *n = (uint32_t)d->hd[k].pending.size();
     return RD_OK;
 }
 
 int rd_check_collision(rd_disk* d, uint32_t k, uint32_t* n)
 {
     if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
-    if (int rc = ensure_lists(d)) return rc;
-    HostDisk& h = d->hd[k];
-    const double f = d->thr[RD_THRESHOLD_RADII_ENHANCE_FACTOR];
-    HIP_TRY(hipMemsetAsync(d->ev_counter, 0, sizeof(uint32_t), d->stream));
-    // only this disk's tiles: the collision list is grouped by disk
-    HIP_TRY(launch_collision(d->stream, d->items_coll, d->n_items_coll,
-                             std::min<uint32_t>(d->n_items_coll, 8u * (uint32_t)d->n_cu), d->x, d->y, d->z,
-                             d->radius, d->id, f * f, d->ev_buf, d->ev_counter, d->ev_cap));
-    uint32_t cnt = 0;
-    HIP_TRY(hipMemcpyAsync(&cnt, d->ev_counter, sizeof(uint32_t), hipMemcpyDeviceToHost, d->stream));
-    HIP_TRY(hipStreamSynchronize(d->stream));
-    if (cnt > d->ev_cap) return fail(RD_EOVERFLOW, "event buffer overflow: more than " + std::to_string(d->ev_cap) +
-                                                       " colliding pairs");
-    if (cnt) {
-        std::vector<uint32_t> buf(2 * (size_t)cnt);
-        HIP_TRY(hipMemcpy(buf.data(), d->ev_buf, buf.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-        std::vector<std::pair<uint32_t, uint32_t>> pr;
-        for (uint32_t q = 0; q < cnt; ++q)
-            if (buf[2 * q] >= h.off && buf[2 * q] < h.off + h.n) pr.push_back({buf[2 * q], buf[2 * q + 1]});
-        std::sort(pr.begin(), pr.end());
-        if (!pr.empty()) {
-            HostView hv;
-            if (int rc = fetch(d, k, hv)) return rc;
-            for (auto& q : pr) {
-                uint32_t s = q.first - h.off, m = q.second - h.off;
-                const double dx = hv.r[m].x - hv.r[s].x, dy = hv.r[m].y - hv.r[s].y, dz = hv.r[m].z - hv.r[s].z;
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                if (hv.p[m].mass > hv.p[s].mass) std::swap(s, m);
-                h.pending.push_back(make_event(RD_EVENT_COLLISION, h.t, std::sqrt(r2), s, m, hv,
-                                               d->bmd.data() + h.off));
-            }
-        }
-    }
-    if (n) *n = (uint32_t)h.pending.size();
+    std::vector<char> want(d->n_disk, 0);
+    want[k] = 1;
+    if (int rc = detect_collisions(d, want)) return rc;
+    if (n) *n = (uint32_t)d->hd[k].pending.size();
     return RD_OK;
 }
 
@@ -1416,15 +1469,12 @@ int rd_run(rd_integrator* it, double length, double output_interval, uint64_t ma
             }
         }
         if (!any) break;
-        for (uint32_t k = 0; k < nd; ++k) {
-            if (!want[k]) continue;
-            if (0.0 < d->thr[RD_THRESHOLD_EJECTION_DISTANCE] || 0.0 < d->thr[RD_THRESHOLD_HIT_CENTRUM_DISTANCE]) {
-                uint32_t ne = 0;
-                if (int rc = rd_check_ejection_hit_centrum(d, k, &ne)) return rc;
-                if (ne) {
-                    if (int rc = rd_handle_ejection_hit_centrum(d, k)) return rc;
-                    rd_clear_event_counter(d, k);
-                }
+        if (0.0 < d->thr[RD_THRESHOLD_EJECTION_DISTANCE] || 0.0 < d->thr[RD_THRESHOLD_HIT_CENTRUM_DISTANCE]) {
+            if (int rc = detect_ejections(d, want)) return rc;
+            for (uint32_t k = 0; k < nd; ++k) {
+                if (!want[k] || d->hd[k].pending.empty()) continue;
+                if (int rc = rd_handle_ejection_hit_centrum(d, k)) return rc;
+                rd_clear_event_counter(d, k);
             }
         }
         if (int rc = step_disks(it, want)) return rc;
@@ -1436,14 +1486,17 @@ int rd_run(rd_integrator* it, double length, double output_interval, uint64_t ma
             ps[k] += dt;
             if (std::fabs(output_interval) < std::fabs(it->dt_next[k]))
                 it->dt_next[k] = 0.0 < dt ? output_interval : -output_interval;
-            if (0.0 < d->thr[RD_THRESHOLD_RADII_ENHANCE_FACTOR]) {
-                uint32_t ne = 0;
-                if (int rc = rd_check_collision(d, k, &ne)) return rc;
-                if (ne) {
-                    if (int rc = rd_handle_collision(d, k)) return rc;
-                    rd_clear_event_counter(d, k);
-                }
+        }
+        if (0.0 < d->thr[RD_THRESHOLD_RADII_ENHANCE_FACTOR]) {
+            if (int rc = detect_collisions(d, want)) return rc;
+            for (uint32_t k = 0; k < nd; ++k) {
+                if (!want[k] || d->hd[k].pending.empty()) continue;
+                if (int rc = rd_handle_collision(d, k)) return rc;
+                rd_clear_event_counter(d, k);
             }
+        }
+        for (uint32_t k = 0; k < nd; ++k) {
+            if (!want[k]) continue;
             if (output_interval <= std::fabs(ps[k])) ps[k] = 0.0;
             if (16 <= d->hd[k].n_event[1]) {
                 d->hd[k].n_event[1] = 0;

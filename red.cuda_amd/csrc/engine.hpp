@@ -25,6 +25,7 @@ enum : uint32_t {
     F_ACTIVE = 1u,   // disk takes part in this attempt
     F_NEED_K0 = 2u,  // k0 = f(t, y_n) must be (re)computed
     F_COMMIT = 4u,   // copy y_{n+1} -> y_n
+    F_CHECK = 8u,    // event detection runs on this disk
 };
 
 // Per-disk static layout.  Indices are local to the disk except off.
@@ -99,10 +100,11 @@ hipError_t launch_operator(hipStream_t s, const StageArgs& a, double t, uint32_t
                            double* az, const double* vx, const double* vy, const double* vz);
 hipError_t launch_collision(hipStream_t s, const WorkItem* items, uint32_t n_items, uint32_t grid,
                             const double* x, const double* y, const double* z, const double* radius,
-                            const int32_t* id, double f2, uint32_t* pairs, uint32_t* counter, uint32_t cap);
-hipError_t launch_ejection(hipStream_t s, uint32_t lo, uint32_t hi, const double* x, const double* y,
-                           const double* z, const int32_t* id, const int32_t* type, double ed2, double hcd2,
-                           uint32_t* out, uint32_t* counter, uint32_t cap);
+                            const int32_t* id, const DiskCtl* ctl, double f2, uint32_t* pairs, uint32_t* counter,
+                            uint32_t cap);
+hipError_t launch_ejection(hipStream_t s, uint32_t n_all, const double* x, const double* y, const double* z,
+                           const int32_t* id, const int32_t* type, const uint32_t* disk_of, const DiskCtl* ctl,
+                           double ed2, double hcd2, uint32_t* out, uint32_t* counter, uint32_t cap);
 hipError_t launch_integral(hipStream_t s, uint32_t lo, uint32_t hi, const double* x, const double* y,
                            const double* z, const double* vx, const double* vy, const double* vz,
                            const double* mass, const int32_t* id, double* scratch, size_t scratch_len,

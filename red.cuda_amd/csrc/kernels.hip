@@ -756,14 +756,15 @@ __global__ __launch_bounds__(kBlock) void k_collision(const WorkItem* __restrict
                                                       const double* __restrict__ x, const double* __restrict__ y,
                                                       const double* __restrict__ z,
                                                       const double* __restrict__ radius,
-                                                      const int32_t* __restrict__ id, double f2, uint32_t* pairs,
-                                                      uint32_t* counter, uint32_t cap)
+                                                      const int32_t* __restrict__ id, const DiskCtl* __restrict__ ctl,
+                                                      double f2, uint32_t* pairs, uint32_t* counter, uint32_t cap)
 {
     __shared__ double4 tile[kTile];
     __shared__ int32_t tid_ok[kTile];
     const uint32_t tid = threadIdx.x;
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         const WorkItem w = items[it];
+        if (!(ctl[w.disk].flags & F_CHECK)) continue;
         const uint32_t i = w.sink_lo + tid;
         const bool valid = i < w.sink_hi && id[i] > 0;
         const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0, zi = valid ? z[i] : 0.0;
@@ -802,23 +803,25 @@ __global__ __launch_bounds__(kBlock) void k_collision(const WorkItem* __restrict
 
 hipError_t launch_collision(hipStream_t st, const WorkItem* items, uint32_t n_items, uint32_t grid,
                             const double* x, const double* y, const double* z, const double* radius,
-                            const int32_t* id, double f2, uint32_t* pairs, uint32_t* counter, uint32_t cap)
+                            const int32_t* id, const DiskCtl* ctl, double f2, uint32_t* pairs, uint32_t* counter,
+                            uint32_t cap)
 {
     if (n_items == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_collision, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, radius, id, f2, pairs,
-                       counter, cap);
+    hipLaunchKernelGGL(k_collision, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, radius, id, ctl, f2,
+                       pairs, counter, cap);
     return hipGetLastError();
 }
 
-// Ejection / hit-centrum test on r^2 from the origin (pp_disk.cu:284-318).
-// out[2k] = body index, out[2k+1] = event name.
-__global__ __launch_bounds__(kBlock) void k_ejection(uint32_t lo, uint32_t hi, const double* x, const double* y,
+// Ejection / hit-centrum test on r^2 from the origin (pp_disk.cu:284-318) for
+// bodies of disks flagged F_CHECK.  out[2k] = body index, out[2k+1] = event name.
+__global__ __launch_bounds__(kBlock) void k_ejection(uint32_t n_all, const double* x, const double* y,
                                                      const double* z, const int32_t* id, const int32_t* type,
-                                                     double ed2, double hcd2, uint32_t* out, uint32_t* counter,
-                                                     uint32_t cap)
+                                                     const uint32_t* disk_of, const DiskCtl* ctl, double ed2,
+                                                     double hcd2, uint32_t* out, uint32_t* counter, uint32_t cap)
 {
-    const uint32_t i = lo + blockIdx.x * kBlock + threadIdx.x;
-    if (i >= hi) return;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_all) return;
+    if (!(ctl[disk_of[i]].flags & F_CHECK)) return;
     if (!(id[i] > 0 && type[i] != 0)) return;
     const double r2 = SQR(x[i]) + SQR(y[i]) + SQR(z[i]);
     if (ed2 < r2) {
@@ -831,14 +834,14 @@ __global__ __launch_bounds__(kBlock) void k_ejection(uint32_t lo, uint32_t hi, c
     }
 }
 
-hipError_t launch_ejection(hipStream_t st, uint32_t lo, uint32_t hi, const double* x, const double* y,
-                           const double* z, const int32_t* id, const int32_t* type, double ed2, double hcd2,
-                           uint32_t* out, uint32_t* counter, uint32_t cap)
+hipError_t launch_ejection(hipStream_t st, uint32_t n_all, const double* x, const double* y, const double* z,
+                           const int32_t* id, const int32_t* type, const uint32_t* disk_of, const DiskCtl* ctl,
+                           double ed2, double hcd2, uint32_t* out, uint32_t* counter, uint32_t cap)
 {
-    if (hi <= lo) return hipSuccess;
-    const uint32_t grid = (hi - lo + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_ejection, dim3(grid), dim3(kBlock), 0, st, lo, hi, x, y, z, id, type, ed2, hcd2, out,
-                       counter, cap);
+    if (n_all == 0) return hipSuccess;
+    const uint32_t grid = (n_all + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_ejection, dim3(grid), dim3(kBlock), 0, st, n_all, x, y, z, id, type, disk_of, ctl, ed2, hcd2,
+                       out, counter, cap);
     return hipGetLastError();
 }
 

@@ -218,3 +218,57 @@ def test_batch_matches_single_disks(prec):
         assert a[0] == b_[0]
         assert bits_equal(a[1], b_[1]) and bits_equal(a[2], b_[2])
         assert bit.counters(k) == sit.counters()
+
+
+@pytest.mark.parametrize("prec", [R.RD_PREC_FP64, R.RD_PREC_MIXED])
+def test_batch_events_match_single_disks(prec):
+    """Batched driver loop (one launch per stage over all disks, batch-wide
+    event detection) gives every disk the event list and state of its own
+    per-disk run, bit for bit."""
+    # (seed offset, N) of crowded rings whose runs stay valid in the reference's
+    # semantics (some seeds merge one body twice and the reference's rebuild
+    # check refuses them, which the engine reproduces)
+    disks = [(0, 40), (1, 96), (3, 70), (4, 120), (6, 140)]
+    parts = [R.make_disk(n, 50 + k, a_min=0.99, a_max=1.01, sig_e=0.02, sig_i=0.001, r_star=1e-5)
+             for k, n in disks]
+    cat = [np.concatenate([q[i] for q in parts]) for i in range(4)]
+    nbt = np.stack([q[4] for q in parts])
+    thr = (0.2, 50.0, 1000.0)
+    bd = R.Disk(*cat, nbt, threshold=thr, batch=True)
+    bd.set_precision(prec)
+    bit = R.Integrator(bd, R.RD_RKF78, KDAY, adaptive=True, tol=1e-10)
+    bit.run(40.0, 2.0, 40)
+    total = 0
+    for k, q in enumerate(parts):
+        sd = R.Disk(*q, threshold=thr)
+        sd.set_precision(prec)
+        sit = R.Integrator(sd, R.RD_RKF78, KDAY, adaptive=True, tol=1e-10)
+        sit.run(40.0, 2.0, 40)
+        ev_b, ev_s = bd.event_log(k), sd.event_log()
+        assert bits_equal(ev_b, ev_s)
+        total += len(ev_s)
+        a, b_ = bd.state(k), sd.state()
+        assert a[0] == b_[0]
+        for x, y in zip(a[1:], b_[1:]):
+            assert bits_equal(x, y)
+        assert bit.counters(k) == sit.counters()
+    assert total > 16
+
+
+def test_double_merge_refused_like_the_reference():
+    """A ring where one body is merged twice in a step: the reference's rebuild
+    throws 'Number of copied bodies does not equal to the number of active
+    bodies.' (pp_disk.cu:1403-1406); the engine returns RD_EINVAL with that text
+    and the oracle refuses the same run."""
+    r, v, p, b, nbt = R.make_disk(150, 52, a_min=0.99, a_max=1.01, sig_e=0.02, sig_i=0.001, r_star=1e-5)
+    thr = (0.2, 50.0, 1000.0)
+    d = R.Disk(r, v, p, b, nbt, threshold=thr)
+    it = R.Integrator(d, R.RD_RKF78, KDAY, adaptive=True, tol=1e-10)
+    with pytest.raises(R.RedError) as ei:
+        it.run(40.0, 2.0, 40)
+    assert ei.value.code == R.RD_EINVAL and "Number of copied bodies" in str(ei.value)
+    o = O.ODisk(r, v, p, b, nbt, thr=thr)
+    oi = O.OInteg(o, 4, KDAY, adaptive=True, tol=1e-10)
+    rc, _ = oi.run(40.0, 2.0, 40)
+    assert rc == 1
+    assert bits_equal(d.event_log(), o.event_log())
