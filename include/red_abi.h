@@ -217,6 +217,46 @@ int rd_rebuild_vectors(rd_disk* d, uint32_t k);
  * computed on the GPU in FP64 with compensated sums. */
 int rd_calc_integral(rd_disk* d, uint32_t k, rd_integral* out);
 
+/* ---- learned step-size proposer (no reference counterpart) ------------------- */
+/* A 12 -> 64 -> 64 -> 1 MLP (tanh hidden layers, BF16 weights, FP32
+ * accumulation) that proposes log h for the next RKF7(8) step from per-disk
+ * features; the Fehlberg estimate still accepts or rejects every attempt and
+ * a rejection falls back to the reference controller
+ * (int_rungekutta8.cu:693-708).  Features, in order:
+ *   0 log h of the last accepted step      6 log h of the step before
+ *   1 log(err/tol) of that step            7 log(err/tol) of the step before
+ *   2 log min_{i<j} |r_ij|/|v_ij|          8 rejected attempts in the last step
+ *   3 log min periapsis about the star     9 log h proposed by the reference controller
+ *   4 log N                               10 log tol
+ *   5 log10 relative drift of the Keplerian energy since the start
+ *                                         11 change of feature 2 since the last step */
+#define RD_MLP_IN 12
+#define RD_MLP_HIDDEN 64
+#define RD_MLP_NPARAM (RD_MLP_IN * RD_MLP_HIDDEN + RD_MLP_HIDDEN + RD_MLP_HIDDEN * RD_MLP_HIDDEN + RD_MLP_HIDDEN + RD_MLP_HIDDEN + 1)
+typedef struct rd_mlp rd_mlp;
+int rd_mlp_create(int device, uint32_t seed, rd_mlp** out);
+int rd_mlp_destroy(rd_mlp* m);
+/* feature / target standardisation applied inside the network */
+int rd_mlp_set_norm(rd_mlp* m, const float mean[RD_MLP_IN], const float stdev[RD_MLP_IN], float t_mean, float t_std);
+/* FP32 master parameters (layout W1[12][64] b1[64] W2[64][64] b2[64] w3[64] b3) */
+int rd_mlp_set_params(rd_mlp* m, const float* params);
+int rd_mlp_get_params(rd_mlp* m, float* params, uint16_t* params_bf16);
+/* y = log h for n raw feature rows */
+int rd_mlp_forward(rd_mlp* m, const float* x, uint32_t n, float* y);
+/* loss 0.5 mean((y - t)/t_std)^2 and its gradient w.r.t. the FP32 parameters */
+int rd_mlp_loss_grad(rd_mlp* m, const float* x, const float* t, uint32_t n, float* loss, float* grad);
+/* Adam on the GPU over shuffled minibatches */
+int rd_mlp_train(rd_mlp* m, const float* x, const float* t, uint32_t n, uint32_t epochs, uint32_t batch, float lr,
+                 float* final_loss);
+/* use m to propose the first attempt of each step (NULL: reference controller only) */
+int rd_integrator_set_proposer(rd_integrator* it, rd_mlp* m, float safety);
+/* record (features -> target log h) of every accepted step; the target is
+ * log(h * (tol/err)^(1/8)), the step the Fehlberg estimate says would have
+ * just met the tolerance */
+int rd_integrator_set_logging(rd_integrator* it, int on);
+int rd_integrator_log(rd_integrator* it, float* x, float* t, uint32_t cap, uint32_t* n);
+int rd_integrator_clear_log(rd_integrator* it);
+
 /* ---- seeded disks ------------------------------------------------------------- */
 /* Star + n planetesimals from orbital elements, barycentric, tau units
  * (src/red.cuda.initial/main.cu:251-342, 1662-1670; samplers

@@ -22,6 +22,7 @@
 
 #include "../../include/red_abi.h"
 #include "engine.hpp"
+#include "mlp.hpp"
 
 using namespace rdamd;
 
@@ -34,6 +35,14 @@ int fail(int code, const std::string& msg)
     g_err = msg;
     return code;
 }
+
+}  // namespace
+
+namespace rdamd {
+void set_last_error(const std::string& msg) { g_err = msg; }
+}  // namespace rdamd
+
+namespace {
 
 #define HIP_TRY(expr)                                                                          \
     do {                                                                                       \
@@ -109,6 +118,8 @@ struct rd_disk {
     double* scratch = nullptr;
     size_t scratch_len = 0;
     double* red_out = nullptr;     // 12 doubles for integrals
+    uint32_t* feat = nullptr;      // step-proposer features, 3 per disk
+    double* esum = nullptr;        // Keplerian energy per disk
 
     // work lists
     WorkItem* items_force = nullptr;
@@ -152,6 +163,21 @@ struct rd_integrator {
     hipGraph_t graph = nullptr;
     uint64_t graph_gen = ~0ull;
     int graph_kind = -1;
+    // learned step proposer
+    struct Hist {
+        double h1 = 0, h2 = 0, e1 = 1, e2 = 1, tenc_prev = 0, ek0 = 0;
+        uint32_t rej = 0;
+        bool ek0_set = false;
+        int accepted = 0;
+        bool has_feat = false;
+        float feat[RD_MLP_IN];
+    };
+    std::vector<Hist> hist;
+    rd_mlp* proposer = nullptr;
+    float safety = 1.0f;
+    bool logging = false;
+    std::vector<float> log_x, log_t;
+    uint64_t proposals = 0;
 };
 
 namespace {
@@ -181,6 +207,8 @@ void free_buffers(rd_disk* d)
     dev_free(d->items_coll);
     dev_free(d->ev_buf);
     dev_free(d->ev_counter);
+    dev_free(d->feat);
+    dev_free(d->esum);
     if (d->h_ctl) (void)hipHostFree(d->h_ctl);
     if (d->h_err) (void)hipHostFree(d->h_err);
     d->h_ctl = nullptr;
@@ -208,6 +236,8 @@ int alloc_buffers(rd_disk* d)
     if (int rc = dev_alloc(&d->ctl, d->n_disk)) return rc;
     if (int rc = dev_alloc(&d->err, d->n_disk)) return rc;
     if (int rc = dev_alloc(&d->red_out, 16)) return rc;
+    if (int rc = dev_alloc(&d->feat, 3 * (size_t)d->n_disk)) return rc;
+    if (int rc = dev_alloc(&d->esum, d->n_disk)) return rc;
     d->ev_cap = std::max<uint32_t>(4096u, (uint32_t)n);
     if (int rc = dev_alloc(&d->ev_buf, 2 * (size_t)d->ev_cap)) return rc;
     if (int rc = dev_alloc(&d->ev_counter, 1)) return rc;
@@ -451,6 +481,17 @@ hipError_t enqueue_force(rd_disk* d, const double* fx, const double* fy, const d
 }
 
 int last_stage(const rd_integrator* it) { return it->n_stage - 1; }
+
+// Uploads ctl with F_CHECK set on the disks in `on` (t and h kept).
+int flag_check_only(rd_disk* d, const std::vector<char>& on)
+{
+    for (uint32_t q = 0; q < d->n_disk; ++q) {
+        d->h_ctl[q].t = d->hd[q].t;
+        d->h_ctl[q].flags = on[q] ? F_CHECK : 0u;
+    }
+    HIP_TRY(hipMemcpyAsync(d->ctl, d->h_ctl, d->n_disk * sizeof(DiskCtl), hipMemcpyHostToDevice, d->stream));
+    return RD_OK;
+}
 
 // One attempt for every disk whose ctl flags say so.
 hipError_t enqueue_attempt(rd_integrator* it)
@@ -878,6 +919,7 @@ int rd_integrator_create(rd_disk* d, int method, double dt, int adaptive, double
     it->tried.assign(d->n_disk, 0);
     it->passed.assign(d->n_disk, 0);
     it->failed.assign(d->n_disk, 0);
+    it->hist.assign(d->n_disk, rd_integrator::Hist{});
     *out = it;
     return RD_OK;
 }
@@ -888,6 +930,93 @@ int rd_integrator_destroy(rd_integrator* it)
     if (it->gexec) (void)hipGraphExecDestroy(it->gexec);
     if (it->graph) (void)hipGraphDestroy(it->graph);
     delete it;
+    return RD_OK;
+}
+
+// Features of the disks that just accepted a step (state y_{n+1}), the
+// logged sample of the step just taken, and the proposer's first try for the
+// next step.
+static int after_accept(rd_integrator* it, const std::vector<char>& acc, const std::vector<double>& err_ratio,
+                        const std::vector<int>& iters)
+{
+    rd_disk* d = it->d;
+    const uint32_t nd = d->n_disk;
+    for (uint32_t k = 0; k < nd; ++k) {
+        if (!acc[k]) continue;
+        rd_integrator::Hist& H = it->hist[k];
+        // sample of the step just accepted: features at its start -> the step
+        // that would just have met the tolerance
+        if (it->logging && H.has_feat) {
+            const double er = std::max(err_ratio[k], 1.0e-12);
+            const double target = std::log(it->dt_did[k]) + std::log(1.0 / er) / 8.0;
+            it->log_x.insert(it->log_x.end(), H.feat, H.feat + RD_MLP_IN);
+            it->log_t.push_back((float)target);
+        }
+        H.h2 = H.accepted ? H.h1 : it->dt_did[k];
+        H.e2 = H.accepted ? H.e1 : err_ratio[k];
+        H.h1 = it->dt_did[k];
+        H.e1 = err_ratio[k];
+        H.rej = (uint32_t)(iters[k] - 1);
+        H.accepted++;
+    }
+    if (!it->proposer && !it->logging) return RD_OK;
+    if (int rc = flag_check_only(d, acc)) return rc;
+    HIP_TRY(hipMemsetAsync(d->feat, 0xff, 3 * (size_t)nd * sizeof(uint32_t), d->stream));
+    HIP_TRY(hipMemsetAsync(d->esum, 0, nd * sizeof(double), d->stream));
+    HIP_TRY(launch_features(d->stream, d->items_coll, d->n_items_coll,
+                            std::min<uint32_t>(d->n_items_coll, 8u * (uint32_t)d->n_cu), d->n_all, d->xo, d->yo, d->zo,
+                            d->vxo, d->vyo, d->vzo, d->mass, d->id, d->disk_of, d->info, d->ctl, d->feat, d->esum));
+    std::vector<uint32_t> f(3 * (size_t)nd);
+    std::vector<double> es(nd);
+    HIP_TRY(hipMemcpyAsync(f.data(), d->feat, f.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipMemcpyAsync(es.data(), d->esum, nd * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    std::vector<float> X;
+    std::vector<uint32_t> who;
+    for (uint32_t k = 0; k < nd; ++k) {
+        if (!acc[k]) continue;
+        rd_integrator::Hist& H = it->hist[k];
+        float r2v2, qmin;
+        std::memcpy(&r2v2, &f[3 * k], 4);
+        std::memcpy(&qmin, &f[3 * k + 1], 4);
+        if (!(r2v2 < 1e38f)) r2v2 = 1e38f;
+        if (!(qmin < 1e38f)) qmin = 1e38f;
+        if (!H.ek0_set) {
+            H.ek0 = es[k];
+            H.ek0_set = true;
+        }
+        const double tenc = 0.5 * std::log(std::max((double)r2v2, 1e-60));
+        float* F = H.feat;
+        F[0] = (float)std::log(H.h1);
+        F[1] = (float)std::log(std::max(H.e1, 1e-30));
+        F[2] = (float)tenc;
+        F[3] = (float)std::log(std::max((double)qmin, 1e-30));
+        F[4] = (float)std::log((double)d->hd[k].n);
+        F[5] = (float)std::log10(std::fabs(es[k] - H.ek0) / std::max(std::fabs(H.ek0), 1e-300) + 1e-20);
+        F[6] = (float)std::log(H.h2);
+        F[7] = (float)std::log(std::max(H.e2, 1e-30));
+        F[8] = (float)H.rej;
+        F[9] = (float)std::log(std::fabs(it->dt_next[k]));
+        F[10] = (float)std::log(it->tol);
+        F[11] = (float)(H.tenc_prev != 0.0 ? tenc - H.tenc_prev : 0.0);
+        H.tenc_prev = tenc;
+        H.has_feat = true;
+        X.insert(X.end(), F, F + RD_MLP_IN);
+        who.push_back(k);
+    }
+    if (it->proposer && !who.empty()) {
+        std::vector<float> y(who.size());
+        if (int rc = rd_mlp_forward(it->proposer, X.data(), (uint32_t)who.size(), y.data())) return rc;
+        for (size_t q = 0; q < who.size(); ++q) {
+            const uint32_t k = who[q];
+            const double hc = it->dt_next[k];
+            double hp = it->safety * std::exp((double)y[q]);
+            if (!(hp > 0.0) || !std::isfinite(hp)) continue;
+            hp = std::min(std::max(hp, std::fabs(hc) / 8.0), 8.0 * std::fabs(hc));
+            it->dt_next[k] = hc < 0.0 ? -hp : hp;
+            it->proposals++;
+        }
+    }
     return RD_OK;
 }
 
@@ -906,6 +1035,8 @@ static int step_disks(rd_integrator* it, const std::vector<char>& want)
     const double lambda = it->method == RD_RKF78 ? 41.0 / 840.0 : 1.0 / 10.0;
     uint32_t commit_flags_pending = 0;
     std::vector<uint32_t> commit(nd, 0);
+    std::vector<char> acc(nd, 0);
+    std::vector<double> err_ratio(nd, 0.0);
     for (;;) {
         bool any = false;
         for (uint32_t k = 0; k < nd; ++k) {
@@ -958,6 +1089,7 @@ static int step_disks(rd_integrator* it, const std::vector<char>& want)
                 else it->dt_next[k] = it->dt_try[k] *= 2.0;
             }
             iter[k]++;
+            err_ratio[k] = it->adaptive ? max_err / it->tol : 1.0;
             if (it->adaptive && max_err > it->tol) {
                 need_k0[k] = 0;       // k0 is reused (int_rungekutta8.cu:648-656)
             } else {
@@ -968,9 +1100,14 @@ static int step_disks(rd_integrator* it, const std::vector<char>& want)
                 pending[k] = 0;
                 commit[k] = F_COMMIT;
                 commit_flags_pending = 1;
+                acc[k] = 1;
             }
         }
         (void)n_act;
+        if (commit_flags_pending && it->adaptive) {
+            if (int rc = after_accept(it, acc, err_ratio, iter)) return rc;
+            std::fill(acc.begin(), acc.end(), 0);
+        }
     }
     HIP_TRY(hipStreamSynchronize(d->stream));
     return RD_OK;
@@ -1056,6 +1193,42 @@ int rd_set_error_check_for_tp(rd_integrator* it, int on)
 {
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     it->err_all = on ? 1 : 0;
+    return RD_OK;
+}
+
+int rd_integrator_set_proposer(rd_integrator* it, rd_mlp* m, float safety)
+{
+    if (!it || !(safety > 0.0f)) return fail(RD_EINVAL, "bad argument");
+    if (m && !it->adaptive) return fail(RD_ESTATE, "the proposer needs an adaptive integrator");
+    it->proposer = m;
+    it->safety = safety;
+    return RD_OK;
+}
+
+int rd_integrator_set_logging(rd_integrator* it, int on)
+{
+    if (!it) return fail(RD_EINVAL, "NULL integrator");
+    if (on && !it->adaptive) return fail(RD_ESTATE, "logging needs an adaptive integrator");
+    it->logging = on != 0;
+    return RD_OK;
+}
+
+int rd_integrator_log(rd_integrator* it, float* x, float* t, uint32_t cap, uint32_t* n)
+{
+    if (!it) return fail(RD_EINVAL, "NULL integrator");
+    const size_t cnt = it->log_t.size();
+    const size_t m = std::min<size_t>(cap, cnt);
+    if (x) std::memcpy(x, it->log_x.data(), m * RD_MLP_IN * sizeof(float));
+    if (t) std::memcpy(t, it->log_t.data(), m * sizeof(float));
+    if (n) *n = (uint32_t)cnt;
+    return RD_OK;
+}
+
+int rd_integrator_clear_log(rd_integrator* it)
+{
+    if (!it) return fail(RD_EINVAL, "NULL integrator");
+    it->log_x.clear();
+    it->log_t.clear();
     return RD_OK;
 }
 

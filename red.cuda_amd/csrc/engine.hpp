@@ -111,6 +111,13 @@ hipError_t launch_integral(hipStream_t s, uint32_t lo, uint32_t hi, const double
                            double* out /* 11 doubles: M, Rx,Ry,Rz, Vx,Vy,Vz, Cx,Cy,Cz, T */,
                            double* out_u);
 hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes);
+// Step-proposer features of disks flagged F_CHECK, from state (x..vz):
+// feat[3*d + 0] = min over pairs of r^2 / v^2 (float bits, atomicMin),
+// feat[3*d + 1] = min periapsis (float bits, atomicMin), esum[d] += Keplerian energy.
+hipError_t launch_features(hipStream_t s, const WorkItem* items, uint32_t n_items, uint32_t grid, uint32_t n_all,
+                           const double* x, const double* y, const double* z, const double* vx, const double* vy,
+                           const double* vz, const double* mass, const int32_t* id, const uint32_t* disk_of,
+                           const DiskInfo* info, const DiskCtl* ctl, uint32_t* feat, double* esum);
 hipError_t launch_set_sources(hipStream_t s, uint32_t n_all, const double* mass, const int32_t* id,
                               const uint32_t* disk_of, const DiskInfo* info, double* m_ex, float* m32);
 

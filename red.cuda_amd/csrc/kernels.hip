@@ -952,6 +952,100 @@ hipError_t launch_integral(hipStream_t st, uint32_t lo, uint32_t hi, const doubl
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Step-proposer features
+// ---------------------------------------------------------------------------
+// min over active pairs i < j of |r_ij|^2 / |v_ij|^2 (FP32), per disk flagged F_CHECK.
+__global__ __launch_bounds__(kBlock) void k_feat_pairs(const WorkItem* __restrict__ items, uint32_t n_items,
+                                                       const double* __restrict__ x, const double* __restrict__ y,
+                                                       const double* __restrict__ z, const double* __restrict__ vx,
+                                                       const double* __restrict__ vy, const double* __restrict__ vz,
+                                                       const int32_t* __restrict__ id, const DiskCtl* __restrict__ ctl,
+                                                       uint32_t* __restrict__ feat)
+{
+    __shared__ double4 tp[kTile];
+    __shared__ float4 tv[kTile];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const WorkItem w = items[it];
+        if (!(ctl[w.disk].flags & F_CHECK)) continue;
+        const uint32_t i = w.sink_lo + tid;
+        const bool valid = i < w.sink_hi && id[i] > 0;
+        const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0, zi = valid ? z[i] : 0.0;
+        const float vxi = valid ? (float)vx[i] : 0.0f, vyi = valid ? (float)vy[i] : 0.0f,
+                    vzi = valid ? (float)vz[i] : 0.0f;
+        float best = 3.0e38f;
+        for (uint32_t t0 = w.src_lo; t0 < w.src_hi; t0 += kTile) {
+            __syncthreads();
+            const uint32_t j = t0 + tid;
+            if (j < w.src_hi) {
+                tp[tid] = make_double4(x[j], y[j], z[j], id[j] > 0 ? 1.0 : 0.0);
+                tv[tid] = make_float4((float)vx[j], (float)vy[j], (float)vz[j], 0.0f);
+            }
+            __syncthreads();
+            const uint32_t nt = min((uint32_t)kTile, w.src_hi - t0);
+            if (valid) {
+                for (uint32_t k = 0; k < nt; ++k) {
+                    const uint32_t jj = t0 + k;
+                    const double4 pj = tp[k];
+                    if (jj <= i || pj.w == 0.0) continue;
+                    const float dx = (float)(pj.x - xi), dy = (float)(pj.y - yi), dz = (float)(pj.z - zi);
+                    const float ux = tv[k].x - vxi, uy = tv[k].y - vyi, uz = tv[k].z - vzi;
+                    const float r2 = dx * dx + dy * dy + dz * dz;
+                    const float u2 = ux * ux + uy * uy + uz * uz + 1.0e-30f;
+                    best = fminf(best, r2 / u2);
+                }
+            }
+        }
+        // positive floats order like their bit patterns
+        for (int off = 32; off > 0; off >>= 1) best = fminf(best, __shfl_xor(best, off, 64));
+        if ((tid & 63u) == 0) atomicMin(&feat[3 * w.disk + 0], __float_as_uint(best));
+        __syncthreads();
+    }
+}
+
+// min periapsis about the disk's star and the sum of Keplerian energies
+// m (v^2/2 - mu/r), mu = m_star + m, relative to the star (body 0 of the disk).
+__global__ __launch_bounds__(kBlock) void k_feat_bodies(uint32_t n_all, const double* x, const double* y,
+                                                        const double* z, const double* vx, const double* vy,
+                                                        const double* vz, const double* mass, const int32_t* id,
+                                                        const uint32_t* disk_of, const DiskInfo* info,
+                                                        const DiskCtl* ctl, uint32_t* feat, double* esum)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_all) return;
+    const uint32_t d = disk_of[i];
+    if (!(ctl[d].flags & F_CHECK)) return;
+    const uint32_t s = info[d].off;
+    if (i == s || !(id[i] > 0)) return;
+    const double rx = x[i] - x[s], ry = y[i] - y[s], rz = z[i] - z[s];
+    const double ux = vx[i] - vx[s], uy = vy[i] - vy[s], uz = vz[i] - vz[s];
+    const double mu = mass[s] + mass[i];
+    const double r = sqrt(rx * rx + ry * ry + rz * rz);
+    const double v2 = ux * ux + uy * uy + uz * uz;
+    const double eps = 0.5 * v2 - mu / r;
+    const double hx = ry * uz - rz * uy, hy = rz * ux - rx * uz, hz = rx * uy - ry * ux;
+    const double h2 = hx * hx + hy * hy + hz * hz;
+    const double e = sqrt(fmax(0.0, 1.0 + 2.0 * eps * h2 / (mu * mu)));
+    const double q = h2 / (mu * (1.0 + e));
+    atomicMin(&feat[3 * d + 1], __float_as_uint((float)q));
+    atomicAdd(&esum[d], mass[i] * eps);
+}
+
+hipError_t launch_features(hipStream_t st, const WorkItem* items, uint32_t n_items, uint32_t grid, uint32_t n_all,
+                           const double* x, const double* y, const double* z, const double* vx, const double* vy,
+                           const double* vz, const double* mass, const int32_t* id, const uint32_t* disk_of,
+                           const DiskInfo* info, const DiskCtl* ctl, uint32_t* feat, double* esum)
+{
+    if (n_items)
+        hipLaunchKernelGGL(k_feat_pairs, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, vx, vy, vz, id, ctl,
+                           feat);
+    if (n_all)
+        hipLaunchKernelGGL(k_feat_bodies, dim3((n_all + kBlock - 1) / kBlock), dim3(kBlock), 0, st, n_all, x, y, z,
+                           vx, vy, vz, mass, id, disk_of, info, ctl, feat, esum);
+    return hipGetLastError();
+}
+
 // Plain device copy used to calibrate hardware byte counters (FETCH_SIZE,
 // WRITE_SIZE) against a known traffic volume.
 __global__ __launch_bounds__(kBlock) void k_copy(const double4* __restrict__ src, double4* __restrict__ dst, size_t n)

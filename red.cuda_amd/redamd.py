@@ -83,6 +83,18 @@ _SIGS = {
     "rd_rebuild_vectors": (C.c_int, [_P, _u32]),
     "rd_calc_integral": (C.c_int, [_P, _u32, _P]),
     "rd_make_disk": (C.c_int, [C.POINTER(DiskSpec), _P, _P, _P, _P]),
+    "rd_mlp_create": (C.c_int, [C.c_int, _u32, C.POINTER(_P)]),
+    "rd_mlp_destroy": (C.c_int, [_P]),
+    "rd_mlp_set_norm": (C.c_int, [_P, _P, _P, C.c_float, C.c_float]),
+    "rd_mlp_set_params": (C.c_int, [_P, _P]),
+    "rd_mlp_get_params": (C.c_int, [_P, _P, _P]),
+    "rd_mlp_forward": (C.c_int, [_P, _P, _u32, _P]),
+    "rd_mlp_loss_grad": (C.c_int, [_P, _P, _P, _u32, C.POINTER(C.c_float), _P]),
+    "rd_mlp_train": (C.c_int, [_P, _P, _P, _u32, _u32, _u32, C.c_float, C.POINTER(C.c_float)]),
+    "rd_integrator_set_proposer": (C.c_int, [_P, _P, C.c_float]),
+    "rd_integrator_set_logging": (C.c_int, [_P, C.c_int]),
+    "rd_integrator_log": (C.c_int, [_P, _P, _P, _u32, C.POINTER(_u32)]),
+    "rd_integrator_clear_log": (C.c_int, [_P]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _f = getattr(lib, _name)
@@ -353,3 +365,80 @@ class Integrator:
         e, p = _u64(), _f64()
         check(lib.rd_work_counters(self.h, C.byref(e), C.byref(p)))
         return e.value, p.value
+
+    def set_proposer(self, mlp, safety=1.0):
+        check(lib.rd_integrator_set_proposer(self.h, mlp.h if mlp is not None else None, safety))
+        self._mlp = mlp
+
+    def set_logging(self, on):
+        check(lib.rd_integrator_set_logging(self.h, 1 if on else 0))
+
+    def log(self):
+        n = _u32()
+        check(lib.rd_integrator_log(self.h, None, None, 0, C.byref(n)))
+        x = np.zeros((n.value, MLP_IN), np.float32)
+        t = np.zeros(n.value, np.float32)
+        check(lib.rd_integrator_log(self.h, _ptr(x), _ptr(t), n.value, C.byref(n)))
+        return x, t
+
+    def clear_log(self):
+        check(lib.rd_integrator_clear_log(self.h))
+
+
+MLP_IN, MLP_HIDDEN = 12, 64
+MLP_NPARAM = MLP_IN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN + 1
+
+
+class Mlp:
+    """The step-size proposer network (12 -> 64 -> 64 -> 1, BF16 weights)."""
+
+    def __init__(self, seed=0, device=0):
+        self.h = C.c_void_p()
+        check(lib.rd_mlp_create(device, seed, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib.rd_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_norm(self, mean, std, t_mean, t_std):
+        m = np.ascontiguousarray(mean, np.float32)
+        s = np.ascontiguousarray(std, np.float32)
+        check(lib.rd_mlp_set_norm(self.h, _ptr(m), _ptr(s), float(t_mean), float(t_std)))
+
+    def params(self):
+        w = np.zeros(MLP_NPARAM, np.float32)
+        wb = np.zeros(MLP_NPARAM, np.uint16)
+        check(lib.rd_mlp_get_params(self.h, _ptr(w), _ptr(wb)))
+        return w, wb
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        check(lib.rd_mlp_set_params(self.h, _ptr(w)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros(len(x), np.float32)
+        check(lib.rd_mlp_forward(self.h, _ptr(x), len(x), _ptr(y)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = C.c_float()
+        check(lib.rd_mlp_loss_grad(self.h, _ptr(x), _ptr(t), len(x), C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def train(self, x, t, epochs=10, batch=256, lr=1e-3):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = C.c_float()
+        check(lib.rd_mlp_train(self.h, _ptr(x), _ptr(t), len(x), epochs, batch, lr, C.byref(loss)))
+        return loss.value
