@@ -31,6 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "red.cuda_amd"))
 import numpy as np  # noqa: E402
 
 import redamd as R  # noqa: E402
+from proposer import batch_sizes  # noqa: E402
 
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
@@ -150,17 +151,6 @@ def bench_config3(args):
     emit(out)
 
 
-def batch_sizes(n_disk=512, lo=64, hi=4096, seed=4):
-    """N_d uniform on [lo, hi] from MT19937(seed) raw outputs, as red_random::uniform(int, int)
-    (src/red.cuda.initial/distribution.cpp:43-46)."""
-    rs = np.random.RandomState(seed)
-    out = []
-    for _ in range(n_disk):
-        u = int.from_bytes(rs.bytes(4), "little")
-        out.append(lo + int(u / 4294967296.0 * (hi - lo + 1)))
-    return out
-
-
 def bench_config4(args):
     """Config 4: 512 independent disks, N_d ~ U[64, 4096], disk d seeded 4000 + d,
     one persistent launch per stage over all disks, attempts replayed from a hipGraph."""
@@ -197,6 +187,80 @@ def bench_config4(args):
     emit(out)
 
 
+def rk_evals(res):
+    """Force evaluations summed over disks: RKF7(8) spends 12 per attempt plus
+    k0 once per step (int_rungekutta8.cu:648-656)."""
+    return 12 * res["tried"] + res["passed"]
+
+
+def bench_config5(args):
+    """Config 5: the learned step proposer. Samples are logged from seeded runs of
+    a 512-disk batch (N_d ~ U[64, 4096], disk k seeded 7000 + k), the MLP is fitted
+    on the GPU, and the ablation runs on held-out disks: the config-4 batch (seeds
+    4000 + k) and the config-2 disk (N = 16384, seed 2), each with the classical
+    controller (mixed and plain FP64) and with the proposer."""
+    import proposer as P
+    sizes = P.batch_sizes()
+    tol = args.tol
+    dt0 = 0.1 * R.K_GAUSS
+    log_steps = args.log_steps
+    print(f"# logging {log_steps} steps x {len(sizes)} disks", file=sys.stderr, flush=True)
+    d = P.make_batch(sizes, 7000)
+    t0 = time.perf_counter()
+    X, T = P.collect(d, log_steps, dt0, tol)
+    t_log = time.perf_counter() - t0
+    d.close()
+    print(f"# {len(T)} samples in {t_log:.1f} s; training", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    m, loss = P.fit(X, T, epochs=args.epochs)
+    t_fit = time.perf_counter() - t0
+    rms = float(np.sqrt(np.mean((m.forward(X) - T) ** 2)))
+    del X, T
+    steps = args.steps if args.steps != 1000 else 100
+    probe = range(0, len(sizes), 32)
+    res = {}
+    for name, prec, prop in (("classical_fp64", R.RD_PREC_FP64, None), ("classical_mixed", R.RD_PREC_MIXED, None),
+                             ("learned_mixed", R.RD_PREC_MIXED, m)):
+        print(f"# batch {name}", file=sys.stderr, flush=True)
+        d = P.make_batch(sizes, 4000, precision=prec)
+        res[name] = P.run(d, steps=steps, proposer=prop, safety=args.safety, dt0=dt0, tol=tol, energy_disks=probe)
+        res[name]["rk_evals"] = rk_evals(res[name])
+        res[name]["disk_steps_per_s"] = res[name]["passed"] / res[name]["seconds"]
+        d.close()
+    single = {}
+    r, v, p, b, nbt = R.make_disk(16384, 2)
+    for name, prec, prop in (("classical_fp64", R.RD_PREC_FP64, None), ("classical_mixed", R.RD_PREC_MIXED, None),
+                             ("learned_mixed", R.RD_PREC_MIXED, m)):
+        print(f"# N=16384 disk {name}", file=sys.stderr, flush=True)
+        d = R.Disk(r, v, p, b, nbt)
+        d.set_precision(prec)
+        d.set_tuning(use_graph=1)
+        single[name] = P.run(d, length=args.c5_length, proposer=prop, safety=args.safety, dt0=0.05 * R.K_GAUSS,
+                             tol=tol)
+        single[name]["rk_evals"] = rk_evals(single[name])
+        d.close()
+    value = res["classical_fp64"]["rk_evals"] / res["learned_mixed"]["rk_evals"]
+    out = {"metric": "RK evaluations, plain FP64 RKF78 / learned proposer (config 5)", "value": value,
+           "unit": "x fewer RK evaluations", "n_gpus": 1, "steps": steps, "warmup": 0,
+           "ms_per_step": res["learned_mixed"]["seconds"] * 1e3 / steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64 state / f32 pair force / bf16 MLP weights",
+           "data": "synthetic seeded disks",
+           "config": {"workload": f"proposer trained on {log_steps} logged steps x {len(sizes)} disks (seeds 7000+k), "
+                                  f"ablation on the 512-disk batch (seeds 4000+k, {steps} steps) and the "
+                                  f"N=16384 disk (seed 2, {args.c5_length:g} time units), RKF7(8) tol {tol:g}",
+                      "safety": args.safety, "epochs": args.epochs},
+           "training": {"samples": int(log_steps * len(sizes)), "log_seconds": t_log, "fit_seconds": t_fit,
+                        "final_loss": loss, "rms_log_h": rms},
+           "batch": res, "single_disk": single,
+           "ratios": {"batch_evals_fp64_over_learned": value,
+                      "batch_evals_mixed_over_learned": res["classical_mixed"]["rk_evals"] / res["learned_mixed"]["rk_evals"],
+                      "batch_time_mixed_over_learned": res["classical_mixed"]["seconds"] / res["learned_mixed"]["seconds"],
+                      "single_evals_fp64_over_learned": single["classical_fp64"]["rk_evals"] / single["learned_mixed"]["rk_evals"],
+                      "single_evals_mixed_over_learned": single["classical_mixed"]["rk_evals"] / single["learned_mixed"]["rk_evals"],
+                      "single_time_mixed_over_learned": single["classical_mixed"]["seconds"] / single["learned_mixed"]["seconds"]}}
+    emit(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,9 +273,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", default="", help="S,CHUNK,BLOCKS_PER_CU for the mixed kernel (skips autotune)")
-    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4],
-                    help="BASELINE config: 1 RK4 N=256, 2 RKF78 N=16384 (default), 3 gas+collisions N=131072, "
-                         "4 batch of 512 disks")
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5],
+                    help="1 RK4 N=256, 2 RKF78 N=16384 (default), 3 gas+collisions N=131072, "
+                         "4 batch of 512 disks, 5 learned step proposer")
+    ap.add_argument("--log-steps", type=int, default=2000, help="config 5: logged steps per disk")
+    ap.add_argument("--epochs", type=int, default=10, help="config 5: training epochs")
+    ap.add_argument("--safety", type=float, default=0.85, help="config 5: proposer safety factor")
+    ap.add_argument("--c5-length", type=float, default=8.0, help="config 5: single-disk run length (time units)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run the byte-counter calibration copy of this size and exit")
     args = ap.parse_args()
@@ -230,6 +298,8 @@ def main():
         return bench_config3(args)
     if args.config == 4:
         return bench_config4(args)
+    if args.config == 5:
+        return bench_config5(args)
 
     r, v, p, b, nbt = R.make_disk(args.n, args.seed)
     # initial step as redinit: min orbital period / 1000 (src/red.cuda.initial/main.cu:317-342)

@@ -120,12 +120,17 @@ struct rd_disk {
     double* red_out = nullptr;     // 12 doubles for integrals
     uint32_t* feat = nullptr;      // step-proposer features, 3 per disk
     double* esum = nullptr;        // Keplerian energy per disk
+    double* eterm = nullptr;       // per-body Keplerian energy [n_all]
+    float* fx = nullptr;           // proposer input rows [n_disk][12]
+    float* fy = nullptr;           // proposer output [n_disk]
 
     // work lists
     WorkItem* items_force = nullptr;
     uint32_t n_items_force = 0, grid_force = 0;
     WorkItem* items_coll = nullptr;
     uint32_t n_items_coll = 0;
+    WorkItem* items_feat = nullptr;
+    uint32_t n_items_feat = 0;
     bool lists_valid = false;
     double pairs_per_eval = 0.0;
 
@@ -137,6 +142,10 @@ struct rd_disk {
     // pinned host mirrors
     DiskCtl* h_ctl = nullptr;
     unsigned long long* h_err = nullptr;
+    uint32_t* h_feat = nullptr;    // [3 * n_disk]
+    double* h_esum = nullptr;      // [n_disk]
+    float* h_fx = nullptr;         // [n_disk][12]
+    float* h_fy = nullptr;         // [n_disk]
 
     std::vector<DiskInfo> h_info;
 
@@ -192,7 +201,7 @@ void free_buffers(rd_disk* d)
 {
     double** dp[] = {&d->x, &d->y, &d->z, &d->vx, &d->vy, &d->vz, &d->xo, &d->yo, &d->zo, &d->vxo, &d->vyo,
                      &d->vzo, &d->K, &d->part, &d->mass, &d->radius, &d->density, &d->cd, &d->m_ex, &d->scratch,
-                     &d->red_out};
+                     &d->red_out, &d->eterm};
     for (auto p : dp) dev_free(*p);
     for (int b = 0; b < 2; ++b)
         for (int c = 0; c < 3; ++c) dev_free(d->P[b][c]);
@@ -205,14 +214,18 @@ void free_buffers(rd_disk* d)
     dev_free(d->err);
     dev_free(d->items_force);
     dev_free(d->items_coll);
+    dev_free(d->items_feat);
     dev_free(d->ev_buf);
     dev_free(d->ev_counter);
     dev_free(d->feat);
     dev_free(d->esum);
-    if (d->h_ctl) (void)hipHostFree(d->h_ctl);
-    if (d->h_err) (void)hipHostFree(d->h_err);
-    d->h_ctl = nullptr;
-    d->h_err = nullptr;
+    dev_free(d->fx);
+    dev_free(d->fy);
+    for (void** h : {(void**)&d->h_ctl, (void**)&d->h_err, (void**)&d->h_feat, (void**)&d->h_esum, (void**)&d->h_fx,
+                     (void**)&d->h_fy}) {
+        if (*h) (void)hipHostFree(*h);
+        *h = nullptr;
+    }
     d->part_len = 0;
     d->scratch_len = 0;
 }
@@ -221,7 +234,7 @@ int alloc_buffers(rd_disk* d)
 {
     const size_t n = d->n_all;
     double** dp[] = {&d->x, &d->y, &d->z, &d->vx, &d->vy, &d->vz, &d->xo, &d->yo, &d->zo, &d->vxo, &d->vyo,
-                     &d->vzo, &d->mass, &d->radius, &d->density, &d->cd, &d->m_ex};
+                     &d->vzo, &d->mass, &d->radius, &d->density, &d->cd, &d->m_ex, &d->eterm};
     for (auto p : dp)
         if (int rc = dev_alloc(p, n)) return rc;
     for (int b = 0; b < 2; ++b)
@@ -238,11 +251,17 @@ int alloc_buffers(rd_disk* d)
     if (int rc = dev_alloc(&d->red_out, 16)) return rc;
     if (int rc = dev_alloc(&d->feat, 3 * (size_t)d->n_disk)) return rc;
     if (int rc = dev_alloc(&d->esum, d->n_disk)) return rc;
+    if (int rc = dev_alloc(&d->fx, (size_t)RD_MLP_IN * d->n_disk)) return rc;
+    if (int rc = dev_alloc(&d->fy, d->n_disk)) return rc;
     d->ev_cap = std::max<uint32_t>(4096u, (uint32_t)n);
     if (int rc = dev_alloc(&d->ev_buf, 2 * (size_t)d->ev_cap)) return rc;
     if (int rc = dev_alloc(&d->ev_counter, 1)) return rc;
     HIP_TRY(hipHostMalloc((void**)&d->h_ctl, sizeof(DiskCtl) * std::max<uint32_t>(1, d->n_disk)));
     HIP_TRY(hipHostMalloc((void**)&d->h_err, sizeof(unsigned long long) * std::max<uint32_t>(1, d->n_disk)));
+    HIP_TRY(hipHostMalloc((void**)&d->h_feat, 3 * sizeof(uint32_t) * std::max<uint32_t>(1, d->n_disk)));
+    HIP_TRY(hipHostMalloc((void**)&d->h_esum, sizeof(double) * std::max<uint32_t>(1, d->n_disk)));
+    HIP_TRY(hipHostMalloc((void**)&d->h_fx, RD_MLP_IN * sizeof(float) * std::max<uint32_t>(1, d->n_disk)));
+    HIP_TRY(hipHostMalloc((void**)&d->h_fy, sizeof(float) * std::max<uint32_t>(1, d->n_disk)));
     HIP_TRY(hipMemsetAsync(d->K, 0, (size_t)kMaxStage * 6 * n * sizeof(double), d->stream));
     HIP_TRY(hipMemsetAsync(d->ctl, 0, sizeof(DiskCtl) * d->n_disk, d->stream));
     d->generation++;
@@ -404,12 +423,40 @@ int build_lists(rd_disk* d)
                 coll.push_back(WorkItem{o.second, h.off + s, h.off + std::min(h.n, s + kBlock), h.off + j,
                                         h.off + std::min(h.n, j + 4096), 0, 0, 0});
     }
+    // proposer-feature work: the pairs j > i in (256 sinks) x (source chunk)
+    // tiles, the chunk cut so that the launch has about 8 items per CU
+    std::vector<WorkItem> feat;
+    {
+        size_t groups = 0;
+        for (auto& o : order) groups += (d->hd[o.second].n + kBlock - 1) / kBlock;
+        const size_t want = 16 * (size_t)d->n_cu;
+        for (auto& o : order) {
+            const HostDisk& h = d->hd[o.second];
+            if (h.n == 0) continue;
+            const uint32_t max_ch = (h.n + 1023) / 1024;
+            const uint32_t nch = (uint32_t)std::min<size_t>(max_ch, std::max<size_t>(1, (want + groups - 1) / groups));
+            const uint32_t C = ((h.n + nch - 1) / nch + kTile - 1) / kTile * kTile;
+            for (uint32_t s = 0; s < h.n; s += kBlock)
+                for (uint32_t j = 0; j < h.n; j += C) {
+                    const uint32_t j_hi = std::min(h.n, j + C);
+                    if (j_hi <= s + 1) continue;   // no pair j > i in this tile
+                    const uint32_t j_lo = std::max(j, s + 1) / kTile * kTile;
+                    feat.push_back(WorkItem{o.second, h.off + s, h.off + std::min(h.n, s + kBlock),
+                                            h.off + std::max(j, j_lo), h.off + j_hi, 0, 0, 0});
+                }
+        }
+    }
     dev_free(d->items_force);
     dev_free(d->items_coll);
+    dev_free(d->items_feat);
     if (int rc = dev_alloc(&d->items_force, items.size())) return rc;
     if (int rc = dev_alloc(&d->items_coll, coll.size())) return rc;
+    if (int rc = dev_alloc(&d->items_feat, feat.size())) return rc;
     d->n_items_force = (uint32_t)items.size();
     d->n_items_coll = (uint32_t)coll.size();
+    d->n_items_feat = (uint32_t)feat.size();
+    if (!feat.empty())
+        HIP_TRY(hipMemcpy(d->items_feat, feat.data(), feat.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
     if (!items.empty())
         HIP_TRY(hipMemcpy(d->items_force, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
     if (!coll.empty())
@@ -962,16 +1009,15 @@ static int after_accept(rd_integrator* it, const std::vector<char>& acc, const s
     if (!it->proposer && !it->logging) return RD_OK;
     if (int rc = flag_check_only(d, acc)) return rc;
     HIP_TRY(hipMemsetAsync(d->feat, 0xff, 3 * (size_t)nd * sizeof(uint32_t), d->stream));
-    HIP_TRY(hipMemsetAsync(d->esum, 0, nd * sizeof(double), d->stream));
-    HIP_TRY(launch_features(d->stream, d->items_coll, d->n_items_coll,
-                            std::min<uint32_t>(d->n_items_coll, 8u * (uint32_t)d->n_cu), d->n_all, d->xo, d->yo, d->zo,
-                            d->vxo, d->vyo, d->vzo, d->mass, d->id, d->disk_of, d->info, d->ctl, d->feat, d->esum));
-    std::vector<uint32_t> f(3 * (size_t)nd);
-    std::vector<double> es(nd);
-    HIP_TRY(hipMemcpyAsync(f.data(), d->feat, f.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, d->stream));
-    HIP_TRY(hipMemcpyAsync(es.data(), d->esum, nd * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(launch_features(d->stream, d->items_feat, d->n_items_feat,
+                            std::min<uint32_t>(d->n_items_feat, 8u * (uint32_t)d->n_cu), d->n_all, nd, d->xo, d->yo,
+                            d->zo, d->vxo, d->vyo, d->vzo, d->mass, d->id, d->disk_of, d->info, d->ctl, d->feat,
+                            d->eterm, d->esum));
+    HIP_TRY(hipMemcpyAsync(d->h_feat, d->feat, 3 * (size_t)nd * sizeof(uint32_t), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipMemcpyAsync(d->h_esum, d->esum, nd * sizeof(double), hipMemcpyDeviceToHost, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
-    std::vector<float> X;
+    const uint32_t* f = d->h_feat;
+    const double* es = d->h_esum;
     std::vector<uint32_t> who;
     for (uint32_t k = 0; k < nd; ++k) {
         if (!acc[k]) continue;
@@ -1001,16 +1047,21 @@ static int after_accept(rd_integrator* it, const std::vector<char>& acc, const s
         F[11] = (float)(H.tenc_prev != 0.0 ? tenc - H.tenc_prev : 0.0);
         H.tenc_prev = tenc;
         H.has_feat = true;
-        X.insert(X.end(), F, F + RD_MLP_IN);
+        std::memcpy(d->h_fx + (size_t)RD_MLP_IN * who.size(), F, RD_MLP_IN * sizeof(float));
         who.push_back(k);
     }
     if (it->proposer && !who.empty()) {
-        std::vector<float> y(who.size());
-        if (int rc = rd_mlp_forward(it->proposer, X.data(), (uint32_t)who.size(), y.data())) return rc;
-        for (size_t q = 0; q < who.size(); ++q) {
+        const uint32_t m = (uint32_t)who.size();
+        HIP_TRY(hipMemcpyAsync(d->fx, d->h_fx, (size_t)m * RD_MLP_IN * sizeof(float), hipMemcpyHostToDevice,
+                               d->stream));
+        if (int rc = mlp_forward_dev(it->proposer, d->stream, d->fx, m, d->fy))
+            return fail(rc, "proposer forward launch");
+        HIP_TRY(hipMemcpyAsync(d->h_fy, d->fy, m * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+        HIP_TRY(hipStreamSynchronize(d->stream));
+        for (uint32_t q = 0; q < m; ++q) {
             const uint32_t k = who[q];
             const double hc = it->dt_next[k];
-            double hp = it->safety * std::exp((double)y[q]);
+            double hp = it->safety * std::exp((double)d->h_fy[q]);
             if (!(hp > 0.0) || !std::isfinite(hp)) continue;
             hp = std::min(std::max(hp, std::fabs(hc) / 8.0), 8.0 * std::fabs(hc));
             it->dt_next[k] = hc < 0.0 ? -hp : hp;
@@ -1021,13 +1072,19 @@ static int after_accept(rd_integrator* it, const std::vector<char>& acc, const s
 }
 
 // Advance every disk in `want` by one accepted step (reference step(),
-// int_rungekutta8.cu:635-716 / int_rungekutta4.cu:173-247).
-static int step_disks(rd_integrator* it, const std::vector<char>& want)
+// int_rungekutta8.cu:635-716 / int_rungekutta4.cu:173-247). `want[k]` is the
+// number of accepted steps disk k takes in this call. Every round makes one
+// attempt for each disk that still owes steps, so a disk that accepts starts
+// its next step in the next round instead of waiting for the disks that
+// rejected; each disk's own sequence of attempts is the same either way.
+static int step_disks(rd_integrator* it, const std::vector<uint32_t>& want)
 {
     rd_disk* d = it->d;
     if (int rc = ensure_lists(d)) return rc;
     const uint32_t nd = d->n_disk;
-    std::vector<char> pending(want);
+    std::vector<uint32_t> remaining(want);
+    std::vector<char> pending(nd);
+    for (uint32_t k = 0; k < nd; ++k) pending[k] = remaining[k] > 0;
     std::vector<int> iter(nd, 0);
     std::vector<char> need_k0(nd, 1);
     for (uint32_t k = 0; k < nd; ++k)
@@ -1098,6 +1155,7 @@ static int step_disks(rd_integrator* it, const std::vector<char>& want)
                 it->passed[k]++;
                 d->hd[k].t += it->dt_did[k];
                 pending[k] = 0;
+                remaining[k]--;
                 commit[k] = F_COMMIT;
                 commit_flags_pending = 1;
                 acc[k] = 1;
@@ -1106,6 +1164,14 @@ static int step_disks(rd_integrator* it, const std::vector<char>& want)
         (void)n_act;
         if (commit_flags_pending && it->adaptive) {
             if (int rc = after_accept(it, acc, err_ratio, iter)) return rc;
+            for (uint32_t k = 0; k < nd; ++k) {
+                if (!acc[k] || remaining[k] == 0) continue;
+                // next step of this disk: committed at the start of the round
+                pending[k] = 1;
+                need_k0[k] = 1;
+                iter[k] = 0;
+                it->dt_try[k] = it->dt_next[k];
+            }
             std::fill(acc.begin(), acc.end(), 0);
         }
     }
@@ -1116,7 +1182,7 @@ static int step_disks(rd_integrator* it, const std::vector<char>& want)
 int rd_step(rd_integrator* it, double* dt_did)
 {
     if (!it) return fail(RD_EINVAL, "NULL integrator");
-    std::vector<char> want(it->d->n_disk, 1);
+    std::vector<uint32_t> want(it->d->n_disk, 1u);
     if (int rc = step_disks(it, want)) return rc;
     if (dt_did) *dt_did = it->dt_did[0];
     return RD_OK;
@@ -1160,10 +1226,8 @@ int rd_steps(rd_integrator* it, uint32_t n_steps)
         HIP_TRY(hipStreamSynchronize(d->stream));
         return RD_OK;
     }
-    std::vector<char> want(d->n_disk, 1);
-    for (uint32_t s = 0; s < n_steps; ++s)
-        if (int rc = step_disks(it, want)) return rc;
-    return RD_OK;
+    std::vector<uint32_t> want(d->n_disk, n_steps);
+    return step_disks(it, want);
 }
 
 int rd_get_dt_next(rd_integrator* it, uint32_t k, double* dt)
@@ -1200,6 +1264,7 @@ int rd_integrator_set_proposer(rd_integrator* it, rd_mlp* m, float safety)
 {
     if (!it || !(safety > 0.0f)) return fail(RD_EINVAL, "bad argument");
     if (m && !it->adaptive) return fail(RD_ESTATE, "the proposer needs an adaptive integrator");
+    if (m && mlp_device(m) != it->d->device) return fail(RD_EINVAL, "proposer and disk are on different devices");
     it->proposer = m;
     it->safety = safety;
     return RD_OK;
@@ -1650,7 +1715,7 @@ int rd_run(rd_integrator* it, double length, double output_interval, uint64_t ma
                 rd_clear_event_counter(d, k);
             }
         }
-        if (int rc = step_disks(it, want)) return rc;
+        if (int rc = step_disks(it, std::vector<uint32_t>(want.begin(), want.end()))) return rc;
         for (uint32_t k = 0; k < nd; ++k) {
             if (!want[k]) continue;
             const double dt = it->dt_did[k];

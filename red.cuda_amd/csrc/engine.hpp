@@ -112,12 +112,14 @@ hipError_t launch_integral(hipStream_t s, uint32_t lo, uint32_t hi, const double
                            double* out_u);
 hipError_t launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes);
 // Step-proposer features of disks flagged F_CHECK, from state (x..vz):
-// feat[3*d + 0] = min over pairs of r^2 / v^2 (float bits, atomicMin),
-// feat[3*d + 1] = min periapsis (float bits, atomicMin), esum[d] += Keplerian energy.
+// feat[3*d + 0] = min over pairs of r^2 / v^2 (float bits, atomicMin; the
+// caller fills feat with 0xff first), feat[3*d + 1] = min periapsis (same),
+// esum[d] = Keplerian energy of the disk (fixed-order sum of eterm[n_all]).
 hipError_t launch_features(hipStream_t s, const WorkItem* items, uint32_t n_items, uint32_t grid, uint32_t n_all,
-                           const double* x, const double* y, const double* z, const double* vx, const double* vy,
-                           const double* vz, const double* mass, const int32_t* id, const uint32_t* disk_of,
-                           const DiskInfo* info, const DiskCtl* ctl, uint32_t* feat, double* esum);
+                           uint32_t n_disk, const double* x, const double* y, const double* z, const double* vx,
+                           const double* vy, const double* vz, const double* mass, const int32_t* id,
+                           const uint32_t* disk_of, const DiskInfo* info, const DiskCtl* ctl, uint32_t* feat,
+                           double* eterm, double* esum);
 hipError_t launch_set_sources(hipStream_t s, uint32_t n_all, const double* mass, const int32_t* id,
                               const uint32_t* disk_of, const DiskInfo* info, double* m_ex, float* m32);
 

@@ -955,94 +955,140 @@ hipError_t launch_integral(hipStream_t st, uint32_t lo, uint32_t hi, const doubl
 // ---------------------------------------------------------------------------
 // Step-proposer features
 // ---------------------------------------------------------------------------
-// min over active pairs i < j of |r_ij|^2 / |v_ij|^2 (FP32), per disk flagged F_CHECK.
+// min over active pairs i < j of |r_ij|^2 / |v_ij|^2, per disk flagged
+// F_CHECK. A feature, not a force: FP32 throughout, positions and velocities
+// taken relative to the disk's star (body 0 of the disk) before rounding.
+// Items are (256 sinks) x (source chunk) tiles of one disk on or above the
+// diagonal; the j > i test inside drops the rest.
 __global__ __launch_bounds__(kBlock) void k_feat_pairs(const WorkItem* __restrict__ items, uint32_t n_items,
                                                        const double* __restrict__ x, const double* __restrict__ y,
                                                        const double* __restrict__ z, const double* __restrict__ vx,
                                                        const double* __restrict__ vy, const double* __restrict__ vz,
-                                                       const int32_t* __restrict__ id, const DiskCtl* __restrict__ ctl,
-                                                       uint32_t* __restrict__ feat)
+                                                       const int32_t* __restrict__ id,
+                                                       const DiskInfo* __restrict__ info,
+                                                       const DiskCtl* __restrict__ ctl, uint32_t* __restrict__ feat)
 {
-    __shared__ double4 tp[kTile];
+    __shared__ float4 tp[kTile];
     __shared__ float4 tv[kTile];
     const uint32_t tid = threadIdx.x;
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         const WorkItem w = items[it];
         if (!(ctl[w.disk].flags & F_CHECK)) continue;
+        const uint32_t s = info[w.disk].off;
+        const double sx = x[s], sy = y[s], sz = z[s], svx = vx[s], svy = vy[s], svz = vz[s];
         const uint32_t i = w.sink_lo + tid;
         const bool valid = i < w.sink_hi && id[i] > 0;
-        const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0, zi = valid ? z[i] : 0.0;
-        const float vxi = valid ? (float)vx[i] : 0.0f, vyi = valid ? (float)vy[i] : 0.0f,
-                    vzi = valid ? (float)vz[i] : 0.0f;
+        const float xi = valid ? (float)(x[i] - sx) : 0.0f, yi = valid ? (float)(y[i] - sy) : 0.0f,
+                    zi = valid ? (float)(z[i] - sz) : 0.0f;
+        const float vxi = valid ? (float)(vx[i] - svx) : 0.0f, vyi = valid ? (float)(vy[i] - svy) : 0.0f,
+                    vzi = valid ? (float)(vz[i] - svz) : 0.0f;
         float best = 3.0e38f;
         for (uint32_t t0 = w.src_lo; t0 < w.src_hi; t0 += kTile) {
             __syncthreads();
             const uint32_t j = t0 + tid;
             if (j < w.src_hi) {
-                tp[tid] = make_double4(x[j], y[j], z[j], id[j] > 0 ? 1.0 : 0.0);
-                tv[tid] = make_float4((float)vx[j], (float)vy[j], (float)vz[j], 0.0f);
+                const bool on = id[j] > 0;
+                tp[tid] = make_float4((float)(x[j] - sx), (float)(y[j] - sy), (float)(z[j] - sz), on ? 0.0f : 1.0f);
+                tv[tid] = make_float4((float)(vx[j] - svx), (float)(vy[j] - svy), (float)(vz[j] - svz), 0.0f);
             }
             __syncthreads();
             const uint32_t nt = min((uint32_t)kTile, w.src_hi - t0);
             if (valid) {
                 for (uint32_t k = 0; k < nt; ++k) {
-                    const uint32_t jj = t0 + k;
-                    const double4 pj = tp[k];
-                    if (jj <= i || pj.w == 0.0) continue;
-                    const float dx = (float)(pj.x - xi), dy = (float)(pj.y - yi), dz = (float)(pj.z - zi);
-                    const float ux = tv[k].x - vxi, uy = tv[k].y - vyi, uz = tv[k].z - vzi;
+                    const float4 pj = tp[k];
+                    const float4 qj = tv[k];
+                    const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+                    const float ux = qj.x - vxi, uy = qj.y - vyi, uz = qj.z - vzi;
                     const float r2 = dx * dx + dy * dy + dz * dz;
                     const float u2 = ux * ux + uy * uy + uz * uz + 1.0e-30f;
-                    best = fminf(best, r2 / u2);
+                    const float q = r2 * __builtin_amdgcn_rcpf(u2);
+                    best = (t0 + k <= i || pj.w != 0.0f) ? best : fminf(best, q);
                 }
             }
         }
         // positive floats order like their bit patterns
         for (int off = 32; off > 0; off >>= 1) best = fminf(best, __shfl_xor(best, off, 64));
-        if ((tid & 63u) == 0) atomicMin(&feat[3 * w.disk + 0], __float_as_uint(best));
+        if ((tid & 63u) == 0 && best < 3.0e38f) atomicMin(&feat[3 * w.disk + 0], __float_as_uint(best));
         __syncthreads();
     }
 }
 
-// min periapsis about the disk's star and the sum of Keplerian energies
-// m (v^2/2 - mu/r), mu = m_star + m, relative to the star (body 0 of the disk).
+// Min periapsis about the disk's star (float bits, atomicMin) and the
+// per-body Keplerian energy m (v^2/2 - mu/r), mu = m_star + m, relative to the
+// star (body 0 of the disk), written to eterm[i] (0 for the star, inactive
+// bodies and disks not flagged F_CHECK).
 __global__ __launch_bounds__(kBlock) void k_feat_bodies(uint32_t n_all, const double* x, const double* y,
                                                         const double* z, const double* vx, const double* vy,
                                                         const double* vz, const double* mass, const int32_t* id,
                                                         const uint32_t* disk_of, const DiskInfo* info,
-                                                        const DiskCtl* ctl, uint32_t* feat, double* esum)
+                                                        const DiskCtl* ctl, uint32_t* feat, double* eterm)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n_all) return;
-    const uint32_t d = disk_of[i];
-    if (!(ctl[d].flags & F_CHECK)) return;
+    const bool in = i < n_all;
+    const uint32_t d = in ? disk_of[i] : 0u;
     const uint32_t s = info[d].off;
-    if (i == s || !(id[i] > 0)) return;
-    const double rx = x[i] - x[s], ry = y[i] - y[s], rz = z[i] - z[s];
-    const double ux = vx[i] - vx[s], uy = vy[i] - vy[s], uz = vz[i] - vz[s];
-    const double mu = mass[s] + mass[i];
-    const double r = sqrt(rx * rx + ry * ry + rz * rz);
-    const double v2 = ux * ux + uy * uy + uz * uz;
-    const double eps = 0.5 * v2 - mu / r;
-    const double hx = ry * uz - rz * uy, hy = rz * ux - rx * uz, hz = rx * uy - ry * ux;
-    const double h2 = hx * hx + hy * hy + hz * hz;
-    const double e = sqrt(fmax(0.0, 1.0 + 2.0 * eps * h2 / (mu * mu)));
-    const double q = h2 / (mu * (1.0 + e));
-    atomicMin(&feat[3 * d + 1], __float_as_uint((float)q));
-    atomicAdd(&esum[d], mass[i] * eps);
+    const bool valid = in && (ctl[d].flags & F_CHECK) && i != s && id[i] > 0;
+    float q = 3.0e38f;
+    double et = 0.0;
+    if (valid) {
+        const double rx = x[i] - x[s], ry = y[i] - y[s], rz = z[i] - z[s];
+        const double ux = vx[i] - vx[s], uy = vy[i] - vy[s], uz = vz[i] - vz[s];
+        const double mu = mass[s] + mass[i];
+        const double r = sqrt(rx * rx + ry * ry + rz * rz);
+        const double v2 = ux * ux + uy * uy + uz * uz;
+        const double eps = 0.5 * v2 - mu / r;
+        const double hx = ry * uz - rz * uy, hy = rz * ux - rx * uz, hz = rx * uy - ry * ux;
+        const double h2 = hx * hx + hy * hy + hz * hz;
+        const double e = sqrt(fmax(0.0, 1.0 + 2.0 * eps * h2 / (mu * mu)));
+        q = (float)(h2 / (mu * (1.0 + e)));
+        et = mass[i] * eps;
+    }
+    if (in) eterm[i] = et;
+    // one atomic per wave when the wave lies in one disk, else one per lane
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    if (__all(d == d0)) {
+        for (int off = 32; off > 0; off >>= 1) q = fminf(q, __shfl_xor(q, off, 64));
+        if ((threadIdx.x & 63u) == 0 && q < 3.0e38f) atomicMin(&feat[3 * d0 + 1], __float_as_uint(q));
+    } else if (valid) {
+        atomicMin(&feat[3 * d + 1], __float_as_uint(q));
+    }
+}
+
+// esum[d] = sum of eterm over the disk's bodies, one block per disk, in a
+// fixed order (so the feature is identical run to run).
+__global__ __launch_bounds__(kBlock) void k_feat_esum(const double* __restrict__ eterm,
+                                                      const DiskInfo* __restrict__ info,
+                                                      const DiskCtl* __restrict__ ctl, double* __restrict__ esum)
+{
+    __shared__ double red[kBlock];
+    const uint32_t d = blockIdx.x;
+    if (!(ctl[d].flags & F_CHECK)) return;
+    const uint32_t lo = info[d].off, hi = lo + info[d].n;
+    double acc = 0.0;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) acc += eterm[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) esum[d] = red[0];
 }
 
 hipError_t launch_features(hipStream_t st, const WorkItem* items, uint32_t n_items, uint32_t grid, uint32_t n_all,
-                           const double* x, const double* y, const double* z, const double* vx, const double* vy,
-                           const double* vz, const double* mass, const int32_t* id, const uint32_t* disk_of,
-                           const DiskInfo* info, const DiskCtl* ctl, uint32_t* feat, double* esum)
+                           uint32_t n_disk, const double* x, const double* y, const double* z, const double* vx,
+                           const double* vy, const double* vz, const double* mass, const int32_t* id,
+                           const uint32_t* disk_of, const DiskInfo* info, const DiskCtl* ctl, uint32_t* feat,
+                           double* eterm, double* esum)
 {
     if (n_items)
-        hipLaunchKernelGGL(k_feat_pairs, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, vx, vy, vz, id, ctl,
-                           feat);
-    if (n_all)
+        hipLaunchKernelGGL(k_feat_pairs, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, vx, vy, vz, id,
+                           info, ctl, feat);
+    if (n_all) {
         hipLaunchKernelGGL(k_feat_bodies, dim3((n_all + kBlock - 1) / kBlock), dim3(kBlock), 0, st, n_all, x, y, z,
-                           vx, vy, vz, mass, id, disk_of, info, ctl, feat, esum);
+                           vx, vy, vz, mass, id, disk_of, info, ctl, feat, eterm);
+        hipLaunchKernelGGL(k_feat_esum, dim3(n_disk), dim3(kBlock), 0, st, eterm, info, ctl, esum);
+    }
     return hipGetLastError();
 }
 

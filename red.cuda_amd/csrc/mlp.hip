@@ -277,6 +277,7 @@ int ensure_part(rd_mlp* p, size_t nblk)
 
 namespace rdamd {
 const uint16_t* mlp_weights(const rd_mlp* p) { return p->wb; }
+int mlp_device(const rd_mlp* p) { return p->device; }
 int mlp_forward_dev(const rd_mlp* p, hipStream_t st, const float* x, uint32_t n, float* y)
 {
     if (n == 0) return RD_OK;

@@ -11,6 +11,8 @@ struct rd_mlp;
 namespace rdamd {
 // Device BF16 weights of a proposer.
 const uint16_t* mlp_weights(const rd_mlp* p);
+// Device ordinal the proposer's buffers live on.
+int mlp_device(const rd_mlp* p);
 // y[k] = log h proposal for raw feature rows x[k] (device pointers), on stream st.
 int mlp_forward_dev(const rd_mlp* p, hipStream_t st, const float* x, uint32_t n, float* y);
 // Sets the thread's rd_last_error() text.

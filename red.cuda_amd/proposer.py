@@ -1,0 +1,90 @@
+"""Learned step proposer workflow: log (features -> step) samples from the
+engine's own RKF7(8) runs, fit the 12 -> 64 -> 64 -> 1 BF16 MLP on the GPU
+with the library's forward/backward/Adam kernels, and run the ablation
+(classical controller vs the proposer) on held-out disks.
+
+The reference controller (int_rungekutta8.cu:690-706) still judges every
+attempt: the proposer only replaces the first try of the next step, clamped
+to [h_c / 8, 8 h_c] around the classical proposal h_c, and a rejected
+attempt falls back to the classical shrink.
+
+Features (one row per accepted step, computed from the accepted state):
+  0 log h_prev            1 log error ratio of that step   2 0.5 log min r^2/v^2 over pairs
+  3 log min periapsis     4 log N                          5 log10 relative Keplerian energy drift
+  6 log h two steps back  7 log error ratio two steps back 8 rejections of the last step
+  9 log classical h_next 10 log tolerance                 11 change of feature 2 since the last step
+Target: log(h * (1 / err_ratio)^(1/8)), the step that would just have met the
+tolerance."""
+import time
+
+import numpy as np
+
+import redamd as R
+
+
+def batch_sizes(n_disk=512, lo=64, hi=4096, seed=4):
+    """N_d uniform on [lo, hi] from MT19937(seed) raw outputs, as red_random::uniform(int, int)
+    (src/red.cuda.initial/distribution.cpp:43-46)."""
+    rs = np.random.RandomState(seed)
+    out = []
+    for _ in range(n_disk):
+        u = int.from_bytes(rs.bytes(4), "little")
+        out.append(lo + int(u / 4294967296.0 * (hi - lo + 1)))
+    return out
+
+
+def make_batch(sizes, base_seed, precision=R.RD_PREC_MIXED, device=0):
+    """One engine holding len(sizes) disks, disk k seeded base_seed + k."""
+    parts = [R.make_disk(n, base_seed + k) for k, n in enumerate(sizes)]
+    cat = [np.concatenate([q[i] for q in parts]) for i in range(4)]
+    nbt = np.stack([q[4] for q in parts])
+    d = R.Disk(*cat, nbt, batch=True, device=device)
+    d.set_precision(precision)
+    d.set_tuning(use_graph=1)
+    return d
+
+
+def collect(disk, steps, dt0=0.1 * R.K_GAUSS, tol=1e-10):
+    """Run `steps` accepted RKF7(8) steps on every disk of `disk` with sample
+    logging on; returns (X [n, 12] float32, T [n] float32)."""
+    it = R.Integrator(disk, R.RD_RKF78, dt0, adaptive=True, tol=tol)
+    it.set_logging(True)
+    it.steps(steps)
+    X, T = it.log()
+    it.close()
+    return X, T
+
+
+def fit(X, T, seed=5, epochs=15, batch=512, lr=1e-3, device=0):
+    """A proposer fitted to the samples on the GPU; returns (Mlp, final loss)."""
+    m = R.Mlp(seed, device=device)
+    m.set_norm(X.mean(0), X.std(0) + 1e-3, float(T.mean()), float(T.std()) + 1e-6)
+    loss = m.train(X, T, epochs=epochs, batch=batch, lr=lr)
+    return m, loss
+
+
+def run(disk, steps=None, length=None, proposer=None, safety=0.85, dt0=0.1 * R.K_GAUSS, tol=1e-10,
+        energy_disks=None):
+    """Integrate `disk` for `steps` accepted steps per disk (or `length` of time),
+    with or without the proposer. Returns a dict of work counters, wall time and
+    the largest relative energy error over `energy_disks` (default: all)."""
+    it = R.Integrator(disk, R.RD_RKF78, dt0, adaptive=True, tol=tol)
+    if proposer is not None:
+        it.set_proposer(proposer, safety)
+    nd = disk.n_disk()
+    ks = list(range(nd)) if energy_disks is None else list(energy_disks)
+    E0 = [disk.integral(k)["E"] for k in ks]
+    disk.sync()
+    t0 = time.perf_counter()
+    if steps is not None:
+        it.steps(steps)
+    else:
+        it.run(length, 1e30)
+    disk.sync()
+    el = time.perf_counter() - t0
+    c = np.array([it.counters(k) for k in range(nd)], np.int64).sum(0)
+    evals, pairs = it.work()
+    dE = max(abs(disk.integral(k)["E"] / e - 1.0) for k, e in zip(ks, E0))
+    it.close()
+    return {"seconds": el, "force_evals": int(evals), "pairs": float(pairs), "tried": int(c[0]),
+            "passed": int(c[1]), "failed": int(c[2]), "rel_energy_error": float(dE)}

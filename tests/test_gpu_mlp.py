@@ -1,0 +1,191 @@
+"""Step-proposer MLP on the GPU against the float64 numpy restatement
+(tests/mlp_ref.py, itself pinned by finite differences), the step features
+against a numpy computation from the engine's state, and the proposer inside
+the RKF78 controller (deterministic, tolerance still enforced)."""
+import numpy as np
+import pytest
+
+import mlp_ref as M
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def R():
+    import redamd
+    if redamd.device_count() < 1:
+        pytest.skip("no GPU")
+    return redamd
+
+
+def _net(R, seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    w = rng.normal(0, scale, M.NPARAM).astype(np.float32)
+    mean = rng.normal(0, 1, M.IN).astype(np.float32)
+    std = rng.uniform(0.5, 2.0, M.IN).astype(np.float32)
+    m = R.Mlp(seed)
+    m.set_params(w)
+    m.set_norm(mean, std, 0.3, 1.7)
+    return m, w, mean, std
+
+
+def test_bf16_weights_match_rounding_rule(R):
+    m, w, _, _ = _net(R, 1)
+    w_back, wb = m.params()
+    assert np.array_equal(w_back.view(np.uint32), w.view(np.uint32))
+    assert np.array_equal(wb.astype(np.uint32) << 16, M.bf16(w).view(np.uint32))
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 333])
+def test_forward_matches_reference(R, n):
+    m, w, mean, std = _net(R, 2)
+    x = np.random.default_rng(n).normal(0, 2, (n, M.IN)).astype(np.float32)
+    y = m.forward(x)
+    y_ref = M.forward(w, x, mean.astype(np.float64), std.astype(np.float64), 0.3, 1.7)
+    # FP32 accumulation over 64-wide dots of BF16 weights: 1e-5 relative
+    assert np.max(np.abs(y - y_ref)) <= 1e-5 * max(1.0, np.max(np.abs(y_ref)))
+
+
+@pytest.mark.parametrize("n", [40, 1000])
+def test_gradient_matches_reference(R, n):
+    m, w, mean, std = _net(R, 3)
+    rng = np.random.default_rng(n)
+    x = rng.normal(0, 2, (n, M.IN)).astype(np.float32)
+    t = rng.normal(0, 1, n).astype(np.float32)
+    loss, g = m.loss_grad(x, t)
+    loss_ref, g_ref = M.loss_grad(w, x, t, mean.astype(np.float64), std.astype(np.float64), 0.3, 1.7)
+    assert abs(loss - loss_ref) <= 1e-5 * abs(loss_ref)
+    # every parameter block (W1, b1, W2, b2, w3, b3) to 1e-4 of its largest entry
+    k = 0
+    for name, size in M.SIZES:
+        gs, rs = g[k:k + size], g_ref[k:k + size]
+        assert np.max(np.abs(gs - rs)) <= 1e-4 * max(np.max(np.abs(rs)), 1e-12), name
+        k += size
+
+
+def test_gradient_is_deterministic(R):
+    m, _, _, _ = _net(R, 4)
+    rng = np.random.default_rng(9)
+    x = rng.normal(0, 1, (777, M.IN)).astype(np.float32)
+    t = rng.normal(0, 1, 777).astype(np.float32)
+    _, g1 = m.loss_grad(x, t)
+    _, g2 = m.loss_grad(x, t)
+    assert np.array_equal(g1.view(np.uint32), g2.view(np.uint32))
+
+
+def test_training_fits_a_smooth_target(R):
+    rng = np.random.default_rng(5)
+    x = rng.normal(0, 1, (4096, M.IN)).astype(np.float32)
+    t = (np.sin(x[:, 0]) + 0.5 * x[:, 1] * x[:, 2] - 0.3 * x[:, 5]).astype(np.float32)
+    m = R.Mlp(7)
+    m.set_norm(np.zeros(M.IN), np.ones(M.IN), float(t.mean()), float(t.std()))
+    loss0, _ = m.loss_grad(x, t)
+    m.train(x, t, epochs=60, batch=128, lr=3e-3)
+    loss1, _ = m.loss_grad(x, t)
+    assert loss1 < 0.1 * loss0, (loss0, loss1)
+    # same seed, same data -> the same weights, bit for bit
+    m2 = R.Mlp(7)
+    m2.set_norm(np.zeros(M.IN), np.ones(M.IN), float(t.mean()), float(t.std()))
+    m2.train(x, t, epochs=60, batch=128, lr=3e-3)
+    assert np.array_equal(m.params()[0].view(np.uint32), m2.params()[0].view(np.uint32))
+
+
+def _np_features(r, v, p, b):
+    """min over active pairs of |dr|^2/|dv|^2 and min periapsis about body 0,
+    and the Keplerian energy sum, from the AoS state (float64)."""
+    act = b["id"] > 0
+    x = np.stack([r["x"], r["y"], r["z"]], 1)[act]
+    u = np.stack([v["x"], v["y"], v["z"]], 1)[act]
+    dr = x[:, None, :] - x[None, :, :]
+    du = u[:, None, :] - u[None, :, :]
+    q = (dr ** 2).sum(-1) / ((du ** 2).sum(-1) + 1e-300)
+    np.fill_diagonal(q, np.inf)
+    rr = x[1:] - x[0]
+    uu = u[1:] - u[0]
+    m = p["mass"][act]
+    mu = m[0] + m[1:]
+    rad = np.sqrt((rr ** 2).sum(1))
+    eps = 0.5 * (uu ** 2).sum(1) - mu / rad
+    h2 = (np.cross(rr, uu) ** 2).sum(1)
+    e = np.sqrt(np.maximum(0.0, 1.0 + 2.0 * eps * h2 / mu ** 2))
+    peri = h2 / (mu * (1.0 + e))
+    return q.min(), peri.min(), float((m[1:] * eps).sum())
+
+
+def test_logged_features_match_state(R):
+    r, v, p, b, nbt = R.make_disk(700, 31)
+    d = R.Disk(r, v, p, b, nbt)
+    d.set_precision(R.RD_PREC_MIXED)
+    it = R.Integrator(d, R.RD_RKF78, 0.05 * R.K_GAUSS, adaptive=True, tol=1e-10)
+    it.set_logging(True)
+    it.steps(1)
+    _, r1, v1, p1, b1 = d.state()
+    dt_next1 = it.dt_next()
+    it.steps(1)
+    X, T = it.log()
+    assert X.shape == (1, 12) and T.shape == (1,)
+    qmin, peri, _ = _np_features(r1, v1, p1, b1)
+    f = X[0].astype(np.float64)
+    assert abs(f[2] - 0.5 * np.log(qmin)) < 1e-3        # FP32, star-relative offsets
+    assert abs(f[3] - np.log(peri)) < 1e-4
+    assert f[4] == np.float32(np.log(701.0))
+    assert f[5] == np.float32(-20.0)                      # no drift at the first sample
+    assert abs(f[9] - np.log(dt_next1)) < 1e-6
+    assert f[10] == np.float32(np.log(1e-10))
+    assert f[8] == 0.0 or f[8] >= 1.0
+    assert np.isfinite(T).all()
+
+
+def test_proposer_run_is_deterministic_and_holds_tolerance(R):
+    r, v, p, b, nbt = R.make_disk(2048, 41)
+    # a quick proposer fitted on the engine's own log
+    d = R.Disk(r, v, p, b, nbt)
+    d.set_precision(R.RD_PREC_MIXED)
+    it = R.Integrator(d, R.RD_RKF78, 0.05 * R.K_GAUSS, adaptive=True, tol=1e-10)
+    it.set_logging(True)
+    it.steps(60)
+    X, T = it.log()
+    m = R.Mlp(11)
+    m.set_norm(X.mean(0), X.std(0) + 1e-3, float(T.mean()), float(T.std()) + 1e-6)
+    m.train(X, T, epochs=40, batch=32, lr=1e-3)
+
+    def run():
+        dd = R.Disk(r, v, p, b, nbt)
+        dd.set_precision(R.RD_PREC_MIXED)
+        ii = R.Integrator(dd, R.RD_RKF78, 0.05 * R.K_GAUSS, adaptive=True, tol=1e-10)
+        ii.set_proposer(m, 0.9)
+        E0 = dd.integral()["E"]
+        ii.steps(40)
+        return dd.state(), ii.counters(), abs(dd.integral()["E"] / E0 - 1.0)
+
+    s1, c1, dE1 = run()
+    s2, c2, dE2 = run()
+    assert c1 == c2
+    for a, bb in zip(s1[1:3], s2[1:3]):
+        assert np.array_equal(a.view(np.uint8), bb.view(np.uint8))
+    assert c1[1] == 40 and dE1 < 1e-9
+
+
+def test_proposer_ablation_gain(R):
+    """What the trained network gains over the classical controller alone:
+    proposer fitted on 64 seeded disks, run on 64 held-out disks of the same
+    sizes for the same number of accepted steps. Fewer rejected attempts and
+    less force work, with the tolerance and energy error held."""
+    import proposer as P
+    sizes = P.batch_sizes()[:64]
+    d = P.make_batch(sizes, 7000)
+    X, T = P.collect(d, 300)
+    d.close()
+    assert len(T) == 299 * len(sizes)
+    m, _ = P.fit(X, T, epochs=15)
+    d = P.make_batch(sizes, 4000)
+    cls = P.run(d, steps=100)
+    d.close()
+    d = P.make_batch(sizes, 4000)
+    lrn = P.run(d, steps=100, proposer=m, safety=0.85)
+    d.close()
+    print("classical", cls, "learned", lrn)
+    assert cls["passed"] == lrn["passed"] == 100 * len(sizes)
+    assert lrn["failed"] <= 0.7 * cls["failed"], (cls, lrn)
+    assert lrn["pairs"] < cls["pairs"], (cls, lrn)
+    assert lrn["rel_energy_error"] <= max(2.0 * cls["rel_energy_error"], 1e-10)
