@@ -22,6 +22,7 @@ import argparse
 import json
 import os
 import platform
+import re
 import sys
 import time
 
@@ -51,8 +52,7 @@ def cpu_model():
 
 def cpu_baseline(r, v, p, b, nbt, h0, tol):
     """One RKF7(8) step of the oracle's reference CPU path (1 thread)."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
+    O = _oracle()
     o = O.ODisk(r, v, p, b, nbt, formula=1)
     oi = O.OInteg(o, 4, h0, adaptive=True, tol=tol)
     t0 = time.perf_counter()
@@ -68,12 +68,92 @@ def cpu_baseline(r, v, p, b, nbt, h0, tol):
                       f"symmetric FP64 loop, {el:.1f} s on {cpu_model()} (nproc {os.cpu_count()})"}
 
 
-def roofline(d, precision):
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    return O
+
+
+def cpu_rk4_baseline(r, v, p, b, nbt, h, steps):
+    """Config 1 on the reference CPU path (symmetric FP64 loop, fixed-step RK4, 1 thread): the whole run."""
+    O = _oracle()
+    o = O.ODisk(r, v, p, b, nbt, formula=1)
+    oi = O.OInteg(o, 2, h, adaptive=False)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        oi.step()
+    el = time.perf_counter() - t0
+    n = len(r)
+    return {"value": 4 * steps * n * (n - 1.0) / el, "unit": "pair interactions/s", "cores": 1, "kind": "port",
+            "steps_per_s": steps / el,
+            "sample": f"all {steps} RK4 steps of the same N={n - 1} disk, symmetric FP64 loop, {el:.1f} s on "
+                      f"{cpu_model()} (nproc {os.cpu_count()})"}
+
+
+def cpu_force_baseline(r, v, p, b, nbt, gas=None, drag_mode=0, pairs_per_step=None):
+    """One force evaluation (gravity + drag) of the reference CPU path on the same disk (1 thread)."""
+    O = _oracle()
+    o = O.ODisk(r, v, p, b, nbt, formula=1, gas=gas, drag_mode=drag_mode)
+    t0 = time.perf_counter()
+    o.calc_dydx(1, 0, 0.0, r, v)
+    el = time.perf_counter() - t0
+    n = len(r)
+    out = {"value": n * (n - 1.0) / el, "unit": "pair interactions/s", "cores": 1, "kind": "port",
+           "sample": f"one force evaluation (gravity + drag) of the same N={n - 1} disk, symmetric FP64 loop, "
+                     f"{el:.1f} s on {cpu_model()} (nproc {os.cpu_count()})"}
+    if pairs_per_step:
+        out["steps_per_s_extrapolated"] = n * (n - 1.0) / el / pairs_per_step
+    return out
+
+
+def cpu_batch_baseline(parts, h0, tol, n_disks):
+    """One RKF7(8) step of each of the first n_disks disks of the batch on the reference CPU
+    path, disk after disk (1 thread)."""
+    O = _oracle()
+    pairs = 0.0
+    steps = 0
+    el = 0.0
+    for r, v, p, b, nbt in parts[:n_disks]:
+        o = O.ODisk(r, v, p, b, nbt, formula=1)
+        oi = O.OInteg(o, 4, h0, adaptive=True, tol=tol)
+        t0 = time.perf_counter()
+        oi.step()
+        el += time.perf_counter() - t0
+        tried, passed, _ = oi.counters()
+        n = len(r)
+        pairs += (1 + 12 * tried) * n * (n - 1.0)
+        steps += passed
+    return {"value": pairs / el, "unit": "pair interactions/s", "cores": 1, "kind": "port",
+            "disk_steps_per_s": steps / el,
+            "sample": f"one RKF7(8) step of each of the first {n_disks} disks of the same batch, symmetric FP64 "
+                      f"loop, {el:.1f} s on {cpu_model()} (nproc {os.cpu_count()})"}
+
+
+def profiled_traffic(kernel, workload_file, tuning):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary of this
+    workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when it was taken at the same
+    launch shape; None otherwise."""
+    try:
+        with open(os.path.join(ROOT, "profiles", workload_file)) as f:
+            text = f.read()
+    except OSError:
+        return None
+    m = re.search(r"--tuning (\d+),(\d+),(\d+)", text)
+    t = re.search(r"Traffic per launch: ([0-9.e+]+) B", text)
+    if not m or not t or kernel not in text:
+        return None
+    if tuning is None or (int(m[1]), int(m[2]), int(m[3])) != (tuning.sinks_per_lane, tuning.chunk,
+                                                               tuning.blocks_per_cu):
+        return None
+    return float(t[1])
+
+
+def roofline(d, precision, traffic=None):
     ms, pairs_launch = d.bench_force(20)
     achieved = FLOP_PER_PAIR * pairs_launch / (ms * 1e-3)
     return {"bound": "valu-fp32" if precision == "mixed" else "valu-fp64", "achieved": achieved / 1e12,
             "peak": (PEAK_FP32 if precision == "mixed" else PEAK_FP64) / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / (PEAK_FP32 if precision == "mixed" else PEAK_FP64), "traffic": None,
+            "frac": achieved / (PEAK_FP32 if precision == "mixed" else PEAK_FP64), "traffic": traffic,
             "kernel": "k_force_mixed" if precision == "mixed" else "k_force_exact",
             "ms_per_launch": ms, "pairs_per_launch": pairs_launch}
 
@@ -105,6 +185,9 @@ def bench_config1(args):
            "dtype": "f64", "data": "synthetic seeded disk",
            "config": {"workload": "star + 256 planetesimals, RK4 fixed h = 0.1 d, FP64, seed 1"},
            "rk4_steps_per_s": steps / el, "rel_energy_error": rel_e, "roofline": roofline(d, "fp64")}
+    if not args.no_cpu:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_rk4_baseline(r, v, p, b, nbt, 0.1 * R.K_GAUSS, 10000)
     emit(out)
 
 
@@ -148,6 +231,10 @@ def bench_config3(args):
            "events": int(len(d.event_log())), "rel_energy_error": rel_e,
            "note": "energy is not conserved with gas drag; the error column is the drag dissipation",
            "roofline": roofline(d, "mixed")}
+    if not args.no_cpu:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_force_baseline(r, v, p, b, nbt, gas=g, drag_mode=1,
+                                                 pairs_per_step=(p1 - p0) / steps)
     emit(out)
 
 
@@ -184,6 +271,9 @@ def bench_config4(args):
                                  "blocks_per_cu": tuning.blocks_per_cu}},
            "disk_steps_per_s": steps * len(sizes) / el, "attempts_all_disks": tried,
            "roofline": roofline(d, args.precision)}
+    if not args.no_cpu:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_batch_baseline(parts, 0.1 * R.K_GAUSS, args.tol, 64)
     emit(out)
 
 
@@ -385,7 +475,9 @@ def main():
         "energy_ok": bool(rel_e <= 1e-9),
         "max_rel_force_err_vs_fp64": f_err,
         "roofline": {"bound": "valu-fp32", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+                     "traffic": profiled_traffic("k_force_mixed", "k_force_mixed_disk16384_rkf78.txt", tuning)
+                     if (args.precision == "mixed" and args.n == 16384 and args.seed == 2) else None,
                      "kernel": "k_force_mixed" if args.precision == "mixed" else "k_force_exact",
                      "ms_per_launch": ms, "pairs_per_launch": pairs_launch},
     }

@@ -248,7 +248,10 @@ int rd_mlp_loss_grad(rd_mlp* m, const float* x, const float* t, uint32_t n, floa
 /* Adam on the GPU over shuffled minibatches */
 int rd_mlp_train(rd_mlp* m, const float* x, const float* t, uint32_t n, uint32_t epochs, uint32_t batch, float lr,
                  float* final_loss);
-/* use m to propose the first attempt of each step (NULL: reference controller only) */
+/* use m to propose the first attempt of each step (NULL: reference controller
+ * only): h = safety * exp(y), clamped to [h_c / 8, 8 h_c] around the reference
+ * controller's proposal h_c.  m must live on the disk's device and outlive its
+ * use by the integrator; the integrator must be adaptive. */
 int rd_integrator_set_proposer(rd_integrator* it, rd_mlp* m, float safety);
 /* record (features -> target log h) of every accepted step; the target is
  * log(h * (tol/err)^(1/8)), the step the Fehlberg estimate says would have
