@@ -396,8 +396,8 @@ int build_lists(rd_disk* d)
             pairs += (double)nsink * (b[3] - b[2]) - (cls == 0 ? nsink : 0);
             if (d->precision == RD_PREC_FP64) {
                 di.nchunk[cls] = 1;
-                for (uint32_t s = b[0]; s < b[1]; s += kBlock)
-                    items.push_back(WorkItem{k, h.off + s, h.off + std::min(b[1], s + kBlock), h.off + b[2],
+                for (uint32_t s = b[0]; s < b[1]; s += kExactSinks)
+                    items.push_back(WorkItem{k, h.off + s, h.off + std::min(b[1], s + kExactSinks), h.off + b[2],
                                              h.off + b[3], 0, 0, 0});
             } else {
                 const uint32_t lo = std::max(b[2], di.n_fp64), hi = b[3];
@@ -462,7 +462,9 @@ int build_lists(rd_disk* d)
     if (!coll.empty())
         HIP_TRY(hipMemcpy(d->items_coll, coll.data(), coll.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
     const uint32_t maxgrid = std::max<uint32_t>(1, d->tuning.blocks_per_cu) * (uint32_t)d->n_cu;
-    d->grid_force = std::max<uint32_t>(1, std::min<uint32_t>(d->n_items_force, maxgrid));
+    // the exact kernel's blocks use 26 KB of LDS: 6 fit on a CU
+    const uint32_t grid_cap = d->precision == RD_PREC_FP64 ? 6u * (uint32_t)d->n_cu : maxgrid;
+    d->grid_force = std::max<uint32_t>(1, std::min<uint32_t>(d->n_items_force, grid_cap));
     const size_t need = max_chunks * 3 * (size_t)d->n_all;
     if (need > d->part_len) {
         dev_free(d->part);

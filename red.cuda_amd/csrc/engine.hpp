@@ -19,6 +19,8 @@ namespace rdamd {
 constexpr int kMaxStage = 13;
 constexpr int kBlock = 256;          // threads per block for every kernel
 constexpr int kTile = 256;           // sources staged in LDS per tile
+// sinks per work item of the exact (FP64, reference-order) pair kernel
+constexpr uint32_t kExactSinks = 16;
 
 // Per-disk control flags (DiskCtl::flags).
 enum : uint32_t {

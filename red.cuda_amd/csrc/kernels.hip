@@ -71,8 +71,18 @@ __host__ __device__ constexpr double rk4_c(int r) { return (r == 1 || r == 2) ? 
 // Pair kernels
 // ---------------------------------------------------------------------------
 // Reference formula (pp_disk.cu:374-416): sources j ascending, skip i == j
-// and inactive j (m_ex < 0), a += (m_j / (d * d^2)) * d_vec.  One sink per
-// lane; the j tile is staged in LDS and read by all four waves.
+// and inactive j (m_ex < 0), a += (m_j / (d * d^2)) * d_vec, every operation
+// rounded separately (the reference builds -fmad=false).
+//
+// The sum per (sink, component) is a chain of FP64 adds in j order, so it
+// cannot be split; the pair terms feeding it can.  A block takes kExactSinks
+// sinks: for each 64-source LDS tile, all 256 threads compute the 16 x 64
+// terms (m_j / (d * d^2)) * dx (sqrt and divide are the costly part) into LDS,
+// then 48 lanes of wave 0 -- one per (sink, component) -- add the tile's
+// terms in j order to their running sums.  Same terms, same order: the result
+// is bit-identical to one lane summing alone, while the term work spreads
+// over N / 16 blocks of 4 waves instead of N lanes.
+constexpr uint32_t kExactTile = 64;
 __global__ __launch_bounds__(kBlock) void k_force_exact(const WorkItem* __restrict__ items, uint32_t n_items,
                                                         const double* __restrict__ x, const double* __restrict__ y,
                                                         const double* __restrict__ z,
@@ -80,41 +90,55 @@ __global__ __launch_bounds__(kBlock) void k_force_exact(const WorkItem* __restri
                                                         const DiskCtl* __restrict__ ctl, uint32_t need,
                                                         double* __restrict__ part, uint32_t n_all)
 {
-    __shared__ double4 tile[kTile];
+    static_assert(kExactSinks * kExactTile == 4 * kBlock, "4 terms per thread");
+    __shared__ double4 src[kExactTile];
+    __shared__ double term[3][kExactSinks][kExactTile + 1];   // +1: sums read with a 65-double stride
     const uint32_t tid = threadIdx.x;
+    const uint32_t g = tid >> 4;              // sink of this thread's terms
+    const uint32_t k_lo = tid & 15u;          // its sources: k_lo + 16 q, q = 0..3
+    const uint32_t sc_c = tid >> 4 & 3u;      // summing lanes (tid < 48): component
+    const uint32_t sc_g = tid & 15u;          //                             sink
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         const WorkItem w = items[it];
         if (!(ctl[w.disk].flags & need)) continue;
-        const uint32_t i = w.sink_lo + tid;
+        const uint32_t i = w.sink_lo + g;
         const bool valid = i < w.sink_hi;
         const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0, zi = valid ? z[i] : 0.0;
-        double ax = 0.0, ay = 0.0, az = 0.0;
-        for (uint32_t t0 = w.src_lo; t0 < w.src_hi; t0 += kTile) {
+        const uint32_t si = w.sink_lo + sc_g;  // sink of a summing lane
+        double acc = 0.0;
+        for (uint32_t t0 = w.src_lo; t0 < w.src_hi; t0 += kExactTile) {
             __syncthreads();
-            const uint32_t j = t0 + tid;
-            if (j < w.src_hi) tile[tid] = make_double4(x[j], y[j], z[j], m_ex[j]);
+            if (tid < kExactTile) {
+                const uint32_t j = t0 + tid;
+                src[tid] = j < w.src_hi ? make_double4(x[j], y[j], z[j], m_ex[j]) : make_double4(0.0, 0.0, 0.0, -1.0);
+            }
             __syncthreads();
-            const uint32_t nt = min((uint32_t)kTile, w.src_hi - t0);
-            for (uint32_t k = 0; k < nt; ++k) {
-                const double4 s = tile[k];
-                if (s.w < 0.0 || t0 + k == i) continue;
-                const double dx = s.x - xi;
-                const double dy = s.y - yi;
-                const double dz = s.z - zi;
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) {
+                const uint32_t k = k_lo + 16u * q;
+                const double4 sj = src[k];
+                const double dx = sj.x - xi;
+                const double dy = sj.y - yi;
+                const double dz = sj.z - zi;
                 const double r2 = SQR(dx) + SQR(dy) + SQR(dz);
                 const double d = sqrt(r2);
-                const double sc = s.w / (d * r2);
-                ax += sc * dx;
-                ay += sc * dy;
-                az += sc * dz;
+                const double sc = sj.w / (d * r2);
+                // a skipped pair (self, inactive or padding source) adds +0:
+                // the running sum starts at +0 and round-to-nearest never
+                // makes it -0, so acc + 0 == acc bit for bit
+                const bool skip = sj.w < 0.0 || t0 + k == i;
+                term[0][g][k] = skip ? 0.0 : sc * dx;
+                term[1][g][k] = skip ? 0.0 : sc * dy;
+                term[2][g][k] = skip ? 0.0 : sc * dz;
+            }
+            __syncthreads();
+            if (tid < 3 * kExactSinks) {
+                const double* tr = &term[sc_c][sc_g][0];
+#pragma unroll
+                for (uint32_t k = 0; k < kExactTile; ++k) acc += tr[k];
             }
         }
-        if (valid) {
-            part[(size_t)(w.slot * 3 + 0) * n_all + i] = ax;
-            part[(size_t)(w.slot * 3 + 1) * n_all + i] = ay;
-            part[(size_t)(w.slot * 3 + 2) * n_all + i] = az;
-        }
-        __syncthreads();
+        if (tid < 3 * kExactSinks && si < w.sink_hi) part[(size_t)(w.slot * 3 + sc_c) * n_all + si] = acc;
     }
 }
 
