@@ -37,6 +37,7 @@ from proposer import batch_sizes  # noqa: E402
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
 FLOP_PER_PAIR = 20.0
+GRAPH = 1   # replay each RK attempt from a captured hipGraph (--no-graph: direct launches)
 
 
 def cpu_model():
@@ -167,7 +168,7 @@ def bench_config1(args):
     steps = args.steps if args.steps != 1000 else 10000
     r, v, p, b, nbt = R.make_disk(256, 1)
     d = R.Disk(r, v, p, b, nbt)
-    d.set_tuning(use_graph=1)
+    d.set_tuning(use_graph=GRAPH)
     it = R.Integrator(d, R.RD_RK4, 0.1 * R.K_GAUSS, adaptive=False)
     it.steps(args.warmup)
     E0 = d.integral()["E"]
@@ -202,7 +203,7 @@ def bench_config3(args):
     d.set_drag_mode(R.RD_DRAG_ALL_MASSIVE)
     d.set_precision(R.RD_PREC_MIXED)
     tuning = d.autotune() if not args.no_autotune else d.tuning()
-    d.set_tuning(use_graph=1)
+    d.set_tuning(use_graph=GRAPH)
     a_min = np.min(np.hypot(r["x"][1:], r["y"][1:]))
     h0 = 2 * np.pi * np.sqrt(a_min ** 3) / 1000.0
     it = R.Integrator(d, R.RD_RKF78, h0, adaptive=True, tol=args.tol)
@@ -248,7 +249,7 @@ def bench_config4(args):
     d = R.Disk(*cat, nbt, batch=True)
     d.set_precision(R.RD_PREC_MIXED if args.precision == "mixed" else R.RD_PREC_FP64)
     tuning = d.autotune() if (args.precision == "mixed" and not args.no_autotune) else d.tuning()
-    d.set_tuning(use_graph=1)
+    d.set_tuning(use_graph=GRAPH)
     steps = args.steps if args.steps != 1000 else 100
     it = R.Integrator(d, R.RD_RKF78, 0.1 * R.K_GAUSS, adaptive=True, tol=args.tol)
     it.steps(args.warmup)
@@ -324,7 +325,7 @@ def bench_config5(args):
         print(f"# N=16384 disk {name}", file=sys.stderr, flush=True)
         d = R.Disk(r, v, p, b, nbt)
         d.set_precision(prec)
-        d.set_tuning(use_graph=1)
+        d.set_tuning(use_graph=GRAPH)
         single[name] = P.run(d, length=args.c5_length, proposer=prop, safety=args.safety, dt0=0.05 * R.K_GAUSS,
                              tol=tol)
         single[name]["rk_evals"] = rk_evals(single[name])
@@ -362,6 +363,7 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch the attempt's kernels directly (no hipGraph)")
     ap.add_argument("--tuning", default="", help="S,CHUNK,BLOCKS_PER_CU for the mixed kernel (skips autotune)")
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5],
                     help="1 RK4 N=256, 2 RKF78 N=16384 (default), 3 gas+collisions N=131072, "
@@ -373,6 +375,8 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run the byte-counter calibration copy of this size and exit")
     args = ap.parse_args()
+    global GRAPH
+    GRAPH = 0 if args.no_graph else 1
     if args.copy_bytes:
         ms = R.bench_copy(args.copy_bytes, 5)
         print(json.dumps({"copy_bytes": args.copy_bytes, "ms_per_launch": ms,
@@ -397,14 +401,14 @@ def main():
     h0 = 2 * np.pi * np.sqrt(a_min ** 3 / 1.0) / 1000.0
     d = R.Disk(r, v, p, b, nbt)
     d.set_precision(R.RD_PREC_MIXED if args.precision == "mixed" else R.RD_PREC_FP64)
-    d.set_tuning(use_graph=1)
+    d.set_tuning(use_graph=GRAPH)
     if args.tuning:
         S, Cs, B = (int(x) for x in args.tuning.split(","))
         d.set_tuning(sinks_per_lane=S, chunk=Cs, blocks_per_cu=B)
         tuning = d.tuning()
     else:
         tuning = d.autotune() if (args.precision == "mixed" and not args.no_autotune) else d.tuning()
-    d.set_tuning(use_graph=1)
+    d.set_tuning(use_graph=GRAPH)
     it = R.Integrator(d, R.RD_RKF78, h0, adaptive=True, tol=args.tol)
     print(f"# warmup {args.warmup} steps", file=sys.stderr, flush=True)
     it.steps(args.warmup)
