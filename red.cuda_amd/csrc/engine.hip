@@ -612,6 +612,7 @@ static int validate_disk(const uint32_t* nbt, const rd_body_md* bmd, uint32_t n)
     uint32_t tot = 0;
     for (int t = 0; t < kNType; ++t) tot += nbt[t];
     if (tot != n) return fail(RD_EINVAL, "n_by_type does not sum to the body count");
+    if (n == 0) return fail(RD_EINVAL, "a disk needs at least one body");
     uint32_t i = 0;
     for (int t = 0; t < kNType; ++t)
         for (uint32_t q = 0; q < nbt[t]; ++q, ++i)

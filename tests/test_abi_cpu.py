@@ -55,6 +55,11 @@ def test_validation_errors():
                                 C.byref(h)) == R.RD_EINVAL
     with pytest.raises(R.RedError):
         R.check(R.lib.rd_integrator_create(None, 4, 1.0, 1, 1e-10, C.byref(h)))
+    # a batch holding an empty disk is refused before any device call
+    nbt2 = np.stack([nbt, np.zeros_like(nbt)])
+    assert R.lib.rd_batch_create(2, R._ptr(nbt2), R._ptr(r), R._ptr(v), R._ptr(p), R._ptr(b), None, None, None, 0,
+                                 C.byref(h)) == R.RD_EINVAL
+    assert "at least one body" in R.lib.rd_last_error().decode()
 
 
 @pytest.mark.parametrize("kw", [dict(n=256, seed=1), dict(n=1000, seed=2),
