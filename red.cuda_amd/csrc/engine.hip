@@ -423,26 +423,27 @@ int build_lists(rd_disk* d)
                 coll.push_back(WorkItem{o.second, h.off + s, h.off + std::min(h.n, s + kBlock), h.off + j,
                                         h.off + std::min(h.n, j + 4096), 0, 0, 0});
     }
-    // proposer-feature work: the pairs j > i in (256 sinks) x (source chunk)
-    // tiles, the chunk cut so that the launch has about 8 items per CU
+    // proposer-feature work: the pairs j > i in (128 sinks) x (source chunk)
+    // tiles, the chunk cut so that the launch has about 16 items per CU
     std::vector<WorkItem> feat;
     {
+        const uint32_t G = 128;   // 2 sinks per lane x 64 lanes (k_feat_pairs)
         size_t groups = 0;
-        for (auto& o : order) groups += (d->hd[o.second].n + kBlock - 1) / kBlock;
+        for (auto& o : order) groups += (d->hd[o.second].n + G - 1) / G;
         const size_t want = 16 * (size_t)d->n_cu;
         for (auto& o : order) {
             const HostDisk& h = d->hd[o.second];
             if (h.n == 0) continue;
-            const uint32_t max_ch = (h.n + 1023) / 1024;
-            const uint32_t nch = (uint32_t)std::min<size_t>(max_ch, std::max<size_t>(1, (want + groups - 1) / groups));
+            const uint32_t max_ch = (h.n + kTile - 1) / kTile;
+            const uint32_t nch = (uint32_t)std::min<size_t>(max_ch, std::max<size_t>(1, 2 * want / groups));
             const uint32_t C = ((h.n + nch - 1) / nch + kTile - 1) / kTile * kTile;
-            for (uint32_t s = 0; s < h.n; s += kBlock)
+            for (uint32_t s = 0; s < h.n; s += G)
                 for (uint32_t j = 0; j < h.n; j += C) {
                     const uint32_t j_hi = std::min(h.n, j + C);
                     if (j_hi <= s + 1) continue;   // no pair j > i in this tile
-                    const uint32_t j_lo = std::max(j, s + 1) / kTile * kTile;
-                    feat.push_back(WorkItem{o.second, h.off + s, h.off + std::min(h.n, s + kBlock),
-                                            h.off + std::max(j, j_lo), h.off + j_hi, 0, 0, 0});
+                    const uint32_t j_lo = std::max(j, (s + 1) / kTile * kTile);
+                    feat.push_back(WorkItem{o.second, h.off + s, h.off + std::min(h.n, s + G), h.off + j_lo,
+                                            h.off + j_hi, 0, 0, 0});
                 }
         }
     }
@@ -1013,7 +1014,7 @@ static int after_accept(rd_integrator* it, const std::vector<char>& acc, const s
     if (int rc = flag_check_only(d, acc)) return rc;
     HIP_TRY(hipMemsetAsync(d->feat, 0xff, 3 * (size_t)nd * sizeof(uint32_t), d->stream));
     HIP_TRY(launch_features(d->stream, d->items_feat, d->n_items_feat,
-                            std::min<uint32_t>(d->n_items_feat, 8u * (uint32_t)d->n_cu), d->n_all, nd, d->xo, d->yo,
+                            std::min<uint32_t>(d->n_items_feat, 16u * (uint32_t)d->n_cu), d->n_all, nd, d->xo, d->yo,
                             d->zo, d->vxo, d->vyo, d->vzo, d->mass, d->id, d->disk_of, d->info, d->ctl, d->feat,
                             d->eterm, d->esum));
     HIP_TRY(hipMemcpyAsync(d->h_feat, d->feat, 3 * (size_t)nd * sizeof(uint32_t), hipMemcpyDeviceToHost, d->stream));

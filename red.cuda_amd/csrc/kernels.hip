@@ -982,8 +982,11 @@ hipError_t launch_integral(hipStream_t st, uint32_t lo, uint32_t hi, const doubl
 // min over active pairs i < j of |r_ij|^2 / |v_ij|^2, per disk flagged
 // F_CHECK. A feature, not a force: FP32 throughout, positions and velocities
 // taken relative to the disk's star (body 0 of the disk) before rounding.
-// Items are (256 sinks) x (source chunk) tiles of one disk on or above the
-// diagonal; the j > i test inside drops the rest.
+// Layout as the mixed pair kernel: a work item is kFeatS * 64 sinks (kFeatS
+// per lane) against a source chunk on or above the diagonal; the four waves
+// take quarters of each 256-source LDS tile; j > i is tested per pair.  The
+// minimum does not depend on the order, so the result is deterministic.
+constexpr uint32_t kFeatS = 2;
 __global__ __launch_bounds__(kBlock) void k_feat_pairs(const WorkItem* __restrict__ items, uint32_t n_items,
                                                        const double* __restrict__ x, const double* __restrict__ y,
                                                        const double* __restrict__ z, const double* __restrict__ vx,
@@ -994,18 +997,28 @@ __global__ __launch_bounds__(kBlock) void k_feat_pairs(const WorkItem* __restric
 {
     __shared__ float4 tp[kTile];
     __shared__ float4 tv[kTile];
-    const uint32_t tid = threadIdx.x;
+    __shared__ float red[kBlock / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         const WorkItem w = items[it];
         if (!(ctl[w.disk].flags & F_CHECK)) continue;
         const uint32_t s = info[w.disk].off;
         const double sx = x[s], sy = y[s], sz = z[s], svx = vx[s], svy = vy[s], svz = vz[s];
-        const uint32_t i = w.sink_lo + tid;
-        const bool valid = i < w.sink_hi && id[i] > 0;
-        const float xi = valid ? (float)(x[i] - sx) : 0.0f, yi = valid ? (float)(y[i] - sy) : 0.0f,
-                    zi = valid ? (float)(z[i] - sz) : 0.0f;
-        const float vxi = valid ? (float)(vx[i] - svx) : 0.0f, vyi = valid ? (float)(vy[i] - svy) : 0.0f,
-                    vzi = valid ? (float)(vz[i] - svz) : 0.0f;
+        uint32_t si[kFeatS];
+        float xi[kFeatS], yi[kFeatS], zi[kFeatS], ui[kFeatS], vi[kFeatS], wi[kFeatS];
+#pragma unroll
+        for (uint32_t u = 0; u < kFeatS; ++u) {
+            const uint32_t i = w.sink_lo + lane + 64u * u;
+            const bool valid = i < w.sink_hi && id[i] > 0;
+            si[u] = valid ? i : 0xffffffffu;     // an invalid sink matches no j > i
+            xi[u] = valid ? (float)(x[i] - sx) : 0.0f;
+            yi[u] = valid ? (float)(y[i] - sy) : 0.0f;
+            zi[u] = valid ? (float)(z[i] - sz) : 0.0f;
+            ui[u] = valid ? (float)(vx[i] - svx) : 0.0f;
+            vi[u] = valid ? (float)(vy[i] - svy) : 0.0f;
+            wi[u] = valid ? (float)(vz[i] - svz) : 0.0f;
+        }
         float best = 3.0e38f;
         for (uint32_t t0 = w.src_lo; t0 < w.src_hi; t0 += kTile) {
             __syncthreads();
@@ -1017,23 +1030,33 @@ __global__ __launch_bounds__(kBlock) void k_feat_pairs(const WorkItem* __restric
             }
             __syncthreads();
             const uint32_t nt = min((uint32_t)kTile, w.src_hi - t0);
-            if (valid) {
-                for (uint32_t k = 0; k < nt; ++k) {
-                    const float4 pj = tp[k];
-                    const float4 qj = tv[k];
-                    const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-                    const float ux = qj.x - vxi, uy = qj.y - vyi, uz = qj.z - vzi;
+            const uint32_t k0 = wave * 64u, k1 = min(k0 + 64u, nt);
+#pragma unroll 4
+            for (uint32_t k = k0; k < k1; ++k) {
+                const float4 pj = tp[k];
+                const float4 qj = tv[k];
+                const uint32_t jj = t0 + k;
+#pragma unroll
+                for (uint32_t u = 0; u < kFeatS; ++u) {
+                    const float dx = pj.x - xi[u], dy = pj.y - yi[u], dz = pj.z - zi[u];
+                    const float du = qj.x - ui[u], dv = qj.y - vi[u], dw = qj.z - wi[u];
                     const float r2 = dx * dx + dy * dy + dz * dz;
-                    const float u2 = ux * ux + uy * uy + uz * uz + 1.0e-30f;
-                    const float q = r2 * __builtin_amdgcn_rcpf(u2);
-                    best = (t0 + k <= i || pj.w != 0.0f) ? best : fminf(best, q);
+                    const float u2 = du * du + dv * dv + dw * dw + 1.0e-30f;
+                    const float qq = r2 * __builtin_amdgcn_rcpf(u2);
+                    // si = 0xffffffff (invalid sink) never satisfies jj > si
+                    best = (jj > si[u] && pj.w == 0.0f) ? fminf(best, qq) : best;
                 }
             }
         }
         // positive floats order like their bit patterns
         for (int off = 32; off > 0; off >>= 1) best = fminf(best, __shfl_xor(best, off, 64));
-        if ((tid & 63u) == 0 && best < 3.0e38f) atomicMin(&feat[3 * w.disk + 0], __float_as_uint(best));
+        if (lane == 0) red[wave] = best;
         __syncthreads();
+        if (tid == 0) {
+            const float m = fminf(fminf(red[0], red[1]), fminf(red[2], red[3]));
+            if (m < 3.0e38f) atomicMin(&feat[3 * w.disk + 0], __float_as_uint(m));
+        }
+        __syncthreads();   // red[] is rewritten by the next item
     }
 }
 
@@ -1089,7 +1112,15 @@ __global__ __launch_bounds__(kBlock) void k_feat_esum(const double* __restrict__
     if (!(ctl[d].flags & F_CHECK)) return;
     const uint32_t lo = info[d].off, hi = lo + info[d].n;
     double acc = 0.0;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += kBlock) acc += eterm[i];
+    uint32_t i = lo + threadIdx.x;
+    for (; i + 7 * kBlock < hi; i += 8 * kBlock) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = eterm[i + u * kBlock];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += v[u];
+    }
+    for (; i < hi; i += kBlock) acc += eterm[i];
     red[threadIdx.x] = acc;
     __syncthreads();
     for (uint32_t w = kBlock / 2; w > 0; w >>= 1) {

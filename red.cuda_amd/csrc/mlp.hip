@@ -74,19 +74,33 @@ __device__ float forward_one(const SharedW& s, const float* x, float* h1, float*
     return y;
 }
 
-// Inference: raw features -> log h (de-normalised).
-__global__ __launch_bounds__(kMlpBlock) void k_mlp_forward(const uint16_t* __restrict__ wb, Norm nm,
-                                                           const float* __restrict__ xraw, uint32_t n,
-                                                           float* __restrict__ y)
+// Inference: raw features -> log h (de-normalised).  One wave per sample,
+// lane j = hidden unit j: layer 1 and layer 2 are 12- and 64-term dot
+// products per lane over inputs held in LDS; the output sums the 64 lanes'
+// w3_j h2_j with a fixed butterfly, so results are reproducible.
+__global__ __launch_bounds__(kH) void k_mlp_forward(const uint16_t* __restrict__ wb, Norm nm,
+                                                    const float* __restrict__ xraw, uint32_t n,
+                                                    float* __restrict__ y)
 {
-    __shared__ SharedW s;
-    load_weights(s, wb);
-    __syncthreads();
-    const uint32_t q = blockIdx.x * kMlpBlock + threadIdx.x;
+    static_assert(kH == 64, "one lane per hidden unit of a 64-wide wave");
+    __shared__ float xs[kIn];
+    __shared__ float h1[kH];
+    const uint32_t q = blockIdx.x;
+    const uint32_t j = threadIdx.x;
     if (q >= n) return;
-    float x[kIn], h1[kH], h2[kH];
-    for (int i = 0; i < kIn; ++i) x[i] = (xraw[(size_t)q * kIn + i] - nm.mean[i]) * nm.inv_std[i];
-    y[q] = forward_one(s, x, h1, h2) * nm.t_std + nm.t_mean;
+    if (j < (uint32_t)kIn) xs[j] = (xraw[(size_t)q * kIn + j] - nm.mean[j]) * nm.inv_std[j];
+    __syncthreads();
+    float a = bf16_to_f32(wb[kOffB1 + j]);
+#pragma unroll
+    for (int i = 0; i < kIn; ++i) a = fmaf(xs[i], bf16_to_f32(wb[kOffW1 + i * kH + j]), a);
+    h1[j] = tanhf(a);
+    __syncthreads();
+    float b = bf16_to_f32(wb[kOffB2 + j]);
+#pragma unroll 16
+    for (int k = 0; k < kH; ++k) b = fmaf(h1[k], bf16_to_f32(wb[kOffW2 + k * kH + j]), b);
+    float o = tanhf(b) * bf16_to_f32(wb[kOffW3 + j]);
+    for (int off = 32; off > 0; off >>= 1) o += __shfl_xor(o, off, 64);
+    if (j == 0) y[q] = (o + bf16_to_f32(wb[kOffB3])) * nm.t_std + nm.t_mean;
 }
 
 // Loss 0.5 * mean (y - t)^2 in normalised target units and its gradient.
@@ -281,8 +295,7 @@ int mlp_device(const rd_mlp* p) { return p->device; }
 int mlp_forward_dev(const rd_mlp* p, hipStream_t st, const float* x, uint32_t n, float* y)
 {
     if (n == 0) return RD_OK;
-    hipLaunchKernelGGL(k_mlp_forward, dim3((n + kMlpBlock - 1) / kMlpBlock), dim3(kMlpBlock), 0, st, p->wb, p->nm, x,
-                       n, y);
+    hipLaunchKernelGGL(k_mlp_forward, dim3(n), dim3(kH), 0, st, p->wb, p->nm, x, n, y);
     return hipGetLastError() == hipSuccess ? RD_OK : RD_EDEVICE;
 }
 }  // namespace rdamd
