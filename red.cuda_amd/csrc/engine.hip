@@ -131,6 +131,7 @@ struct rd_disk {
     uint32_t n_items_coll = 0;
     WorkItem* items_feat = nullptr;
     uint32_t n_items_feat = 0;
+    bool exact_lane = false;       // FP64 pair kernel: one sink per lane (see build_lists)
     bool lists_valid = false;
     double pairs_per_eval = 0.0;
 
@@ -383,6 +384,19 @@ int build_lists(rd_disk* d)
     double pairs = 0.0;
     const uint32_t S = d->tuning.sinks_per_lane;
     const uint32_t C = std::max<uint32_t>(kTile, d->tuning.chunk / kTile * kTile);
+    // FP64: one sink per lane when 256-sink items give >= 4 blocks per CU,
+    // else the term-parallel kernel on 16-sink items (same bits either way)
+    if (d->precision == RD_PREC_FP64) {
+        size_t groups = 0;
+        for (auto& o : order)
+            for (int cls = 0; cls < 3; ++cls) {
+                uint32_t b[4];
+                class_bounds(d->hd[o.second], cls, b);
+                if (b[1] > b[0]) groups += (b[1] - b[0] + kBlock - 1) / kBlock;
+            }
+        d->exact_lane = groups >= 4 * (size_t)d->n_cu;
+    }
+    const uint32_t exact_group = d->exact_lane ? (uint32_t)kBlock : kExactSinks;
     for (auto& o : order) {
         const uint32_t k = o.second;
         const HostDisk& h = d->hd[k];
@@ -396,8 +410,8 @@ int build_lists(rd_disk* d)
             pairs += (double)nsink * (b[3] - b[2]) - (cls == 0 ? nsink : 0);
             if (d->precision == RD_PREC_FP64) {
                 di.nchunk[cls] = 1;
-                for (uint32_t s = b[0]; s < b[1]; s += kExactSinks)
-                    items.push_back(WorkItem{k, h.off + s, h.off + std::min(b[1], s + kExactSinks), h.off + b[2],
+                for (uint32_t s = b[0]; s < b[1]; s += exact_group)
+                    items.push_back(WorkItem{k, h.off + s, h.off + std::min(b[1], s + exact_group), h.off + b[2],
                                              h.off + b[3], 0, 0, 0});
             } else {
                 const uint32_t lo = std::max(b[2], di.n_fp64), hi = b[3];
@@ -464,7 +478,7 @@ int build_lists(rd_disk* d)
         HIP_TRY(hipMemcpy(d->items_coll, coll.data(), coll.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
     const uint32_t maxgrid = std::max<uint32_t>(1, d->tuning.blocks_per_cu) * (uint32_t)d->n_cu;
     // the exact kernel's blocks use 26 KB of LDS: 6 fit on a CU
-    const uint32_t grid_cap = d->precision == RD_PREC_FP64 ? 6u * (uint32_t)d->n_cu : maxgrid;
+    const uint32_t grid_cap = d->precision == RD_PREC_FP64 && !d->exact_lane ? 6u * (uint32_t)d->n_cu : maxgrid;
     d->grid_force = std::max<uint32_t>(1, std::min<uint32_t>(d->n_items_force, grid_cap));
     const size_t need = max_chunks * 3 * (size_t)d->n_all;
     if (need > d->part_len) {
@@ -524,8 +538,8 @@ StageArgs make_args(rd_disk* d, int r, int adaptive, int err_all)
 hipError_t enqueue_force(rd_disk* d, const double* fx, const double* fy, const double* fz, uint32_t need)
 {
     if (d->precision == RD_PREC_FP64)
-        return launch_force_exact(d->stream, d->items_force, d->n_items_force, d->grid_force, fx, fy, fz, d->m_ex,
-                                  d->ctl, need, d->part, d->n_all);
+        return launch_force_exact(d->stream, d->exact_lane ? 1 : 0, d->items_force, d->n_items_force, d->grid_force,
+                                  fx, fy, fz, d->m_ex, d->ctl, need, d->part, d->n_all);
     return launch_force_mixed(d->stream, (int)d->tuning.sinks_per_lane, d->items_force, d->n_items_force,
                               d->grid_force, fx, fy, fz, d->m32, d->ctl, need, d->part, d->n_all);
 }

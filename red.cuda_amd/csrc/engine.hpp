@@ -88,9 +88,9 @@ struct StageArgs {
 };
 
 // Launchers (kernels.hip).
-hipError_t launch_force_exact(hipStream_t s, const WorkItem* items, uint32_t n_items, uint32_t grid,
+hipError_t launch_force_exact(hipStream_t s, int lane, const WorkItem* items, uint32_t n_items, uint32_t grid,
                               const double* x, const double* y, const double* z, const double* m_ex,
-                              const DiskCtl* ctl, uint32_t need_mask, double* part, uint32_t n_all);
+                              const DiskCtl* ctl, uint32_t need, double* part, uint32_t n_all);
 hipError_t launch_force_mixed(hipStream_t s, int sinks_per_lane, const WorkItem* items, uint32_t n_items,
                               uint32_t grid, const double* x, const double* y, const double* z,
                               const float* m32, const DiskCtl* ctl, uint32_t need_mask, double* part,

@@ -142,6 +142,56 @@ __global__ __launch_bounds__(kBlock) void k_force_exact(const WorkItem* __restri
     }
 }
 
+// The same sum with one sink per lane, each lane adding its terms in j order
+// as it computes them (the reference kernel's structure): cheaper per pair,
+// but only one lane per sink, so it is used when 256-sink work items alone
+// fill the device (large batches); bit-identical to k_force_exact.
+__global__ __launch_bounds__(kBlock) void k_force_exact_lane(const WorkItem* __restrict__ items, uint32_t n_items,
+                                                             const double* __restrict__ x,
+                                                             const double* __restrict__ y,
+                                                             const double* __restrict__ z,
+                                                             const double* __restrict__ m_ex,
+                                                             const DiskCtl* __restrict__ ctl, uint32_t need,
+                                                             double* __restrict__ part, uint32_t n_all)
+{
+    __shared__ double4 tile[kTile];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const WorkItem w = items[it];
+        if (!(ctl[w.disk].flags & need)) continue;
+        const uint32_t i = w.sink_lo + tid;
+        const bool valid = i < w.sink_hi;
+        const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0, zi = valid ? z[i] : 0.0;
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        for (uint32_t t0 = w.src_lo; t0 < w.src_hi; t0 += kTile) {
+            __syncthreads();
+            const uint32_t j = t0 + tid;
+            if (j < w.src_hi) tile[tid] = make_double4(x[j], y[j], z[j], m_ex[j]);
+            __syncthreads();
+            const uint32_t nt = min((uint32_t)kTile, w.src_hi - t0);
+            for (uint32_t k = 0; k < nt; ++k) {
+                const double4 sj = tile[k];
+                if (sj.w < 0.0 || t0 + k == i) continue;
+                const double dx = sj.x - xi;
+                const double dy = sj.y - yi;
+                const double dz = sj.z - zi;
+                const double r2 = SQR(dx) + SQR(dy) + SQR(dz);
+                const double d = sqrt(r2);
+                const double sc = sj.w / (d * r2);
+                ax += sc * dx;
+                ay += sc * dy;
+                az += sc * dz;
+            }
+        }
+        if (valid) {
+            part[(size_t)(w.slot * 3 + 0) * n_all + i] = ax;
+            part[(size_t)(w.slot * 3 + 1) * n_all + i] = ay;
+            part[(size_t)(w.slot * 3 + 2) * n_all + i] = az;
+        }
+        __syncthreads();
+    }
+}
+
 struct alignas(16) SrcF {
     double x, y, z;
     float m, pad;
@@ -263,13 +313,17 @@ __global__ __launch_bounds__(kBlock) void k_force_mixed(const WorkItem* __restri
     }
 }
 
-hipError_t launch_force_exact(hipStream_t st, const WorkItem* items, uint32_t n_items, uint32_t grid,
+hipError_t launch_force_exact(hipStream_t st, int lane, const WorkItem* items, uint32_t n_items, uint32_t grid,
                               const double* x, const double* y, const double* z, const double* m_ex,
                               const DiskCtl* ctl, uint32_t need, double* part, uint32_t n_all)
 {
     if (n_items == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_force_exact, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, m_ex, ctl, need,
-                       part, n_all);
+    if (lane)
+        hipLaunchKernelGGL(k_force_exact_lane, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, m_ex, ctl,
+                           need, part, n_all);
+    else
+        hipLaunchKernelGGL(k_force_exact, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, m_ex, ctl, need,
+                           part, n_all);
     return hipGetLastError();
 }
 
