@@ -285,20 +285,21 @@ def rk_evals(res):
 
 
 def bench_config5(args):
-    """Config 5: the learned step proposer. Samples are logged from seeded runs of
-    a 512-disk batch (N_d ~ U[64, 4096], disk k seeded 7000 + k), the MLP is fitted
-    on the GPU, and the ablation runs on held-out disks: the config-4 batch (seeds
-    4000 + k) and the config-2 disk (N = 16384, seed 2), each with the classical
-    controller (mixed and plain FP64) and with the proposer."""
+    """Config 5: the learned step proposer on the config-3 batch. Samples are logged
+    from seeded runs of a 512-disk batch with the config-3 physics (N_d ~ U[64, 4096],
+    gas drag on every massive body, merging f = 1, hit-centrum 0.1 AU; disk k seeded
+    7000 + k), the MLP is fitted on the GPU, and the ablation integrates held-out disks
+    to the same simulated time with the classical controller (mixed and plain FP64) and
+    with the proposer: the config-3 disk (N = 131072, seed 3) and the held-out batch
+    (seeds 4000 + k)."""
     import proposer as P
-    sizes = P.batch_sizes()
+    sizes = batch_sizes()
     tol = args.tol
     dt0 = 0.1 * R.K_GAUSS
-    log_steps = args.log_steps
-    print(f"# logging {log_steps} steps x {len(sizes)} disks", file=sys.stderr, flush=True)
-    d = P.make_batch(sizes, 7000)
+    print(f"# logging {args.log_steps} steps x {len(sizes)} disks", file=sys.stderr, flush=True)
+    d = P.make_batch(sizes, 7000, gas=True)
     t0 = time.perf_counter()
-    X, T = P.collect(d, log_steps, dt0, tol)
+    X, T = P.collect(d, args.log_steps, dt0, tol)
     t_log = time.perf_counter() - t0
     d.close()
     print(f"# {len(T)} samples in {t_log:.1f} s; training", file=sys.stderr, flush=True)
@@ -306,49 +307,44 @@ def bench_config5(args):
     m, loss = P.fit(X, T, epochs=args.epochs)
     t_fit = time.perf_counter() - t0
     rms = float(np.sqrt(np.mean((m.forward(X) - T) ** 2)))
+    n_samples = int(len(T))
     del X, T
-    steps = args.steps if args.steps != 1000 else 100
-    probe = range(0, len(sizes), 32)
-    res = {}
-    for name, prec, prop in (("classical_fp64", R.RD_PREC_FP64, None), ("classical_mixed", R.RD_PREC_MIXED, None),
-                             ("learned_mixed", R.RD_PREC_MIXED, m)):
-        print(f"# batch {name}", file=sys.stderr, flush=True)
-        d = P.make_batch(sizes, 4000, precision=prec)
-        res[name] = P.run(d, steps=steps, proposer=prop, safety=args.safety, dt0=dt0, tol=tol, energy_disks=probe)
-        res[name]["rk_evals"] = rk_evals(res[name])
-        res[name]["disk_steps_per_s"] = res[name]["passed"] / res[name]["seconds"]
+    runs = (("classical_fp64", R.RD_PREC_FP64, None), ("classical_mixed", R.RD_PREC_MIXED, None),
+            ("learned_mixed", R.RD_PREC_MIXED, m))
+    disk3 = {}
+    for name, prec, prop in runs:
+        print(f"# config-3 disk {name}", file=sys.stderr, flush=True)
+        d, h0 = P.config3_disk(precision=prec)
+        disk3[name] = P.run(d, length=args.c5_length, proposer=prop, safety=args.safety, dt0=h0, tol=tol)
+        disk3[name]["rk_evals"] = rk_evals(disk3[name])
         d.close()
-    single = {}
-    r, v, p, b, nbt = R.make_disk(16384, 2)
-    for name, prec, prop in (("classical_fp64", R.RD_PREC_FP64, None), ("classical_mixed", R.RD_PREC_MIXED, None),
-                             ("learned_mixed", R.RD_PREC_MIXED, m)):
-        print(f"# N=16384 disk {name}", file=sys.stderr, flush=True)
-        d = R.Disk(r, v, p, b, nbt)
-        d.set_precision(prec)
-        d.set_tuning(use_graph=GRAPH)
-        single[name] = P.run(d, length=args.c5_length, proposer=prop, safety=args.safety, dt0=0.05 * R.K_GAUSS,
-                             tol=tol)
-        single[name]["rk_evals"] = rk_evals(single[name])
+    batch = {}
+    for name, prec, prop in runs[1:]:
+        print(f"# held-out batch {name}", file=sys.stderr, flush=True)
+        d = P.make_batch(sizes, 4000, precision=prec, gas=True)
+        batch[name] = P.run(d, length=args.c5_batch_length, proposer=prop, safety=args.safety, dt0=dt0, tol=tol,
+                            energy_disks=range(0, len(sizes), 32))
+        batch[name]["rk_evals"] = rk_evals(batch[name])
         d.close()
-    value = res["classical_fp64"]["rk_evals"] / res["learned_mixed"]["rk_evals"]
+    value = disk3["classical_fp64"]["rk_evals"] / disk3["learned_mixed"]["rk_evals"]
     out = {"metric": "RK evaluations, plain FP64 RKF78 / learned proposer (config 5)", "value": value,
-           "unit": "x fewer RK evaluations", "n_gpus": 1, "steps": steps, "warmup": 0,
-           "ms_per_step": res["learned_mixed"]["seconds"] * 1e3 / steps, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64 state / f32 pair force / bf16 MLP weights",
-           "data": "synthetic seeded disks",
-           "config": {"workload": f"proposer trained on {log_steps} logged steps x {len(sizes)} disks (seeds 7000+k), "
-                                  f"ablation on the 512-disk batch (seeds 4000+k, {steps} steps) and the "
-                                  f"N=16384 disk (seed 2, {args.c5_length:g} time units), RKF7(8) tol {tol:g}",
+           "unit": "x fewer RK evaluations", "n_gpus": 1, "steps": disk3["learned_mixed"]["passed"], "warmup": 0,
+           "ms_per_step": disk3["learned_mixed"]["seconds"] * 1e3 / max(1, disk3["learned_mixed"]["passed"]),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64 state / f32 pair force / bf16 MLP weights", "data": "synthetic seeded disks",
+           "config": {"workload": f"proposer trained on {args.log_steps} logged steps x {len(sizes)} disks of the "
+                                  "config-3 physics (seeds 7000+k); ablation to equal simulated time on the config-3 "
+                                  f"disk (N=131072, seed 3, {args.c5_length:g} time units) and on the held-out batch "
+                                  f"(seeds 4000+k, {args.c5_batch_length:g} time units), RKF7(8) tol {tol:g}",
                       "safety": args.safety, "epochs": args.epochs},
-           "training": {"samples": int(log_steps * len(sizes)), "log_seconds": t_log, "fit_seconds": t_fit,
-                        "final_loss": loss, "rms_log_h": rms},
-           "batch": res, "single_disk": single,
-           "ratios": {"batch_evals_fp64_over_learned": value,
-                      "batch_evals_mixed_over_learned": res["classical_mixed"]["rk_evals"] / res["learned_mixed"]["rk_evals"],
-                      "batch_time_mixed_over_learned": res["classical_mixed"]["seconds"] / res["learned_mixed"]["seconds"],
-                      "single_evals_fp64_over_learned": single["classical_fp64"]["rk_evals"] / single["learned_mixed"]["rk_evals"],
-                      "single_evals_mixed_over_learned": single["classical_mixed"]["rk_evals"] / single["learned_mixed"]["rk_evals"],
-                      "single_time_mixed_over_learned": single["classical_mixed"]["seconds"] / single["learned_mixed"]["seconds"]}}
+           "training": {"samples": n_samples, "log_seconds": t_log, "fit_seconds": t_fit, "final_loss": loss,
+                        "rms_log_h": rms},
+           "config3_disk": disk3, "heldout_batch": batch,
+           "ratios": {"config3_evals_fp64_over_learned": value,
+                      "config3_evals_mixed_over_learned": disk3["classical_mixed"]["rk_evals"] / disk3["learned_mixed"]["rk_evals"],
+                      "config3_time_mixed_over_learned": disk3["classical_mixed"]["seconds"] / disk3["learned_mixed"]["seconds"],
+                      "batch_evals_mixed_over_learned": batch["classical_mixed"]["rk_evals"] / batch["learned_mixed"]["rk_evals"],
+                      "batch_time_mixed_over_learned": batch["classical_mixed"]["seconds"] / batch["learned_mixed"]["seconds"]}}
     emit(out)
 
 
@@ -371,7 +367,8 @@ def main():
     ap.add_argument("--log-steps", type=int, default=2000, help="config 5: logged steps per disk")
     ap.add_argument("--epochs", type=int, default=10, help="config 5: training epochs")
     ap.add_argument("--safety", type=float, default=0.85, help="config 5: proposer safety factor")
-    ap.add_argument("--c5-length", type=float, default=8.0, help="config 5: single-disk run length (time units)")
+    ap.add_argument("--c5-length", type=float, default=1.5, help="config 5: config-3 disk run length (time units)")
+    ap.add_argument("--c5-batch-length", type=float, default=2.9, help="config 5: held-out batch run length")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run the byte-counter calibration copy of this size and exit")
     args = ap.parse_args()
