@@ -121,6 +121,45 @@ class pp_disk {
     rd_disk* h_ = nullptr;
 };
 
+// The learned step proposer (no reference counterpart): a 12 -> 64 -> 64 -> 1
+// BF16 MLP trained on the GPU from the engine's own logged steps.
+class mlp {
+  public:
+    explicit mlp(int id_dev = 0, uint32_t seed = 0) { rd_check(rd_mlp_create(id_dev, seed, &h_), "mlp"); }
+    ~mlp() { rd_mlp_destroy(h_); }
+    mlp(const mlp&) = delete;
+    mlp& operator=(const mlp&) = delete;
+
+    rd_mlp* handle() { return h_; }
+    // standardisation of the 12 features and of the target log h
+    void set_norm(const float mean[RD_MLP_IN], const float stdev[RD_MLP_IN], float t_mean, float t_std)
+    {
+        rd_check(rd_mlp_set_norm(h_, mean, stdev, t_mean, t_std), "mlp::set_norm");
+    }
+    // fits the samples (x: n rows of RD_MLP_IN features, t: n targets); returns the last epoch's loss
+    float train(const std::vector<float>& x, const std::vector<float>& t, uint32_t epochs, uint32_t batch, float lr)
+    {
+        if (x.size() != t.size() * RD_MLP_IN) throw std::string("mlp::train: x must hold 12 features per target");
+        float loss = 0.0f;
+        rd_check(rd_mlp_train(h_, x.data(), t.data(), (uint32_t)t.size(), epochs, batch, lr, &loss), "mlp::train");
+        return loss;
+    }
+    std::vector<float> params()
+    {
+        std::vector<float> w(RD_MLP_NPARAM);
+        rd_check(rd_mlp_get_params(h_, w.data(), nullptr), "mlp::params");
+        return w;
+    }
+    void set_params(const std::vector<float>& w)
+    {
+        if (w.size() != RD_MLP_NPARAM) throw std::string("mlp::set_params: wrong parameter count");
+        rd_check(rd_mlp_set_params(h_, w.data()), "mlp::set_params");
+    }
+
+  private:
+    rd_mlp* h_ = nullptr;
+};
+
 // integrator (src/red.cuda/integrator.h:10-84)
 class integrator {
   public:
@@ -150,6 +189,22 @@ class integrator {
     uint64_t get_n_failed_step() const { return counter(2); }
     void set_error_check_for_tp(bool on) { rd_check(rd_set_error_check_for_tp(h_, on ? 1 : 0), "error_check_for_tp"); }
     rd_integrator* handle() { return h_; }
+
+    // learned step proposer: first try of each step from m (nullptr: reference controller only)
+    void set_proposer(mlp* m, float safety = 0.85f)
+    {
+        rd_check(rd_integrator_set_proposer(h_, m ? m->handle() : nullptr, safety), "set_proposer");
+    }
+    // records (features -> target log h) of every accepted step
+    void set_logging(bool on) { rd_check(rd_integrator_set_logging(h_, on ? 1 : 0), "set_logging"); }
+    void log(std::vector<float>& x, std::vector<float>& t)
+    {
+        uint32_t n = 0;
+        rd_check(rd_integrator_log(h_, nullptr, nullptr, 0, &n), "log");
+        x.resize((size_t)n * RD_MLP_IN);
+        t.resize(n);
+        rd_check(rd_integrator_log(h_, x.data(), t.data(), n, &n), "log");
+    }
 
   protected:
     uint64_t counter(int k) const
