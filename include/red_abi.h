@@ -255,7 +255,8 @@ int rd_mlp_train(rd_mlp* m, const float* x, const float* t, uint32_t n, uint32_t
 int rd_integrator_set_proposer(rd_integrator* it, rd_mlp* m, float safety);
 /* record (features -> target log h) of every accepted step; the target is
  * log(h * (tol/err)^(1/8)), the step the Fehlberg estimate says would have
- * just met the tolerance */
+ * just met the tolerance.  The log is host memory that grows by 13 floats per
+ * accepted step and disk until rd_integrator_clear_log. */
 int rd_integrator_set_logging(rd_integrator* it, int on);
 int rd_integrator_log(rd_integrator* it, float* x, float* t, uint32_t cap, uint32_t* n);
 int rd_integrator_clear_log(rd_integrator* it);

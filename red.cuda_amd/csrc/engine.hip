@@ -22,6 +22,7 @@
 
 #include "../../include/red_abi.h"
 #include "engine.hpp"
+#include "device_scope.hpp"
 #include "mlp.hpp"
 
 using namespace rdamd;
@@ -87,6 +88,27 @@ void dev_free(T*& p)
     if (p) (void)hipFree((void*)p);
     p = nullptr;
 }
+
+// Timing events released on every return path.
+struct EventPair {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t create()
+    {
+        hipError_t e = hipEventCreate(&e0);
+        return e == hipSuccess ? hipEventCreate(&e1) : e;
+    }
+    // ms between the two events, after waiting for the second
+    hipError_t elapsed(float* ms)
+    {
+        hipError_t e = hipEventSynchronize(e1);
+        return e == hipSuccess ? hipEventElapsedTime(ms, e0, e1) : e;
+    }
+    ~EventPair()
+    {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
 
 }  // namespace
 
@@ -591,8 +613,11 @@ int run_attempt(rd_integrator* it, bool use_graph)
             hipError_t e = enqueue_attempt(it);
             hipGraph_t g = nullptr;
             hipError_t e2 = hipStreamEndCapture(d->stream, &g);
-            if (e != hipSuccess) return fail(RD_EDEVICE, std::string("capture: ") + hipGetErrorString(e));
-            if (e2 != hipSuccess) return fail(RD_EDEVICE, std::string("end capture: ") + hipGetErrorString(e2));
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                if (e != hipSuccess) return fail(RD_EDEVICE, std::string("capture: ") + hipGetErrorString(e));
+                return fail(RD_EDEVICE, std::string("end capture: ") + hipGetErrorString(e2));
+            }
             it->graph = g;
             HIP_TRY(hipGraphInstantiate(&it->gexec, g, nullptr, nullptr, 0));
             it->graph_gen = d->generation;
@@ -643,6 +668,7 @@ int rd_batch_create(uint32_t n_disk, const uint32_t* n_by_type, const rd_var4* r
                     const rd_param* p, const rd_body_md* bmd, const double* t, const double threshold[3],
                     const rd_gas_params* gas, int device, rd_disk** out)
 {
+    DeviceScope dev_scope(device);
     if (!out || !n_by_type || !r || !v || !p || !bmd || n_disk == 0) return fail(RD_EINVAL, "NULL argument");
     *out = nullptr;
     rd_disk* d = new rd_disk();
@@ -713,8 +739,8 @@ int rd_disk_create(uint32_t n_total, const uint32_t n_by_type[RD_BODY_TYPE_N], c
 
 int rd_disk_destroy(rd_disk* d)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d) return RD_OK;
-    (void)hipSetDevice(d->device);
     (void)hipStreamSynchronize(d->stream);
     free_buffers(d);
     (void)hipStreamDestroy(d->stream);
@@ -724,6 +750,7 @@ int rd_disk_destroy(rd_disk* d)
 
 int rd_disk_set_precision(rd_disk* d, int precision)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || (precision != RD_PREC_FP64 && precision != RD_PREC_MIXED)) return fail(RD_EINVAL, "bad precision");
     d->precision = precision;
     d->lists_valid = false;
@@ -732,6 +759,7 @@ int rd_disk_set_precision(rd_disk* d, int precision)
 
 int rd_disk_set_drag_mode(rd_disk* d, int mode)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || (mode != RD_DRAG_NSI && mode != RD_DRAG_ALL_MASSIVE)) return fail(RD_EINVAL, "bad drag mode");
     d->drag_mode = mode;
     d->lists_valid = false;
@@ -740,6 +768,7 @@ int rd_disk_set_drag_mode(rd_disk* d, int mode)
 
 int rd_disk_set_tuning(rd_disk* d, const rd_tuning* t)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || !t) return fail(RD_EINVAL, "NULL argument");
     if (t->sinks_per_lane != 1 && t->sinks_per_lane != 2 && t->sinks_per_lane != 4)
         return fail(RD_EINVAL, "sinks_per_lane must be 1, 2 or 4");
@@ -752,6 +781,7 @@ int rd_disk_set_tuning(rd_disk* d, const rd_tuning* t)
 
 int rd_disk_get_tuning(rd_disk* d, rd_tuning* t)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || !t) return fail(RD_EINVAL, "NULL argument");
     *t = d->tuning;
     return RD_OK;
@@ -759,6 +789,7 @@ int rd_disk_get_tuning(rd_disk* d, rd_tuning* t)
 
 int rd_disk_count(rd_disk* d, uint32_t* n)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || !n) return fail(RD_EINVAL, "NULL argument");
     *n = d->n_disk;
     return RD_OK;
@@ -766,6 +797,7 @@ int rd_disk_count(rd_disk* d, uint32_t* n)
 
 int rd_disk_n_total(rd_disk* d, uint32_t k, uint32_t* n)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || !n || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
     *n = d->hd[k].n;
     return RD_OK;
@@ -774,6 +806,7 @@ int rd_disk_n_total(rd_disk* d, uint32_t k, uint32_t* n)
 int rd_disk_n_objects(rd_disk* d, uint32_t k, uint32_t playing[RD_BODY_TYPE_N], uint32_t inactive[RD_BODY_TYPE_N],
                       uint32_t removed[RD_BODY_TYPE_N])
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
     if (playing) std::memcpy(playing, d->hd[k].playing, sizeof(d->hd[k].playing));
     if (inactive) std::memcpy(inactive, d->hd[k].inactive, sizeof(d->hd[k].inactive));
@@ -783,6 +816,7 @@ int rd_disk_n_objects(rd_disk* d, uint32_t k, uint32_t playing[RD_BODY_TYPE_N], 
 
 int rd_sync(rd_disk* d)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d) return fail(RD_EINVAL, "NULL disk");
     HIP_TRY(hipStreamSynchronize(d->stream));
     return RD_OK;
@@ -790,6 +824,7 @@ int rd_sync(rd_disk* d)
 
 int rd_get_state(rd_disk* d, uint32_t k, double* t, rd_var4* r, rd_var4* v, rd_param* p, rd_body_md* bmd)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
     const HostDisk& h = d->hd[k];
     if (t) *t = h.t;
@@ -800,6 +835,7 @@ int rd_get_state(rd_disk* d, uint32_t k, double* t, rd_var4* r, rd_var4* v, rd_p
 
 int rd_set_state(rd_disk* d, uint32_t k, double t, const rd_var4* r, const rd_var4* v)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
     d->hd[k].t = t;
     return upload_bodies(d, d->hd[k].off, d->hd[k].n, r, v, nullptr, nullptr);
@@ -809,6 +845,7 @@ int rd_set_state(rd_disk* d, uint32_t k, double t, const rd_var4* r, const rd_va
 int rd_calc_dydx(rd_disk* d, uint32_t k, uint32_t i, uint32_t rr, double t, const rd_var4* r, const rd_var4* v,
                  rd_var4* dy)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     (void)rr;
     if (!d || k >= d->n_disk || !r || !v || !dy) return fail(RD_EINVAL, "bad argument");
     const HostDisk& h = d->hd[k];
@@ -844,22 +881,19 @@ int rd_calc_dydx(rd_disk* d, uint32_t k, uint32_t i, uint32_t rr, double t, cons
 
 int rd_bench_force(rd_disk* d, uint32_t n_iter, double* ms, double* pairs)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || n_iter == 0) return fail(RD_EINVAL, "bad argument");
     if (int rc = ensure_lists(d)) return rc;
     for (uint32_t q = 0; q < d->n_disk; ++q) d->h_ctl[q] = DiskCtl{d->hd[q].t, 0.0, F_NEED_K0 | F_ACTIVE, 0};
     HIP_TRY(hipMemcpyAsync(d->ctl, d->h_ctl, d->n_disk * sizeof(DiskCtl), hipMemcpyHostToDevice, d->stream));
     HIP_TRY(enqueue_force(d, d->x, d->y, d->z, F_NEED_K0));   // warm-up
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventRecord(e0, d->stream));
+    EventPair ev;
+    HIP_TRY(ev.create());
+    HIP_TRY(hipEventRecord(ev.e0, d->stream));
     for (uint32_t q = 0; q < n_iter; ++q) HIP_TRY(enqueue_force(d, d->x, d->y, d->z, F_NEED_K0));
-    HIP_TRY(hipEventRecord(e1, d->stream));
-    HIP_TRY(hipEventSynchronize(e1));
+    HIP_TRY(hipEventRecord(ev.e1, d->stream));
     float el = 0.0f;
-    HIP_TRY(hipEventElapsedTime(&el, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIP_TRY(ev.elapsed(&el));
     if (ms) *ms = (double)el / n_iter;
     if (pairs) *pairs = d->pairs_per_eval;
     return RD_OK;
@@ -876,55 +910,55 @@ static int time_force_stage(rd_disk* d, uint32_t n_iter, double* ms)
     a.fpos[0] = d->x; a.fpos[1] = d->y; a.fpos[2] = d->z;
     HIP_TRY(enqueue_force(d, d->x, d->y, d->z, F_NEED_K0));
     HIP_TRY(launch_stage(d->stream, RD_RKF78, 1, a));
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventRecord(e0, d->stream));
+    EventPair ev;
+    HIP_TRY(ev.create());
+    HIP_TRY(hipEventRecord(ev.e0, d->stream));
     for (uint32_t q = 0; q < n_iter; ++q) {
         HIP_TRY(enqueue_force(d, d->x, d->y, d->z, F_NEED_K0));
         HIP_TRY(launch_stage(d->stream, RD_RKF78, 1, a));
     }
-    HIP_TRY(hipEventRecord(e1, d->stream));
-    HIP_TRY(hipEventSynchronize(e1));
+    HIP_TRY(hipEventRecord(ev.e1, d->stream));
     float el = 0.0f;
-    HIP_TRY(hipEventElapsedTime(&el, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIP_TRY(ev.elapsed(&el));
     *ms = (double)el / n_iter;
     return RD_OK;
 }
 
 int rd_bench_copy(int device, size_t bytes, uint32_t n_iter, double* ms)
 {
+    DeviceScope dev_scope(device);
     if (bytes < 32 || n_iter == 0) return fail(RD_EINVAL, "bad argument");
     HIP_TRY(hipSetDevice(device));
-    hipStream_t st;
-    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    void *a = nullptr, *b = nullptr;
-    HIP_TRY(hipMalloc(&a, bytes));
-    HIP_TRY(hipMalloc(&b, bytes));
-    HIP_TRY(hipMemsetAsync(a, 0, bytes, st));
-    HIP_TRY(launch_copy(st, a, b, bytes));
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventRecord(e0, st));
-    for (uint32_t q = 0; q < n_iter; ++q) HIP_TRY(launch_copy(st, a, b, bytes));
-    HIP_TRY(hipEventRecord(e1, st));
-    HIP_TRY(hipEventSynchronize(e1));
+    struct Res {   // released on every return path
+        hipStream_t st = nullptr;
+        void *a = nullptr, *b = nullptr;
+        ~Res()
+        {
+            if (st) (void)hipStreamSynchronize(st);
+            if (a) (void)hipFree(a);
+            if (b) (void)hipFree(b);
+            if (st) (void)hipStreamDestroy(st);
+        }
+    } res;
+    HIP_TRY(hipStreamCreateWithFlags(&res.st, hipStreamNonBlocking));
+    HIP_TRY(hipMalloc(&res.a, bytes));
+    HIP_TRY(hipMalloc(&res.b, bytes));
+    HIP_TRY(hipMemsetAsync(res.a, 0, bytes, res.st));
+    HIP_TRY(launch_copy(res.st, res.a, res.b, bytes));
+    EventPair ev;
+    HIP_TRY(ev.create());
+    HIP_TRY(hipEventRecord(ev.e0, res.st));
+    for (uint32_t q = 0; q < n_iter; ++q) HIP_TRY(launch_copy(res.st, res.a, res.b, bytes));
+    HIP_TRY(hipEventRecord(ev.e1, res.st));
     float el = 0.0f;
-    HIP_TRY(hipEventElapsedTime(&el, e0, e1));
+    HIP_TRY(ev.elapsed(&el));
     if (ms) *ms = (double)el / n_iter;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipFree(a);
-    (void)hipFree(b);
-    (void)hipStreamDestroy(st);
     return RD_OK;
 }
 
 int rd_bench_force_stage(rd_disk* d, uint32_t n_iter, double* ms)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || n_iter == 0 || !ms) return fail(RD_EINVAL, "bad argument");
     if (int rc = ensure_lists(d)) return rc;
     return time_force_stage(d, n_iter, ms);
@@ -932,6 +966,7 @@ int rd_bench_force_stage(rd_disk* d, uint32_t n_iter, double* ms)
 
 int rd_autotune(rd_disk* d, rd_tuning* best)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d) return fail(RD_EINVAL, "NULL disk");
     if (d->precision != RD_PREC_MIXED) {
         if (best) *best = d->tuning;
@@ -969,6 +1004,7 @@ int rd_autotune(rd_disk* d, rd_tuning* best)
 // ---- integrators ---------------------------------------------------------------
 int rd_integrator_create(rd_disk* d, int method, double dt, int adaptive, double tol, rd_integrator** out)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || !out) return fail(RD_EINVAL, "NULL argument");
     if (method != RD_RK4 && method != RD_RKF78) return fail(RD_ENOTIMPL, "only RK4 and RKF78 are built");
     rd_integrator* it = new rd_integrator();
@@ -991,6 +1027,7 @@ int rd_integrator_create(rd_disk* d, int method, double dt, int adaptive, double
 
 int rd_integrator_destroy(rd_integrator* it)
 {
+    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
     if (!it) return RD_OK;
     if (it->gexec) (void)hipGraphExecDestroy(it->gexec);
     if (it->graph) (void)hipGraphDestroy(it->graph);
@@ -1199,6 +1236,7 @@ static int step_disks(rd_integrator* it, const std::vector<uint32_t>& want)
 
 int rd_step(rd_integrator* it, double* dt_did)
 {
+    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     std::vector<uint32_t> want(it->d->n_disk, 1u);
     if (int rc = step_disks(it, want)) return rc;
@@ -1208,6 +1246,7 @@ int rd_step(rd_integrator* it, double* dt_did)
 
 int rd_steps(rd_integrator* it, uint32_t n_steps)
 {
+    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     rd_disk* d = it->d;
     if (!it->adaptive) {
@@ -1250,6 +1289,7 @@ int rd_steps(rd_integrator* it, uint32_t n_steps)
 
 int rd_get_dt_next(rd_integrator* it, uint32_t k, double* dt)
 {
+    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
     if (!it || !dt || k >= it->d->n_disk) return fail(RD_EINVAL, "bad argument");
     *dt = it->dt_next[k];
     return RD_OK;
@@ -1257,6 +1297,7 @@ int rd_get_dt_next(rd_integrator* it, uint32_t k, double* dt)
 
 int rd_set_dt_next(rd_integrator* it, uint32_t k, double dt)
 {
+    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
     if (!it || k >= it->d->n_disk) return fail(RD_EINVAL, "bad argument");
     it->dt_next[k] = dt;
     return RD_OK;
@@ -1264,6 +1305,7 @@ int rd_set_dt_next(rd_integrator* it, uint32_t k, double dt)
 
 int rd_counters(rd_integrator* it, uint32_t k, uint64_t* tried, uint64_t* passed, uint64_t* failed)
 {
+    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
     if (!it || k >= it->d->n_disk) return fail(RD_EINVAL, "bad argument");
     if (tried) *tried = it->tried[k];
     if (passed) *passed = it->passed[k];
@@ -1273,6 +1315,7 @@ int rd_counters(rd_integrator* it, uint32_t k, uint64_t* tried, uint64_t* passed
 
 int rd_set_error_check_for_tp(rd_integrator* it, int on)
 {
+    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     it->err_all = on ? 1 : 0;
     return RD_OK;
@@ -1280,6 +1323,7 @@ int rd_set_error_check_for_tp(rd_integrator* it, int on)
 
 int rd_integrator_set_proposer(rd_integrator* it, rd_mlp* m, float safety)
 {
+    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
     if (!it || !(safety > 0.0f)) return fail(RD_EINVAL, "bad argument");
     if (m && !it->adaptive) return fail(RD_ESTATE, "the proposer needs an adaptive integrator");
     if (m && mlp_device(m) != it->d->device) return fail(RD_EINVAL, "proposer and disk are on different devices");
@@ -1290,6 +1334,7 @@ int rd_integrator_set_proposer(rd_integrator* it, rd_mlp* m, float safety)
 
 int rd_integrator_set_logging(rd_integrator* it, int on)
 {
+    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     if (on && !it->adaptive) return fail(RD_ESTATE, "logging needs an adaptive integrator");
     it->logging = on != 0;
@@ -1298,6 +1343,7 @@ int rd_integrator_set_logging(rd_integrator* it, int on)
 
 int rd_integrator_log(rd_integrator* it, float* x, float* t, uint32_t cap, uint32_t* n)
 {
+    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     const size_t cnt = it->log_t.size();
     const size_t m = std::min<size_t>(cap, cnt);
@@ -1309,6 +1355,7 @@ int rd_integrator_log(rd_integrator* it, float* x, float* t, uint32_t cap, uint3
 
 int rd_integrator_clear_log(rd_integrator* it)
 {
+    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     it->log_x.clear();
     it->log_t.clear();
@@ -1317,6 +1364,7 @@ int rd_integrator_clear_log(rd_integrator* it)
 
 int rd_work_counters(rd_integrator* it, uint64_t* evals, double* pairs)
 {
+    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     if (evals) *evals = it->force_evals;
     if (pairs) *pairs = it->pairs;
@@ -1535,6 +1583,7 @@ extern "C" {
 
 int rd_check_ejection_hit_centrum(rd_disk* d, uint32_t k, uint32_t* n)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
     std::vector<char> want(d->n_disk, 0);
     want[k] = 1;
@@ -1545,6 +1594,7 @@ int rd_check_ejection_hit_centrum(rd_disk* d, uint32_t k, uint32_t* n)
 
 int rd_check_collision(rd_disk* d, uint32_t k, uint32_t* n)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
     std::vector<char> want(d->n_disk, 0);
     want[k] = 1;
@@ -1555,6 +1605,7 @@ int rd_check_collision(rd_disk* d, uint32_t k, uint32_t* n)
 
 int rd_get_events(rd_disk* d, uint32_t k, rd_event* out, uint32_t cap, uint32_t* n)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
     const auto& p = d->hd[k].pending;
     if (out) std::memcpy(out, p.data(), std::min<size_t>(cap, p.size()) * sizeof(rd_event));
@@ -1564,6 +1615,7 @@ int rd_get_events(rd_disk* d, uint32_t k, rd_event* out, uint32_t cap, uint32_t*
 
 int rd_event_log(rd_disk* d, uint32_t k, rd_event* out, uint32_t cap, uint32_t* n)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
     const auto& p = d->hd[k].log;
     if (out) std::memcpy(out, p.data(), std::min<size_t>(cap, p.size()) * sizeof(rd_event));
@@ -1573,6 +1625,7 @@ int rd_event_log(rd_disk* d, uint32_t k, rd_event* out, uint32_t cap, uint32_t* 
 
 int rd_clear_event_counter(rd_disk* d, uint32_t k)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
     d->hd[k].pending.clear();
     return RD_OK;
@@ -1580,6 +1633,7 @@ int rd_clear_event_counter(rd_disk* d, uint32_t k)
 
 int rd_handle_collision(rd_disk* d, uint32_t k)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
     HostDisk& h = d->hd[k];
     if (h.pending.empty()) return RD_OK;
@@ -1613,6 +1667,7 @@ int rd_handle_collision(rd_disk* d, uint32_t k)
 
 int rd_handle_ejection_hit_centrum(rd_disk* d, uint32_t k)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
     HostDisk& h = d->hd[k];
     if (h.pending.empty()) return RD_OK;
@@ -1633,6 +1688,7 @@ int rd_handle_ejection_hit_centrum(rd_disk* d, uint32_t k)
 
 int rd_rebuild_vectors(rd_disk* d, uint32_t k)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
     // download everything, compact disk k, re-lay all disks
     std::vector<HostView> all(d->n_disk);
@@ -1684,6 +1740,7 @@ int rd_rebuild_vectors(rd_disk* d, uint32_t k)
 
 int rd_calc_integral(rd_disk* d, uint32_t k, rd_integral* out)
 {
+    DeviceScope dev_scope(d ? d->device : -1);
     if (!d || !out || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
     const HostDisk& h = d->hd[k];
     const size_t nch = (h.n + 2047) / 2048;
@@ -1710,6 +1767,7 @@ int rd_calc_integral(rd_disk* d, uint32_t k, rd_integral* out)
 // lock step: each iteration every running disk takes one accepted step.
 int rd_run(rd_integrator* it, double length, double output_interval, uint64_t max_steps)
 {
+    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     rd_disk* d = it->d;
     const uint32_t nd = d->n_disk;

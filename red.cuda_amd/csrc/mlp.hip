@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/red_abi.h"
+#include "device_scope.hpp"
 #include "mlp.hpp"
 
 namespace rdamd {
@@ -304,6 +305,7 @@ extern "C" {
 
 int rd_mlp_create(int device, uint32_t seed, rd_mlp** out)
 {
+    DeviceScope dev_scope(device);
     if (!out) return mfail(RD_EINVAL, "NULL argument");
     rd_mlp* p = new rd_mlp();
     p->device = device;
@@ -314,11 +316,15 @@ int rd_mlp_create(int device, uint32_t seed, rd_mlp** out)
         delete p;
         return mfail(RD_EDEVICE, std::string("rd_mlp_create: ") + hipGetErrorString(e));
     }
-    MLP_TRY(hipMalloc((void**)&p->w, kNParam * sizeof(float)));
-    MLP_TRY(hipMalloc((void**)&p->m, kNParam * sizeof(float)));
-    MLP_TRY(hipMalloc((void**)&p->v, kNParam * sizeof(float)));
-    MLP_TRY(hipMalloc((void**)&p->g, (kNParam + 1) * sizeof(float)));
-    MLP_TRY(hipMalloc((void**)&p->wb, kNParam * sizeof(uint16_t)));
+    e = hipMalloc((void**)&p->w, kNParam * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->m, kNParam * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->v, kNParam * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->g, (kNParam + 1) * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&p->wb, kNParam * sizeof(uint16_t));
+    if (e != hipSuccess) {
+        rd_mlp_destroy(p);
+        return mfail(RD_ENOMEM, std::string("rd_mlp_create: hipMalloc: ") + hipGetErrorString(e));
+    }
     // Glorot-uniform weights, zero biases
     std::vector<float> w(kNParam, 0.0f);
     auto fill = [&](int off, int fan_in, int fan_out, int count) {
@@ -332,14 +338,18 @@ int rd_mlp_create(int device, uint32_t seed, rd_mlp** out)
     for (int i = 0; i < kIn; ++i) { p->nm.mean[i] = 0.0f; p->nm.inv_std[i] = 1.0f; }
     p->nm.t_mean = 0.0f;
     p->nm.t_std = 1.0f;
+    if (int rc = rd_mlp_set_params(p, w.data())) {
+        rd_mlp_destroy(p);
+        return rc;
+    }
     *out = p;
-    return rd_mlp_set_params(p, w.data());
+    return RD_OK;
 }
 
 int rd_mlp_destroy(rd_mlp* p)
 {
+    DeviceScope dev_scope(p ? p->device : -1);
     if (!p) return RD_OK;
-    (void)hipSetDevice(p->device);
     (void)hipStreamSynchronize(p->stream);
     for (void* q : {(void*)p->w, (void*)p->m, (void*)p->v, (void*)p->g, (void*)p->wb, (void*)p->part, (void*)p->dx,
                     (void*)p->dt, (void*)p->dy, (void*)p->didx})
@@ -351,6 +361,7 @@ int rd_mlp_destroy(rd_mlp* p)
 
 int rd_mlp_set_norm(rd_mlp* p, const float mean[RD_MLP_IN], const float stdev[RD_MLP_IN], float t_mean, float t_std)
 {
+    DeviceScope dev_scope(p ? p->device : -1);
     if (!p || !mean || !stdev || !(t_std > 0.0f)) return mfail(RD_EINVAL, "bad argument");
     for (int i = 0; i < kIn; ++i) {
         if (!(stdev[i] > 0.0f)) return mfail(RD_EINVAL, "feature std must be > 0");
@@ -364,6 +375,7 @@ int rd_mlp_set_norm(rd_mlp* p, const float mean[RD_MLP_IN], const float stdev[RD
 
 int rd_mlp_set_params(rd_mlp* p, const float* params)
 {
+    DeviceScope dev_scope(p ? p->device : -1);
     if (!p || !params) return mfail(RD_EINVAL, "bad argument");
     MLP_TRY(hipMemcpyAsync(p->w, params, kNParam * sizeof(float), hipMemcpyHostToDevice, p->stream));
     MLP_TRY(hipMemsetAsync(p->m, 0, kNParam * sizeof(float), p->stream));
@@ -378,6 +390,7 @@ int rd_mlp_set_params(rd_mlp* p, const float* params)
 
 int rd_mlp_get_params(rd_mlp* p, float* params, uint16_t* params_bf16)
 {
+    DeviceScope dev_scope(p ? p->device : -1);
     if (!p) return mfail(RD_EINVAL, "bad argument");
     if (params) MLP_TRY(hipMemcpyAsync(params, p->w, kNParam * sizeof(float), hipMemcpyDeviceToHost, p->stream));
     if (params_bf16)
@@ -388,6 +401,7 @@ int rd_mlp_get_params(rd_mlp* p, float* params, uint16_t* params_bf16)
 
 int rd_mlp_forward(rd_mlp* p, const float* x, uint32_t n, float* y)
 {
+    DeviceScope dev_scope(p ? p->device : -1);
     if (!p || (!x && n) || (!y && n)) return mfail(RD_EINVAL, "bad argument");
     if (n == 0) return RD_OK;
     if (int rc = ensure_cap(p, n)) return rc;
@@ -412,6 +426,7 @@ static int grad_dev(rd_mlp* p, const uint32_t* idx, uint32_t n, float scale)
 
 int rd_mlp_loss_grad(rd_mlp* p, const float* x, const float* t, uint32_t n, float* loss, float* grad)
 {
+    DeviceScope dev_scope(p ? p->device : -1);
     if (!p || !x || !t || n == 0) return mfail(RD_EINVAL, "bad argument");
     if (int rc = ensure_cap(p, n)) return rc;
     MLP_TRY(hipMemcpyAsync(p->dx, x, (size_t)n * kIn * sizeof(float), hipMemcpyHostToDevice, p->stream));
@@ -428,6 +443,7 @@ int rd_mlp_loss_grad(rd_mlp* p, const float* x, const float* t, uint32_t n, floa
 int rd_mlp_train(rd_mlp* p, const float* x, const float* t, uint32_t n, uint32_t epochs, uint32_t batch, float lr,
                  float* final_loss)
 {
+    DeviceScope dev_scope(p ? p->device : -1);
     if (!p || !x || !t || n == 0 || batch == 0) return mfail(RD_EINVAL, "bad argument");
     if (int rc = ensure_cap(p, n)) return rc;
     MLP_TRY(hipMemcpyAsync(p->dx, x, (size_t)n * kIn * sizeof(float), hipMemcpyHostToDevice, p->stream));
