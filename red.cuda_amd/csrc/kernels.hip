@@ -106,11 +106,19 @@ __global__ __launch_bounds__(kBlock) void k_force_exact(const WorkItem* __restri
         const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0, zi = valid ? z[i] : 0.0;
         const uint32_t si = w.sink_lo + sc_g;  // sink of a summing lane
         double acc = 0.0;
+        // the next tile's sources are loaded into registers while the current
+        // tile is worked on, so global-load latency is paid once per item
+        double4 nxt = make_double4(0.0, 0.0, 0.0, -1.0);
+        if (tid < kExactTile && w.src_lo + tid < w.src_hi) {
+            const uint32_t j = w.src_lo + tid;
+            nxt = make_double4(x[j], y[j], z[j], m_ex[j]);
+        }
         for (uint32_t t0 = w.src_lo; t0 < w.src_hi; t0 += kExactTile) {
             __syncthreads();
             if (tid < kExactTile) {
-                const uint32_t j = t0 + tid;
-                src[tid] = j < w.src_hi ? make_double4(x[j], y[j], z[j], m_ex[j]) : make_double4(0.0, 0.0, 0.0, -1.0);
+                src[tid] = nxt;
+                const uint32_t j = t0 + kExactTile + tid;
+                nxt = j < w.src_hi ? make_double4(x[j], y[j], z[j], m_ex[j]) : make_double4(0.0, 0.0, 0.0, -1.0);
             }
             __syncthreads();
 #pragma unroll
