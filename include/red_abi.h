@@ -209,6 +209,13 @@ int rd_handle_ejection_hit_centrum(rd_disk* d, uint32_t k);
 int rd_clear_event_counter(rd_disk* d, uint32_t k);
 /* all events handled so far on disk k, in handling order (what print_event_data writes) */
 int rd_event_log(rd_disk* d, uint32_t k, rd_event* out, uint32_t cap, uint32_t* n);
+/* event counters of disk k, each indexed by RD_EVENT_COUNTER_* (pp_disk.h:164-167:
+ * n_event, n_collision, n_ejection, n_hit_centrum); any pointer may be NULL */
+enum { RD_EVENT_COUNTER_TOTAL = 0, RD_EVENT_COUNTER_LAST_CLEAR = 1, RD_EVENT_COUNTER_LAST_STEP = 2 };
+int rd_event_counters(rd_disk* d, uint32_t k, uint32_t n_event[3], uint32_t n_collision[3], uint32_t n_ejection[3],
+                      uint32_t n_hit_centrum[3]);
+/* pp_disk::set_event_counter (pp_disk.cu:1442-1448): field of every counter set to value */
+int rd_set_event_counter(rd_disk* d, uint32_t k, int field, uint32_t value);
 /* pp_disk::rebuild_vectors (pp_disk.cu:1370-1430) on disk k */
 int rd_rebuild_vectors(rd_disk* d, uint32_t k);
 

@@ -61,6 +61,8 @@ struct HostDisk {
     std::vector<rd_event> pending;
     std::vector<rd_event> log;
     uint32_t n_event[3] = {0, 0, 0};   // TOTAL, LAST_CLEAR, LAST_STEP (red_type.h:154-160)
+    // the same counters per kind: collision, ejection, hit-centrum (pp_disk.h:164-166)
+    uint32_t n_kind[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 
     uint32_t n_si() const { return playing[0] + playing[1] + playing[2] + playing[3]; }
     uint32_t n_nsi() const { return playing[4] + playing[5]; }
@@ -1462,9 +1464,14 @@ int push_back_disk(rd_disk* d, uint32_t k, const HostView& hv)
     return RD_OK;
 }
 
-void inc_counters(HostDisk& h)
+// pp_disk::increment_event_counter (pp_disk.cu:1431-1440); kind 0 collision,
+// 1 ejection, 2 hit-centrum
+void inc_counters(HostDisk& h, int kind)
 {
-    for (int q = 0; q < 3; ++q) h.n_event[q]++;
+    for (int q = 0; q < 3; ++q) {
+        h.n_event[q]++;
+        h.n_kind[kind][q]++;
+    }
 }
 
 }  // namespace
@@ -1657,11 +1664,12 @@ int rd_handle_collision(rd_disk* d, uint32_t k)
     for (auto& e : sp) {
         const uint32_t ictv = e.idx2;
         if (int rc = merge_pair(hv, bmd, e)) return rc;
-        inc_counters(h);
+        inc_counters(h, 0);
         bmd[ictv].id *= -1;
         h.inactive[bmd[ictv].body_type]++;
         h.log.push_back(e);
     }
+    h.n_kind[0][2] = 0;   // pp_disk.cu:1247
     return push_back_disk(d, k, hv);
 }
 
@@ -1678,12 +1686,36 @@ int rd_handle_ejection_hit_centrum(rd_disk* d, uint32_t k)
         const uint32_t ictv = e.idx2;
         if (RD_EVENT_EJECTION != e.event_name)
             if (int rc = merge_pair(hv, bmd, e)) return rc;
-        inc_counters(h);
+        inc_counters(h, RD_EVENT_EJECTION == e.event_name ? 1 : 2);
         bmd[ictv].id *= -1;
         h.inactive[bmd[ictv].body_type]++;
         h.log.push_back(e);
     }
+    h.n_kind[1][2] = h.n_kind[2][2] = 0;   // pp_disk.cu:1324-1325
     return push_back_disk(d, k, hv);
+}
+
+int rd_event_counters(rd_disk* d, uint32_t k, uint32_t n_event[3], uint32_t n_collision[3], uint32_t n_ejection[3],
+                      uint32_t n_hit_centrum[3])
+{
+    if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    const HostDisk& h = d->hd[k];
+    for (int q = 0; q < 3; ++q) {
+        if (n_event) n_event[q] = h.n_event[q];
+        if (n_collision) n_collision[q] = h.n_kind[0][q];
+        if (n_ejection) n_ejection[q] = h.n_kind[1][q];
+        if (n_hit_centrum) n_hit_centrum[q] = h.n_kind[2][q];
+    }
+    return RD_OK;
+}
+
+int rd_set_event_counter(rd_disk* d, uint32_t k, int field, uint32_t value)
+{
+    if (!d || k >= d->n_disk || field < 0 || field > 2) return fail(RD_EINVAL, "bad argument");
+    HostDisk& h = d->hd[k];
+    h.n_event[field] = value;
+    for (int q = 0; q < 3; ++q) h.n_kind[q][field] = value;
+    return RD_OK;
 }
 
 int rd_rebuild_vectors(rd_disk* d, uint32_t k)
@@ -1813,7 +1845,7 @@ int rd_run(rd_integrator* it, double length, double output_interval, uint64_t ma
             if (!want[k]) continue;
             if (output_interval <= std::fabs(ps[k])) ps[k] = 0.0;
             if (16 <= d->hd[k].n_event[1]) {
-                d->hd[k].n_event[1] = 0;
+                rd_set_event_counter(d, k, RD_EVENT_COUNTER_LAST_CLEAR, 0);
                 if (int rc = rd_rebuild_vectors(d, k)) return rc;
             }
         }

@@ -105,6 +105,23 @@ class pp_disk {
     void handle_collision() { rd_check(rd_handle_collision(h_, 0), "handle_collision"); }
     void handle_ejection_hit_centrum() { rd_check(rd_handle_ejection_hit_centrum(h_, 0), "handle_ejection_hit_centrum"); }
     void clear_event_counter() { rd_check(rd_clear_event_counter(h_, 0), "clear_event_counter"); }
+    // n_event / n_collision / n_ejection / n_hit_centrum (pp_disk.h:164-167), each indexed by
+    // RD_EVENT_COUNTER_TOTAL / _LAST_CLEAR / _LAST_STEP
+    struct event_counters {
+        uint32_t n_event[3], n_collision[3], n_ejection[3], n_hit_centrum[3];
+    };
+    event_counters counters_of_events() const
+    {
+        event_counters c;
+        rd_check(rd_event_counters(h_, 0, c.n_event, c.n_collision, c.n_ejection, c.n_hit_centrum),
+                 "event_counters");
+        return c;
+    }
+    // pp_disk::set_event_counter (pp_disk.cu:1442-1448)
+    void set_event_counter(int field, uint32_t value)
+    {
+        rd_check(rd_set_event_counter(h_, 0, field, value), "set_event_counter");
+    }
     void rebuild_vectors() { rd_check(rd_rebuild_vectors(h_, 0), "rebuild_vectors"); }
     std::vector<rd_event> event_log() const
     {

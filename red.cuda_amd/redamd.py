@@ -23,7 +23,8 @@ from redamd_types import (VAR4, PARAM, BODY_MD, INTEGRAL, EVENT, GasParams, Tuni
                           RD_OK, RD_EINVAL, RD_EDEVICE, RD_ENOTIMPL, RD_EOVERFLOW, RD_ESTATE, RD_ENOMEM,
                           RD_RK4, RD_RKF78, RD_PREC_FP64, RD_PREC_MIXED, RD_DRAG_NSI, RD_DRAG_ALL_MASSIVE,
                           BODY_STAR, BODY_GIANT, BODY_ROCKY, BODY_PROTO, BODY_SUPERPL, BODY_PL, BODY_TP,
-                          EVENT_NONE, EVENT_HIT_CENTRUM, EVENT_EJECTION, EVENT_COLLISION, K_GAUSS)
+                          EVENT_NONE, EVENT_HIT_CENTRUM, EVENT_EJECTION, EVENT_COLLISION, K_GAUSS,
+                          RD_EVENT_COUNTER_TOTAL, RD_EVENT_COUNTER_LAST_CLEAR, RD_EVENT_COUNTER_LAST_STEP)
 
 
 class RedError(RuntimeError):
@@ -80,6 +81,8 @@ _SIGS = {
     "rd_handle_ejection_hit_centrum": (C.c_int, [_P, _u32]),
     "rd_clear_event_counter": (C.c_int, [_P, _u32]),
     "rd_event_log": (C.c_int, [_P, _u32, _P, _u32, C.POINTER(_u32)]),
+    "rd_event_counters": (C.c_int, [_P, _u32, _P, _P, _P, _P]),
+    "rd_set_event_counter": (C.c_int, [_P, _u32, C.c_int, _u32]),
     "rd_rebuild_vectors": (C.c_int, [_P, _u32]),
     "rd_calc_integral": (C.c_int, [_P, _u32, _P]),
     "rd_make_disk": (C.c_int, [C.POINTER(DiskSpec), _P, _P, _P, _P]),
@@ -293,6 +296,15 @@ class Disk:
         out = np.zeros(n.value, EVENT)
         check(lib.rd_event_log(self.h, k, _ptr(out), n.value, C.byref(n)))
         return out
+
+    def event_counters(self, k=0):
+        """{'event'|'collision'|'ejection'|'hit_centrum': [total, since last clear, last step]}"""
+        arrs = [np.zeros(3, np.uint32) for _ in range(4)]
+        check(lib.rd_event_counters(self.h, k, *[_ptr(a) for a in arrs]))
+        return dict(zip(("event", "collision", "ejection", "hit_centrum"), (a.tolist() for a in arrs)))
+
+    def set_event_counter(self, field, value, k=0):
+        check(lib.rd_set_event_counter(self.h, k, field, value))
 
     def handle_collision(self, k=0):
         check(lib.rd_handle_collision(self.h, k))

@@ -180,6 +180,17 @@ def test_driver_loop_events_match_oracle():
     assert t == ot
     assert bits_equal(rr, orr) and bits_equal(vv, ovv) and bits_equal(pp, opp) and bits_equal(bb, obb)
     assert all(np.array_equal(a, b_) for a, b_ in zip(d.n_objects(), o.counts()))
+    # event counters (pp_disk.h:164-167) agree with the log: totals per kind, and
+    # the since-last-clear count was reset by the rebuild after 16 events
+    c = d.event_counters()
+    kinds = ev["event_name"]
+    assert c["event"][0] == len(ev)
+    assert c["collision"][0] == int(np.sum(kinds == R.EVENT_COLLISION))
+    assert c["ejection"][0] == int(np.sum(kinds == R.EVENT_EJECTION))
+    assert c["hit_centrum"][0] == int(np.sum(kinds == R.EVENT_HIT_CENTRUM))
+    assert c["event"][1] < 16 and c["collision"][2] == 0
+    d.set_event_counter(R.RD_EVENT_COUNTER_TOTAL, 7)
+    assert all(v[0] == 7 for v in d.event_counters().values())
 
 
 def test_ejection_and_hit_centrum():
