@@ -96,7 +96,14 @@ typedef struct rd_tuning {
     uint32_t chunk;            /* mixed kernel: sources per work item (mult. of 256) */
     uint32_t blocks_per_cu;    /* persistent grid size = blocks_per_cu * CUs      */
     uint32_t use_graph;        /* capture one RK attempt in a hipGraph            */
+    uint32_t pair_mode;        /* mixed kernel, SI x SI pairs: RD_PAIRS_*         */
 } rd_tuning;
+
+/* Mixed precision: RD_PAIRS_SYMMETRIC evaluates each SI x SI pair once and
+ * applies it to both bodies (the reference CPU path's i < j scheme,
+ * pp_disk.cu:822-865); RD_PAIRS_ONE_SIDED evaluates every ordered pair (the
+ * reference GPU kernel's scheme, pp_disk.cu:374-416). */
+enum { RD_PAIRS_ONE_SIDED = 0, RD_PAIRS_SYMMETRIC = 1 };
 
 typedef struct rd_disk rd_disk;
 typedef struct rd_integrator rd_integrator;

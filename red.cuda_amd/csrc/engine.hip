@@ -151,6 +151,8 @@ struct rd_disk {
     // work lists
     WorkItem* items_force = nullptr;
     uint32_t n_items_force = 0, grid_force = 0;
+    WorkItem* items_sym = nullptr;   // mixed precision, symmetric SI x SI pairs
+    uint32_t n_items_sym = 0, grid_sym = 0, sym_R = 1;
     WorkItem* items_coll = nullptr;
     uint32_t n_items_coll = 0;
     WorkItem* items_feat = nullptr;
@@ -176,7 +178,7 @@ struct rd_disk {
 
     int precision = RD_PREC_FP64;
     int drag_mode = RD_DRAG_NSI;
-    rd_tuning tuning{2, 1024, 4, 0};
+    rd_tuning tuning{2, 1024, 4, 0, 1};
     double thr[3] = {0, 0, 0};
     GasDev gas{};
     uint64_t generation = 0;       // bumps when buffers / lists change (invalidates graphs)
@@ -184,6 +186,7 @@ struct rd_disk {
 
 struct rd_integrator {
     rd_disk* d = nullptr;
+    int device = -1;   // the disk's device, kept so that destroy never reads the disk
     int method = RD_RKF78;
     int adaptive = 1;
     double tol = 0.0;
@@ -238,6 +241,7 @@ void free_buffers(rd_disk* d)
     dev_free(d->ctl);
     dev_free(d->err);
     dev_free(d->items_force);
+    dev_free(d->items_sym);
     dev_free(d->items_coll);
     dev_free(d->items_feat);
     dev_free(d->ev_buf);
@@ -421,6 +425,9 @@ int build_lists(rd_disk* d)
         d->exact_lane = groups >= 4 * (size_t)d->n_cu;
     }
     const uint32_t exact_group = d->exact_lane ? (uint32_t)kBlock : kExactSinks;
+    const bool sym = d->precision == RD_PREC_MIXED && d->tuning.pair_mode == RD_PAIRS_SYMMETRIC;
+    std::vector<std::pair<uint32_t, WorkItem>> sym_items;   // (tiles, item)
+    d->sym_R = 1;
     for (auto& o : order) {
         const uint32_t k = o.second;
         const HostDisk& h = d->hd[k];
@@ -438,16 +445,39 @@ int build_lists(rd_disk* d)
                     items.push_back(WorkItem{k, h.off + s, h.off + std::min(b[1], s + exact_group), h.off + b[2],
                                              h.off + b[3], 0, 0, 0});
             } else {
-                const uint32_t lo = std::max(b[2], di.n_fp64), hi = b[3];
+                // symmetric: SI x SI pairs once each (the FP64 prefix takes part
+                // with FP32 mass 0: its own sink sums are kept, it pulls no one);
+                // the SI sinks' NSI sources stay one-sided, in slots after them
+                uint32_t base = 0, lo = std::max(b[2], di.n_fp64);
+                if (sym && cls == 0) {
+                    const uint32_t nblk = (b[1] + kSymBlockBodies - 1) / kSymBlockBodies;
+                    // kSymChunks..2 kSymChunks - 1 chunks once the disk is that large:
+                    // enough items to fill the device, few partial-sum slots
+                    const uint32_t R = std::min<uint32_t>(kSymMaxR, std::max<uint32_t>(1, nblk / kSymChunks));
+                    const uint32_t nc = (nblk + R - 1) / R, span = R * kSymBlockBodies;
+                    d->sym_R = std::max(d->sym_R, R);
+                    for (uint32_t a = 0; a < nc; ++a)
+                        for (uint32_t c = a; c < nc; ++c) {
+                            const uint32_t ia = std::min(nblk - a * R, R), ic = std::min(nblk - c * R, R);
+                            const uint32_t tiles = a == c ? ia * (ia + 1) / 2 : ia * ic;
+                            sym_items.push_back({tiles, WorkItem{k, h.off + a * span, h.off + std::min(b[1], (a + 1) * span),
+                                                                 h.off + c * span, h.off + std::min(b[1], (c + 1) * span),
+                                                                 c, a, 0}});
+                        }
+                    base = nc;
+                    lo = std::max(lo, b[1]);
+                }
+                const uint32_t hi = b[3];
                 const uint32_t nsrc = hi > lo ? hi - lo : 0;
                 const uint32_t nch = (nsrc + C - 1) / C;
-                di.nchunk[cls] = nch;
-                max_chunks = std::max<size_t>(max_chunks, nch);
+                di.nchunk[cls] = base + nch;
+                max_chunks = std::max<size_t>(max_chunks, base + nch);
                 const uint32_t G = 64u * S;
                 for (uint32_t s = b[0]; s < b[1]; s += G)
                     for (uint32_t c = 0; c < nch; ++c)
                         items.push_back(WorkItem{k, h.off + s, h.off + std::min(b[1], s + G),
-                                                 h.off + lo + c * C, h.off + std::min(hi, lo + (c + 1) * C), c, 0, 0});
+                                                 h.off + lo + c * C, h.off + std::min(hi, lo + (c + 1) * C), base + c,
+                                                 0, 0});
             }
         }
     }
@@ -485,13 +515,22 @@ int build_lists(rd_disk* d)
                 }
         }
     }
+    // largest items first; ties keep the disk order
+    std::stable_sort(sym_items.begin(), sym_items.end(), [](auto& a, auto& b) { return a.first > b.first; });
+    std::vector<WorkItem> symv;
+    for (auto& q : sym_items) symv.push_back(q.second);
     dev_free(d->items_force);
+    dev_free(d->items_sym);
     dev_free(d->items_coll);
     dev_free(d->items_feat);
+    if (int rc = dev_alloc(&d->items_sym, symv.size())) return rc;
     if (int rc = dev_alloc(&d->items_force, items.size())) return rc;
     if (int rc = dev_alloc(&d->items_coll, coll.size())) return rc;
     if (int rc = dev_alloc(&d->items_feat, feat.size())) return rc;
     d->n_items_force = (uint32_t)items.size();
+    d->n_items_sym = (uint32_t)symv.size();
+    if (!symv.empty())
+        HIP_TRY(hipMemcpy(d->items_sym, symv.data(), symv.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
     d->n_items_coll = (uint32_t)coll.size();
     d->n_items_feat = (uint32_t)feat.size();
     if (!feat.empty())
@@ -504,6 +543,7 @@ int build_lists(rd_disk* d)
     // the exact kernel's blocks use 26 KB of LDS: 6 fit on a CU
     const uint32_t grid_cap = d->precision == RD_PREC_FP64 && !d->exact_lane ? 6u * (uint32_t)d->n_cu : maxgrid;
     d->grid_force = std::max<uint32_t>(1, std::min<uint32_t>(d->n_items_force, grid_cap));
+    d->grid_sym = std::max<uint32_t>(1, std::min<uint32_t>(d->n_items_sym, maxgrid));
     const size_t need = max_chunks * 3 * (size_t)d->n_all;
     if (need > d->part_len) {
         dev_free(d->part);
@@ -564,6 +604,9 @@ hipError_t enqueue_force(rd_disk* d, const double* fx, const double* fy, const d
     if (d->precision == RD_PREC_FP64)
         return launch_force_exact(d->stream, d->exact_lane ? 1 : 0, d->items_force, d->n_items_force, d->grid_force,
                                   fx, fy, fz, d->m_ex, d->ctl, need, d->part, d->n_all);
+    if (hipError_t e = launch_force_sym(d->stream, d->sym_R, d->items_sym, d->n_items_sym, d->grid_sym, fx, fy, fz,
+                                        d->m32, d->ctl, need, d->part, d->n_all))
+        return e;
     return launch_force_mixed(d->stream, (int)d->tuning.sinks_per_lane, d->items_force, d->n_items_force,
                               d->grid_force, fx, fy, fz, d->m32, d->ctl, need, d->part, d->n_all);
 }
@@ -776,6 +819,8 @@ int rd_disk_set_tuning(rd_disk* d, const rd_tuning* t)
         return fail(RD_EINVAL, "sinks_per_lane must be 1, 2 or 4");
     if (t->chunk < (uint32_t)kTile || t->chunk % kTile) return fail(RD_EINVAL, "chunk must be a multiple of 256");
     if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return fail(RD_EINVAL, "blocks_per_cu must be in [1, 64]");
+    if (t->pair_mode != RD_PAIRS_ONE_SIDED && t->pair_mode != RD_PAIRS_SYMMETRIC)
+        return fail(RD_EINVAL, "pair_mode must be RD_PAIRS_ONE_SIDED or RD_PAIRS_SYMMETRIC");
     d->tuning = *t;
     d->lists_valid = false;
     return build_lists(d);
@@ -981,22 +1026,35 @@ int rd_autotune(rd_disk* d, rd_tuning* best)
     const uint32_t C_opts[] = {512, 1024, 2048, 4096, 8192};
     const uint32_t B_opts[] = {1, 2, 4, 8};
     uint32_t max_n = 0;
-    for (auto& h : d->hd) max_n = std::max(max_n, h.n);
-    for (uint32_t S : S_opts)
-        for (uint32_t C : C_opts) {
-            if (C > 2 * std::max<uint32_t>(max_n, kTile) && C != 512) continue;
-            for (uint32_t B : B_opts) {
-                rd_tuning t = keep;
-                t.sinks_per_lane = S;
-                t.chunk = C;
-                t.blocks_per_cu = B;
-                d->tuning = t;
-                if (int rc = build_lists(d)) { d->tuning = keep; build_lists(d); return rc; }
-                double ms = 0.0;
-                if (int rc = time_force_stage(d, 3, &ms)) { d->tuning = keep; build_lists(d); return rc; }
-                if (ms < best_ms) { best_ms = ms; win = t; }
+    bool one_sided_rest = false;   // pairs the symmetric kernel leaves to the one-sided one
+    for (auto& h : d->hd) {
+        max_n = std::max(max_n, h.n);
+        one_sided_rest |= h.n_massive() > h.n_si() || h.n > h.n_massive();
+    }
+    auto trial = [&](const rd_tuning& t) -> int {
+        d->tuning = t;
+        if (int rc = build_lists(d)) { d->tuning = keep; build_lists(d); return rc; }
+        double ms = 0.0;
+        if (int rc = time_force_stage(d, 3, &ms)) { d->tuning = keep; build_lists(d); return rc; }
+        if (ms < best_ms) { best_ms = ms; win = t; }
+        return RD_OK;
+    };
+    for (uint32_t mode : {(uint32_t)RD_PAIRS_SYMMETRIC, (uint32_t)RD_PAIRS_ONE_SIDED})
+        for (uint32_t S : S_opts)
+            for (uint32_t C : C_opts) {
+                if (C > 2 * std::max<uint32_t>(max_n, kTile) && C != 512) continue;
+                // with every pair symmetric, S and C shape nothing: one pass
+                if (mode == RD_PAIRS_SYMMETRIC && !one_sided_rest && (S != keep.sinks_per_lane || C != keep.chunk))
+                    continue;
+                for (uint32_t B : B_opts) {
+                    rd_tuning t = keep;
+                    t.sinks_per_lane = S;
+                    t.chunk = C;
+                    t.blocks_per_cu = B;
+                    t.pair_mode = mode;
+                    if (int rc = trial(t)) return rc;
+                }
             }
-        }
     d->tuning = win;
     if (int rc = build_lists(d)) return rc;
     if (best) *best = win;
@@ -1011,6 +1069,7 @@ int rd_integrator_create(rd_disk* d, int method, double dt, int adaptive, double
     if (method != RD_RK4 && method != RD_RKF78) return fail(RD_ENOTIMPL, "only RK4 and RKF78 are built");
     rd_integrator* it = new rd_integrator();
     it->d = d;
+    it->device = d->device;
     it->method = method;
     it->adaptive = adaptive ? 1 : 0;
     it->tol = tol;
@@ -1029,7 +1088,9 @@ int rd_integrator_create(rd_disk* d, int method, double dt, int adaptive, double
 
 int rd_integrator_destroy(rd_integrator* it)
 {
-    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
+    // may run after the disk is gone (destroy order is the caller's; garbage
+    // collectors pick one): touch nothing but the integrator
+    DeviceScope dev_scope(it ? it->device : -1);
     if (!it) return RD_OK;
     if (it->gexec) (void)hipGraphExecDestroy(it->gexec);
     if (it->graph) (void)hipGraphDestroy(it->graph);
@@ -1238,7 +1299,7 @@ static int step_disks(rd_integrator* it, const std::vector<uint32_t>& want)
 
 int rd_step(rd_integrator* it, double* dt_did)
 {
-    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
+    DeviceScope dev_scope(it ? it->device : -1);
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     std::vector<uint32_t> want(it->d->n_disk, 1u);
     if (int rc = step_disks(it, want)) return rc;
@@ -1248,7 +1309,7 @@ int rd_step(rd_integrator* it, double* dt_did)
 
 int rd_steps(rd_integrator* it, uint32_t n_steps)
 {
-    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
+    DeviceScope dev_scope(it ? it->device : -1);
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     rd_disk* d = it->d;
     if (!it->adaptive) {
@@ -1291,7 +1352,7 @@ int rd_steps(rd_integrator* it, uint32_t n_steps)
 
 int rd_get_dt_next(rd_integrator* it, uint32_t k, double* dt)
 {
-    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
+    DeviceScope dev_scope(it ? it->device : -1);
     if (!it || !dt || k >= it->d->n_disk) return fail(RD_EINVAL, "bad argument");
     *dt = it->dt_next[k];
     return RD_OK;
@@ -1299,7 +1360,7 @@ int rd_get_dt_next(rd_integrator* it, uint32_t k, double* dt)
 
 int rd_set_dt_next(rd_integrator* it, uint32_t k, double dt)
 {
-    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
+    DeviceScope dev_scope(it ? it->device : -1);
     if (!it || k >= it->d->n_disk) return fail(RD_EINVAL, "bad argument");
     it->dt_next[k] = dt;
     return RD_OK;
@@ -1307,7 +1368,7 @@ int rd_set_dt_next(rd_integrator* it, uint32_t k, double dt)
 
 int rd_counters(rd_integrator* it, uint32_t k, uint64_t* tried, uint64_t* passed, uint64_t* failed)
 {
-    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
+    DeviceScope dev_scope(it ? it->device : -1);
     if (!it || k >= it->d->n_disk) return fail(RD_EINVAL, "bad argument");
     if (tried) *tried = it->tried[k];
     if (passed) *passed = it->passed[k];
@@ -1317,7 +1378,7 @@ int rd_counters(rd_integrator* it, uint32_t k, uint64_t* tried, uint64_t* passed
 
 int rd_set_error_check_for_tp(rd_integrator* it, int on)
 {
-    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
+    DeviceScope dev_scope(it ? it->device : -1);
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     it->err_all = on ? 1 : 0;
     return RD_OK;
@@ -1325,7 +1386,7 @@ int rd_set_error_check_for_tp(rd_integrator* it, int on)
 
 int rd_integrator_set_proposer(rd_integrator* it, rd_mlp* m, float safety)
 {
-    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
+    DeviceScope dev_scope(it ? it->device : -1);
     if (!it || !(safety > 0.0f)) return fail(RD_EINVAL, "bad argument");
     if (m && !it->adaptive) return fail(RD_ESTATE, "the proposer needs an adaptive integrator");
     if (m && mlp_device(m) != it->d->device) return fail(RD_EINVAL, "proposer and disk are on different devices");
@@ -1336,7 +1397,7 @@ int rd_integrator_set_proposer(rd_integrator* it, rd_mlp* m, float safety)
 
 int rd_integrator_set_logging(rd_integrator* it, int on)
 {
-    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
+    DeviceScope dev_scope(it ? it->device : -1);
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     if (on && !it->adaptive) return fail(RD_ESTATE, "logging needs an adaptive integrator");
     it->logging = on != 0;
@@ -1345,7 +1406,7 @@ int rd_integrator_set_logging(rd_integrator* it, int on)
 
 int rd_integrator_log(rd_integrator* it, float* x, float* t, uint32_t cap, uint32_t* n)
 {
-    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
+    DeviceScope dev_scope(it ? it->device : -1);
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     const size_t cnt = it->log_t.size();
     const size_t m = std::min<size_t>(cap, cnt);
@@ -1357,7 +1418,7 @@ int rd_integrator_log(rd_integrator* it, float* x, float* t, uint32_t cap, uint3
 
 int rd_integrator_clear_log(rd_integrator* it)
 {
-    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
+    DeviceScope dev_scope(it ? it->device : -1);
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     it->log_x.clear();
     it->log_t.clear();
@@ -1366,7 +1427,7 @@ int rd_integrator_clear_log(rd_integrator* it)
 
 int rd_work_counters(rd_integrator* it, uint64_t* evals, double* pairs)
 {
-    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
+    DeviceScope dev_scope(it ? it->device : -1);
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     if (evals) *evals = it->force_evals;
     if (pairs) *pairs = it->pairs;
@@ -1799,7 +1860,7 @@ int rd_calc_integral(rd_disk* d, uint32_t k, rd_integral* out)
 // lock step: each iteration every running disk takes one accepted step.
 int rd_run(rd_integrator* it, double length, double output_interval, uint64_t max_steps)
 {
-    DeviceScope dev_scope(it && it->d ? it->d->device : -1);
+    DeviceScope dev_scope(it ? it->device : -1);
     if (!it) return fail(RD_EINVAL, "NULL integrator");
     rd_disk* d = it->d;
     const uint32_t nd = d->n_disk;

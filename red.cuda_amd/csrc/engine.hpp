@@ -21,6 +21,15 @@ constexpr int kBlock = 256;          // threads per block for every kernel
 constexpr int kTile = 256;           // sources staged in LDS per tile
 // sinks per work item of the exact (FP64, reference-order) pair kernel
 constexpr uint32_t kExactSinks = 16;
+// bodies per block of the symmetric pair kernel (64 lanes x 4 sinks)
+constexpr uint32_t kSymBlockBodies = 256;
+// largest chunk (in blocks) of a symmetric-kernel work item; its reaction
+// accumulators take 6 KB of LDS per block (8 KB + 8 x 6 KB = 56 KB at most)
+constexpr uint32_t kSymMaxR = 8;
+// the symmetric kernel groups blocks into R-block chunks, R = blocks / kSymChunks
+// (at least 1, at most kSymMaxR): a disk has kSymChunks..2 kSymChunks - 1
+// chunks (partial-sum slots per body) once it has that many blocks
+constexpr uint32_t kSymChunks = 128;
 
 // Per-disk control flags (DiskCtl::flags).
 enum : uint32_t {
@@ -54,8 +63,9 @@ struct WorkItem {
     uint32_t disk;
     uint32_t sink_lo, sink_hi;
     uint32_t src_lo, src_hi;
-    uint32_t slot;
-    uint32_t pad0, pad1;
+    uint32_t slot;       // partial-sum slot of the sinks
+    uint32_t src_slot;   // symmetric kernel: partial-sum slot of the sources
+    uint32_t pad;
 };
 
 // Analytic gas disk (analytic_gas_disk_params_t red_type.h:464-485), device copy.
@@ -95,6 +105,13 @@ hipError_t launch_force_mixed(hipStream_t s, int sinks_per_lane, const WorkItem*
                               uint32_t grid, const double* x, const double* y, const double* z,
                               const float* m32, const DiskCtl* ctl, uint32_t need_mask, double* part,
                               uint32_t n_all);
+// Symmetric pair kernel over (chunk a, chunk c) items: sink_lo/hi = chunk a,
+// src_lo/hi = chunk c (256-body blocks, R per chunk), slot = c (sink side),
+// src_slot = a (source side).
+hipError_t launch_force_sym(hipStream_t s, uint32_t R, const WorkItem* items, uint32_t n_items, uint32_t grid,
+                            const double* x, const double* y, const double* z, const float* m32,
+                            const DiskCtl* ctl, uint32_t need_mask, double* part, uint32_t n_all);
+size_t force_sym_lds_bytes(uint32_t R);
 hipError_t launch_stage(hipStream_t s, int method, int r, const StageArgs& a);
 hipError_t launch_comb1(hipStream_t s, int method, const StageArgs& a);
 hipError_t launch_commit(hipStream_t s, const StageArgs& a, int advance_time, DiskCtl* ctl_rw);

@@ -321,6 +321,251 @@ __global__ __launch_bounds__(kBlock) void k_force_mixed(const WorkItem* __restri
     }
 }
 
+// ---------------------------------------------------------------------------
+// Symmetric mixed-precision pair kernel: every unordered pair of the SI class
+// is evaluated once and applied to both bodies (the CPU path's i < j scheme,
+// pp_disk.cu:822-865), halving the pair work of the one-sided kernel.
+//
+// Bodies are cut in blocks of kSymBlock = 256 (64 lanes x 4 sinks).  A work
+// item is a pair of block chunks (a, c), a <= c, R blocks each; it runs the
+// tiles (I, J) of chunk a x chunk c with J >= I.  In a tile, each of the four
+// waves holds all 256 sinks of block I and sweeps one 64-source quarter of
+// block J: at step u lane l takes source (l - u) mod 64, so the 64 lanes
+// touch 64 different sources, and the reaction sum of each source rides one
+// lane up per step (a DPP wave rotation folded into the accumulating
+// subtract).  After 64 steps lane l holds the reaction on source (l + 1) mod 64
+// over all 256 sinks.  Diagonal tiles (I = J) are one-sided (no reaction).
+//
+// Partial-sum slots: a body of chunk k gets one slot per partner chunk p:
+// as a sink of item (k, p) for p >= k, as a source of item (p, k) for p < k;
+// in the diagonal item both land in slot k.  Every slot is written by exactly
+// one item in a fixed order, so results do not depend on the schedule.
+// Sink sums: FP32 per 64-source quarter, folded into FP64, waves combined in
+// fixed order.  Reactions: FP32 over the tile's 256 sinks, accumulated over
+// the chunk's I blocks in FP64 (LDS).
+constexpr uint32_t kSymBlock = kSymBlockBodies;
+constexpr uint32_t kSymS = kSymBlock / 64;
+
+struct alignas(16) SrcZM {
+    double z;
+    float m, pad;
+};
+
+// acc taken from lane l - 1 (wave_ror:1), minus v
+__device__ inline float rot_sub(float acc, float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(acc), 0x13C, 0xF, 0xF, false)) - v;
+}
+
+// One wave's sweep of its 64-source quarter: sink sums into f*, reactions
+// (when kReact) into r* (lane l: source (l + 1) mod 64 of the quarter).
+template <bool kReact>
+__device__ inline void sym_sweep(const double2* __restrict__ txy, const SrcZM* __restrict__ tzm, uint32_t qb,
+                                 uint32_t lane, const double* xi, const double* yi, const double* zi,
+                                 const float* mi, float* fx, float* fy, float* fz, float& rx, float& ry, float& rz)
+{
+    rx = ry = rz = 0.0f;
+    // Each phase runs over the S sinks before the next starts, so that the
+    // S independent chains interleave (the FP64 subtract, the conversion and
+    // the rsq each have latency the next phase would otherwise wait for).
+#pragma unroll 1
+    for (uint32_t u = 0; u < 64; ++u) {
+        const uint32_t k = qb + ((lane - u) & 63u);
+        const double2 a = txy[k];
+        const SrcZM b = tzm[k];
+        float dx[kSymS], dy[kSymS], dz[kSymS], r2[kSymS], ri3[kSymS];
+#pragma unroll
+        for (int s = 0; s < (int)kSymS; ++s) {
+            dx[s] = (float)(a.x - xi[s]);
+            dy[s] = (float)(a.y - yi[s]);
+            dz[s] = (float)(b.z - zi[s]);
+        }
+#pragma unroll
+        for (int s = 0; s < (int)kSymS; ++s) r2[s] = __builtin_fmaf(dz[s], dz[s], kEps2);
+#pragma unroll
+        for (int s = 0; s < (int)kSymS; ++s) r2[s] = __builtin_fmaf(dy[s], dy[s], r2[s]);
+#pragma unroll
+        for (int s = 0; s < (int)kSymS; ++s) r2[s] = __builtin_fmaf(dx[s], dx[s], r2[s]);
+#pragma unroll
+        for (int s = 0; s < (int)kSymS; ++s) ri3[s] = __builtin_amdgcn_rsqf(r2[s]);
+#pragma unroll
+        for (int s = 0; s < (int)kSymS; ++s) ri3[s] = (ri3[s] * ri3[s]) * ri3[s];
+#pragma unroll
+        for (int s = 0; s < (int)kSymS; ++s) {
+            const float scj = b.m * ri3[s];
+            fx[s] = __builtin_fmaf(scj, dx[s], fx[s]);
+            fy[s] = __builtin_fmaf(scj, dy[s], fy[s]);
+            fz[s] = __builtin_fmaf(scj, dz[s], fz[s]);
+        }
+        if constexpr (kReact) {
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+#pragma unroll
+            for (int s = 0; s < (int)kSymS; ++s) {
+                const float sci = mi[s] * ri3[s];
+                sx = __builtin_fmaf(sci, dx[s], sx);
+                sy = __builtin_fmaf(sci, dy[s], sy);
+                sz = __builtin_fmaf(sci, dz[s], sz);
+            }
+            rx = rot_sub(rx, sx);
+            ry = rot_sub(ry, sy);
+            rz = rot_sub(rz, sz);
+        }
+    }
+}
+
+// Dynamic LDS: the source tile (xy, zm: 8 KB; between tiles it also carries
+// one wave's sink sums, 6 KB, for the combine) and the reaction accumulators
+// (R x 6 KB).
+__host__ __device__ constexpr size_t sym_lds_bytes(uint32_t R)
+{
+    return kSymBlock * (sizeof(double2) + sizeof(SrcZM)) + (size_t)R * 3 * kSymBlock * sizeof(double);
+}
+static_assert(kSymS * 3 * 64 * sizeof(double) <= kSymBlock * (sizeof(double2) + sizeof(SrcZM)),
+              "the wave-combine buffer must fit in the tile");
+
+__global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict__ items, uint32_t n_items,
+                                                      const double* __restrict__ x, const double* __restrict__ y,
+                                                      const double* __restrict__ z, const float* __restrict__ m32,
+                                                      const DiskCtl* __restrict__ ctl, uint32_t need,
+                                                      double* __restrict__ part, uint32_t n_all)
+{
+    extern __shared__ __align__(16) unsigned char sym_lds[];
+    double2* txy = reinterpret_cast<double2*>(sym_lds);
+    SrcZM* tzm = reinterpret_cast<SrcZM*>(txy + kSymBlock);
+    double* red = reinterpret_cast<double*>(sym_lds);           // [S][3][64], aliases the tile
+    double* acc = reinterpret_cast<double*>(tzm + kSymBlock);   // [R][3][256]
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const WorkItem w = items[it];
+        if (!(ctl[w.disk].flags & need)) continue;
+        const bool diag = w.sink_lo == w.src_lo;
+        const uint32_t nI = (w.sink_hi - w.sink_lo + kSymBlock - 1) / kSymBlock;
+        const uint32_t nJ = (w.src_hi - w.src_lo + kSymBlock - 1) / kSymBlock;
+        for (uint32_t q = tid; q < nJ * 3 * kSymBlock; q += kBlock) acc[q] = 0.0;
+        for (uint32_t bi = 0; bi < nI; ++bi) {
+            const uint32_t i0 = w.sink_lo + bi * kSymBlock;
+            double xi[kSymS], yi[kSymS], zi[kSymS], ax[kSymS], ay[kSymS], az[kSymS];
+            float mi[kSymS];
+#pragma unroll
+            for (int s = 0; s < (int)kSymS; ++s) {
+                const uint32_t i = i0 + lane + 64u * s;
+                const bool valid = i < w.sink_hi;
+                xi[s] = valid ? x[i] : 0.0;
+                yi[s] = valid ? y[i] : 0.0;
+                zi[s] = valid ? z[i] : 0.0;
+                mi[s] = valid ? m32[i] : 0.0f;
+                ax[s] = ay[s] = az[s] = 0.0;
+            }
+            for (uint32_t bj = diag ? bi : 0; bj < nJ; ++bj) {
+                const uint32_t j0 = w.src_lo + bj * kSymBlock;
+                __syncthreads();
+                {
+                    const uint32_t j = j0 + tid;
+                    double2 a = make_double2(0.0, 0.0);
+                    SrcZM b{0.0, 0.0f, 0.0f};
+                    if (j < w.src_hi) {
+                        a = make_double2(x[j], y[j]);
+                        b.z = z[j];
+                        b.m = m32[j];
+                    }
+                    txy[tid] = a;
+                    tzm[tid] = b;
+                }
+                __syncthreads();
+                float fx[kSymS], fy[kSymS], fz[kSymS];
+#pragma unroll
+                for (int s = 0; s < (int)kSymS; ++s) fx[s] = fy[s] = fz[s] = 0.0f;
+                float rx, ry, rz;
+                const bool self = diag && bj == bi;
+                if (self)
+                    sym_sweep<false>(txy, tzm, wave * 64u, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz);
+                else
+                    sym_sweep<true>(txy, tzm, wave * 64u, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz);
+#pragma unroll
+                for (int s = 0; s < (int)kSymS; ++s) {
+                    ax[s] += (double)fx[s];
+                    ay[s] += (double)fy[s];
+                    az[s] += (double)fz[s];
+                }
+                if (!self) {
+                    const uint32_t k = wave * 64u + ((lane + 1u) & 63u);
+                    double* ab = acc + (size_t)bj * 3 * kSymBlock;
+                    ab[k] += (double)rx;
+                    ab[kSymBlock + k] += (double)ry;
+                    ab[2 * kSymBlock + k] += (double)rz;
+                }
+            }
+            // sinks of block I: ((w0 + w1) + w2) + w3 through the tile buffer
+            for (uint32_t src = 1; src < 4; ++src) {
+                __syncthreads();
+                if (wave == src) {
+#pragma unroll
+                    for (int s = 0; s < (int)kSymS; ++s) {
+                        double* r = red + (size_t)s * 3 * 64;
+                        r[lane] = ax[s];
+                        r[64 + lane] = ay[s];
+                        r[128 + lane] = az[s];
+                    }
+                }
+                __syncthreads();
+                if (wave == 0) {
+#pragma unroll
+                    for (int s = 0; s < (int)kSymS; ++s) {
+                        const double* r = red + (size_t)s * 3 * 64;
+                        ax[s] += r[lane];
+                        ay[s] += r[64 + lane];
+                        az[s] += r[128 + lane];
+                    }
+                }
+            }
+            if (wave == 0) {
+#pragma unroll
+                for (int s = 0; s < (int)kSymS; ++s) {
+                    const double sx = ax[s], sy = ay[s], sz = az[s];
+                    const uint32_t k = lane + 64u * s;
+                    if (diag) {
+                        // same slot as the reactions this block got from I' < I
+                        double* ab = acc + (size_t)bi * 3 * kSymBlock;
+                        ab[k] += sx;
+                        ab[kSymBlock + k] += sy;
+                        ab[2 * kSymBlock + k] += sz;
+                    } else if (i0 + k < w.sink_hi) {
+                        part[(size_t)(w.slot * 3 + 0) * n_all + i0 + k] = sx;
+                        part[(size_t)(w.slot * 3 + 1) * n_all + i0 + k] = sy;
+                        part[(size_t)(w.slot * 3 + 2) * n_all + i0 + k] = sz;
+                    }
+                }
+            }
+        }
+        // reactions (and, on the diagonal, sink sums) of chunk c -> slot a
+        __syncthreads();
+        for (uint32_t q = tid; q < nJ * kSymBlock; q += kBlock) {
+            const uint32_t j = w.src_lo + q;
+            if (j < w.src_hi) {
+                const double* ab = acc + (size_t)(q / kSymBlock) * 3 * kSymBlock + q % kSymBlock;
+                part[(size_t)(w.src_slot * 3 + 0) * n_all + j] = ab[0];
+                part[(size_t)(w.src_slot * 3 + 1) * n_all + j] = ab[kSymBlock];
+                part[(size_t)(w.src_slot * 3 + 2) * n_all + j] = ab[2 * kSymBlock];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_force_sym(hipStream_t st, uint32_t R, const WorkItem* items, uint32_t n_items, uint32_t grid,
+                            const double* x, const double* y, const double* z, const float* m32,
+                            const DiskCtl* ctl, uint32_t need, double* part, uint32_t n_all)
+{
+    if (n_items == 0) return hipSuccess;
+    if (R < 1 || R > kSymMaxR) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_force_sym, dim3(grid), dim3(kBlock), sym_lds_bytes(R), st, items, n_items, x, y, z, m32,
+                       ctl, need, part, n_all);
+    return hipGetLastError();
+}
+
+size_t force_sym_lds_bytes(uint32_t R) { return sym_lds_bytes(R); }
+
 hipError_t launch_force_exact(hipStream_t st, int lane, const WorkItem* items, uint32_t n_items, uint32_t grid,
                               const double* x, const double* y, const double* z, const double* m_ex,
                               const DiskCtl* ctl, uint32_t need, double* part, uint32_t n_all)

@@ -24,7 +24,8 @@ from redamd_types import (VAR4, PARAM, BODY_MD, INTEGRAL, EVENT, GasParams, Tuni
                           RD_RK4, RD_RKF78, RD_PREC_FP64, RD_PREC_MIXED, RD_DRAG_NSI, RD_DRAG_ALL_MASSIVE,
                           BODY_STAR, BODY_GIANT, BODY_ROCKY, BODY_PROTO, BODY_SUPERPL, BODY_PL, BODY_TP,
                           EVENT_NONE, EVENT_HIT_CENTRUM, EVENT_EJECTION, EVENT_COLLISION, K_GAUSS,
-                          RD_EVENT_COUNTER_TOTAL, RD_EVENT_COUNTER_LAST_CLEAR, RD_EVENT_COUNTER_LAST_STEP)
+                          RD_EVENT_COUNTER_TOTAL, RD_EVENT_COUNTER_LAST_CLEAR, RD_EVENT_COUNTER_LAST_STEP,
+                          RD_PAIRS_ONE_SIDED, RD_PAIRS_SYMMETRIC)
 
 
 class RedError(RuntimeError):
@@ -214,7 +215,7 @@ class Disk:
         check(lib.rd_disk_get_tuning(self.h, C.byref(t)))
         return t
 
-    def set_tuning(self, sinks_per_lane=None, chunk=None, blocks_per_cu=None, use_graph=None):
+    def set_tuning(self, sinks_per_lane=None, chunk=None, blocks_per_cu=None, use_graph=None, pair_mode=None):
         t = self.tuning()
         if sinks_per_lane is not None:
             t.sinks_per_lane = sinks_per_lane
@@ -224,6 +225,8 @@ class Disk:
             t.blocks_per_cu = blocks_per_cu
         if use_graph is not None:
             t.use_graph = use_graph
+        if pair_mode is not None:
+            t.pair_mode = pair_mode
         check(lib.rd_disk_set_tuning(self.h, C.byref(t)))
 
     def autotune(self):

@@ -34,6 +34,7 @@ RD_RK4, RD_RKF78 = 2, 4
 RD_EVENT_COUNTER_TOTAL, RD_EVENT_COUNTER_LAST_CLEAR, RD_EVENT_COUNTER_LAST_STEP = 0, 1, 2
 RD_PREC_FP64, RD_PREC_MIXED = 0, 1
 RD_DRAG_NSI, RD_DRAG_ALL_MASSIVE = 0, 1
+RD_PAIRS_ONE_SIDED, RD_PAIRS_SYMMETRIC = 0, 1
 BODY_STAR, BODY_GIANT, BODY_ROCKY, BODY_PROTO, BODY_SUPERPL, BODY_PL, BODY_TP = range(7)
 EVENT_NONE, EVENT_HIT_CENTRUM, EVENT_EJECTION, EVENT_COLLISION = range(4)
 
@@ -52,7 +53,7 @@ class GasParams(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("sinks_per_lane", C.c_uint32), ("chunk", C.c_uint32),
-                ("blocks_per_cu", C.c_uint32), ("use_graph", C.c_uint32)]
+                ("blocks_per_cu", C.c_uint32), ("use_graph", C.c_uint32), ("pair_mode", C.c_uint32)]
 
 
 class DiskSpec(C.Structure):

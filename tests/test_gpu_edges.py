@@ -109,3 +109,19 @@ def test_exact_kernels_agree_at_batch_scale():
         assert a[0] == s[0]
         assert bits_equal(a[1], s[1]) and bits_equal(a[2], s[2]), k
         assert bit.counters(k) == sit.counters()
+
+
+def test_integrator_destroyed_after_its_disk():
+    """A garbage collector may finalize a disk before the integrator that holds
+    it (objects in a reference cycle): destroying the integrator then must not
+    read the disk, and later calls must not see an error from it."""
+    r, v, p, b, nbt = R.make_disk(100, 3)
+    d = R.Disk(r, v, p, b, nbt)
+    it = R.Integrator(d, R.RD_RKF78, 1.0 * R.K_GAUSS, adaptive=True, tol=1e-10)
+    it.steps(2)
+    d.close()
+    it.close()
+    d2 = R.Disk(r, v, p, b, nbt)
+    d2.set_precision(R.RD_PREC_MIXED)          # rebuilds the lists and launches
+    a = d2.calc_dydx(1, 0, 0.0, r, v)
+    assert np.all(np.isfinite(a["x"]))

@@ -44,12 +44,13 @@ def test_force_fp64_bitexact(case):
     assert bits_equal(dv, v)
 
 
-@pytest.mark.parametrize("spl", [1, 2, 4])
-def test_force_mixed_vs_fp64(spl):
+@pytest.mark.parametrize("mode,spl", [(R.RD_PAIRS_ONE_SIDED, 1), (R.RD_PAIRS_ONE_SIDED, 2),
+                                      (R.RD_PAIRS_ONE_SIDED, 4), (R.RD_PAIRS_SYMMETRIC, 2)])
+def test_force_mixed_vs_fp64(mode, spl):
     r, v, p, b, nbt = R.make_disk(3000, 21)
     d, o = pair(r, v, p, b, nbt)
     d.set_precision(R.RD_PREC_MIXED)
-    d.set_tuning(sinks_per_lane=spl, chunk=512)
+    d.set_tuning(sinks_per_lane=spl, chunk=512, pair_mode=mode)
     dy = as3(d.calc_dydx(1, 0, 0.0, r, v))
     rc, ody = o.calc_dydx(1, 0, 0.0, r, v)
     ref = as3(ody)
