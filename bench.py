@@ -139,14 +139,34 @@ def profiled_traffic(kernel, workload_file, tuning):
             text = f.read()
     except OSError:
         return None
-    m = re.search(r"--tuning (\d+),(\d+),(\d+)", text)
+    m = re.search(r"--tuning (\d+),(\d+),(\d+)(?:,(\d+))?", text)
     t = re.search(r"Traffic per launch: ([0-9.e+]+) B", text)
     if not m or not t or kernel not in text:
         return None
-    if tuning is None or (int(m[1]), int(m[2]), int(m[3])) != (tuning.sinks_per_lane, tuning.chunk,
-                                                               tuning.blocks_per_cu):
+    shape = (int(m[1]), int(m[2]), int(m[3]), int(m[4]) if m[4] else R.RD_PAIRS_ONE_SIDED)
+    if tuning is None or shape != (tuning.sinks_per_lane, tuning.chunk, tuning.blocks_per_cu, tuning.pair_mode):
         return None
     return float(t[1])
+
+
+def pair_kernel(precision, tuning):
+    """Name of the pair kernel a force evaluation runs."""
+    if precision != "mixed":
+        return "k_force_exact"
+    return "k_force_sym" if tuning.pair_mode == R.RD_PAIRS_SYMMETRIC else "k_force_mixed"
+
+
+def tuning_json(t, **extra):
+    return {"sinks_per_lane": t.sinks_per_lane, "chunk": t.chunk, "blocks_per_cu": t.blocks_per_cu,
+            "pairs": "symmetric" if t.pair_mode == R.RD_PAIRS_SYMMETRIC else "one-sided", **extra}
+
+
+def set_tuning_arg(d, text):
+    """--tuning S,CHUNK,BLOCKS_PER_CU[,PAIR_MODE] (pair mode 1 = symmetric, 0 = one-sided)."""
+    f = [int(x) for x in text.split(",")]
+    d.set_tuning(sinks_per_lane=f[0], chunk=f[1], blocks_per_cu=f[2],
+                 pair_mode=f[3] if len(f) > 3 else R.RD_PAIRS_SYMMETRIC)
+    return d.tuning()
 
 
 def roofline(d, precision, traffic=None):
@@ -155,7 +175,7 @@ def roofline(d, precision, traffic=None):
     return {"bound": "valu-fp32" if precision == "mixed" else "valu-fp64", "achieved": achieved / 1e12,
             "peak": (PEAK_FP32 if precision == "mixed" else PEAK_FP64) / 1e12, "unit": "TFLOP/s",
             "frac": achieved / (PEAK_FP32 if precision == "mixed" else PEAK_FP64), "traffic": traffic,
-            "kernel": "k_force_mixed" if precision == "mixed" else "k_force_exact",
+            "kernel": pair_kernel(precision, d.tuning()),
             "ms_per_launch": ms, "pairs_per_launch": pairs_launch}
 
 
@@ -202,7 +222,7 @@ def bench_config3(args):
     d = R.Disk(r, v, p, b, nbt, threshold=(0.1, 0.0, 1.0), gas=g)
     d.set_drag_mode(R.RD_DRAG_ALL_MASSIVE)
     d.set_precision(R.RD_PREC_MIXED)
-    tuning = d.autotune() if not args.no_autotune else d.tuning()
+    tuning = set_tuning_arg(d, args.tuning) if args.tuning else (d.autotune() if not args.no_autotune else d.tuning())
     d.set_tuning(use_graph=GRAPH)
     a_min = np.min(np.hypot(r["x"][1:], r["y"][1:]))
     h0 = 2 * np.pi * np.sqrt(a_min ** 3) / 1000.0
@@ -226,8 +246,7 @@ def bench_config3(args):
            "config": {"workload": f"N={n} in a gas disk (Sigma=1700 r^-1.5, H/r=0.05 r^0.25, eta=0.002), drag on "
                                   "all massive bodies, R 1-100 km, merging f=1, hit-centrum 0.1 AU, mixed, RKF7(8) "
                                   f"tol {args.tol:g}",
-                      "tuning": {"sinks_per_lane": tuning.sinks_per_lane, "chunk": tuning.chunk,
-                                 "blocks_per_cu": tuning.blocks_per_cu}},
+                      "tuning": tuning_json(tuning)},
            "rkf78_steps_per_s": steps / el, "attempts": c1[0] - c0[0], "rejected": c1[2] - c0[2],
            "events": int(len(d.event_log())), "rel_energy_error": rel_e,
            "note": "energy is not conserved with gas drag; the error column is the drag dissipation",
@@ -248,7 +267,10 @@ def bench_config4(args):
     nbt = np.stack([q[4] for q in parts])
     d = R.Disk(*cat, nbt, batch=True)
     d.set_precision(R.RD_PREC_MIXED if args.precision == "mixed" else R.RD_PREC_FP64)
-    tuning = d.autotune() if (args.precision == "mixed" and not args.no_autotune) else d.tuning()
+    if args.tuning:
+        tuning = set_tuning_arg(d, args.tuning)
+    else:
+        tuning = d.autotune() if (args.precision == "mixed" and not args.no_autotune) else d.tuning()
     d.set_tuning(use_graph=GRAPH)
     steps = args.steps if args.steps != 1000 else 100
     it = R.Integrator(d, R.RD_RKF78, 0.1 * R.K_GAUSS, adaptive=True, tol=args.tol)
@@ -268,8 +290,7 @@ def bench_config4(args):
            "data": "synthetic seeded disks",
            "config": {"workload": f"512 disks, N_d ~ U[64, 4096] (total {sum(sizes) + len(sizes)} bodies), "
                                   f"RKF7(8) tol {args.tol:g}, {args.precision}",
-                      "tuning": {"sinks_per_lane": tuning.sinks_per_lane, "chunk": tuning.chunk,
-                                 "blocks_per_cu": tuning.blocks_per_cu}},
+                      "tuning": tuning_json(tuning)},
            "disk_steps_per_s": steps * len(sizes) / el, "attempts_all_disks": tried,
            "roofline": roofline(d, args.precision)}
     if not args.no_cpu:
@@ -360,7 +381,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch the attempt's kernels directly (no hipGraph)")
-    ap.add_argument("--tuning", default="", help="S,CHUNK,BLOCKS_PER_CU for the mixed kernel (skips autotune)")
+    ap.add_argument("--tuning", default="", help="S,CHUNK,BLOCKS_PER_CU[,PAIR_MODE] for the mixed kernel "
+                                                 "(skips autotune; pair mode 1 symmetric, 0 one-sided)")
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5],
                     help="1 RK4 N=256, 2 RKF78 N=16384 (default), 3 gas+collisions N=131072, "
                          "4 batch of 512 disks, 5 learned step proposer")
@@ -400,9 +422,7 @@ def main():
     d.set_precision(R.RD_PREC_MIXED if args.precision == "mixed" else R.RD_PREC_FP64)
     d.set_tuning(use_graph=GRAPH)
     if args.tuning:
-        S, Cs, B = (int(x) for x in args.tuning.split(","))
-        d.set_tuning(sinks_per_lane=S, chunk=Cs, blocks_per_cu=B)
-        tuning = d.tuning()
+        tuning = set_tuning_arg(d, args.tuning)
     else:
         tuning = d.autotune() if (args.precision == "mixed" and not args.no_autotune) else d.tuning()
     d.set_tuning(use_graph=GRAPH)
@@ -465,8 +485,7 @@ def main():
         "config": {"workload": f"N={args.n} planetesimals + star, RKF7(8) tol {args.tol:g}, "
                                f"{args.precision} pair kernel, seed {args.seed}",
                    "n_bodies": args.n + 1, "tolerance": args.tol, "precision": args.precision,
-                   "tuning": {"sinks_per_lane": tuning.sinks_per_lane, "chunk": tuning.chunk,
-                              "blocks_per_cu": tuning.blocks_per_cu, "use_graph": 1}},
+                   "tuning": tuning_json(tuning, use_graph=GRAPH)},
         "rkf78_steps_per_s": steps_per_s,
         "attempts": tried1 - tried0,
         "rejected": failed1 - failed0,
@@ -477,9 +496,10 @@ def main():
         "max_rel_force_err_vs_fp64": f_err,
         "roofline": {"bound": "valu-fp32", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
-                     "traffic": profiled_traffic("k_force_mixed", "k_force_mixed_disk16384_rkf78.txt", tuning)
+                     "traffic": profiled_traffic(pair_kernel(args.precision, tuning),
+                                                 f"{pair_kernel(args.precision, tuning)}_disk16384_rkf78.txt", tuning)
                      if (args.precision == "mixed" and args.n == 16384 and args.seed == 2) else None,
-                     "kernel": "k_force_mixed" if args.precision == "mixed" else "k_force_exact",
+                     "kernel": pair_kernel(args.precision, tuning),
                      "ms_per_launch": ms, "pairs_per_launch": pairs_launch},
     }
     if not args.no_cpu:

@@ -341,8 +341,8 @@ __global__ __launch_bounds__(kBlock) void k_force_mixed(const WorkItem* __restri
 // in the diagonal item both land in slot k.  Every slot is written by exactly
 // one item in a fixed order, so results do not depend on the schedule.
 // Sink sums: FP32 per 64-source quarter, folded into FP64, waves combined in
-// fixed order.  Reactions: FP32 over the tile's 256 sinks, accumulated over
-// the chunk's I blocks in FP64 (LDS).
+// fixed order.  Reactions: FP32 over 16 sweep steps (64 terms), folded into
+// FP64, accumulated over the chunk's I blocks in FP64 (LDS).
 constexpr uint32_t kSymBlock = kSymBlockBodies;
 constexpr uint32_t kSymS = kSymBlock / 64;
 
@@ -358,57 +358,72 @@ __device__ inline float rot_sub(float acc, float v)
 }
 
 // One wave's sweep of its 64-source quarter: sink sums into f*, reactions
-// (when kReact) into r* (lane l: source (l + 1) mod 64 of the quarter).
+// (when kReact) into r* (lane l: source (l + 1) mod 64 of the quarter).  The
+// FP32 reaction sums are folded into FP64 every 16 steps (64 terms, as on the
+// sink side): the FP64 running sum is rotated 16 lanes at each fold so that
+// it stays with the sources the FP32 sum in the same lane belongs to.
 template <bool kReact>
 __device__ inline void sym_sweep(const double2* __restrict__ txy, const SrcZM* __restrict__ tzm, uint32_t qb,
                                  uint32_t lane, const double* xi, const double* yi, const double* zi,
-                                 const float* mi, float* fx, float* fy, float* fz, float& rx, float& ry, float& rz)
+                                 const float* mi, float* fx, float* fy, float* fz, double& rx, double& ry,
+                                 double& rz)
 {
-    rx = ry = rz = 0.0f;
-    // Each phase runs over the S sinks before the next starts, so that the
-    // S independent chains interleave (the FP64 subtract, the conversion and
-    // the rsq each have latency the next phase would otherwise wait for).
+    rx = ry = rz = 0.0;
+    const int from16 = (int)((lane - 16u) & 63u);
+    for (uint32_t u0 = 0; u0 < 64; u0 += 16) {
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        // Each phase runs over the S sinks before the next starts, so that the
+        // S independent chains interleave (the FP64 subtract, the conversion
+        // and the rsq each have latency the next phase would otherwise wait for).
 #pragma unroll 1
-    for (uint32_t u = 0; u < 64; ++u) {
-        const uint32_t k = qb + ((lane - u) & 63u);
-        const double2 a = txy[k];
-        const SrcZM b = tzm[k];
-        float dx[kSymS], dy[kSymS], dz[kSymS], r2[kSymS], ri3[kSymS];
-#pragma unroll
-        for (int s = 0; s < (int)kSymS; ++s) {
-            dx[s] = (float)(a.x - xi[s]);
-            dy[s] = (float)(a.y - yi[s]);
-            dz[s] = (float)(b.z - zi[s]);
-        }
-#pragma unroll
-        for (int s = 0; s < (int)kSymS; ++s) r2[s] = __builtin_fmaf(dz[s], dz[s], kEps2);
-#pragma unroll
-        for (int s = 0; s < (int)kSymS; ++s) r2[s] = __builtin_fmaf(dy[s], dy[s], r2[s]);
-#pragma unroll
-        for (int s = 0; s < (int)kSymS; ++s) r2[s] = __builtin_fmaf(dx[s], dx[s], r2[s]);
-#pragma unroll
-        for (int s = 0; s < (int)kSymS; ++s) ri3[s] = __builtin_amdgcn_rsqf(r2[s]);
-#pragma unroll
-        for (int s = 0; s < (int)kSymS; ++s) ri3[s] = (ri3[s] * ri3[s]) * ri3[s];
-#pragma unroll
-        for (int s = 0; s < (int)kSymS; ++s) {
-            const float scj = b.m * ri3[s];
-            fx[s] = __builtin_fmaf(scj, dx[s], fx[s]);
-            fy[s] = __builtin_fmaf(scj, dy[s], fy[s]);
-            fz[s] = __builtin_fmaf(scj, dz[s], fz[s]);
-        }
-        if constexpr (kReact) {
-            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        for (uint32_t u = u0; u < u0 + 16; ++u) {
+            const uint32_t k = qb + ((lane - u) & 63u);
+            const double2 a = txy[k];
+            const SrcZM b = tzm[k];
+            float dx[kSymS], dy[kSymS], dz[kSymS], r2[kSymS], ri3[kSymS];
 #pragma unroll
             for (int s = 0; s < (int)kSymS; ++s) {
-                const float sci = mi[s] * ri3[s];
-                sx = __builtin_fmaf(sci, dx[s], sx);
-                sy = __builtin_fmaf(sci, dy[s], sy);
-                sz = __builtin_fmaf(sci, dz[s], sz);
+                dx[s] = (float)(a.x - xi[s]);
+                dy[s] = (float)(a.y - yi[s]);
+                dz[s] = (float)(b.z - zi[s]);
             }
-            rx = rot_sub(rx, sx);
-            ry = rot_sub(ry, sy);
-            rz = rot_sub(rz, sz);
+#pragma unroll
+            for (int s = 0; s < (int)kSymS; ++s) r2[s] = __builtin_fmaf(dz[s], dz[s], kEps2);
+#pragma unroll
+            for (int s = 0; s < (int)kSymS; ++s) r2[s] = __builtin_fmaf(dy[s], dy[s], r2[s]);
+#pragma unroll
+            for (int s = 0; s < (int)kSymS; ++s) r2[s] = __builtin_fmaf(dx[s], dx[s], r2[s]);
+#pragma unroll
+            for (int s = 0; s < (int)kSymS; ++s) ri3[s] = __builtin_amdgcn_rsqf(r2[s]);
+#pragma unroll
+            for (int s = 0; s < (int)kSymS; ++s) ri3[s] = (ri3[s] * ri3[s]) * ri3[s];
+#pragma unroll
+            for (int s = 0; s < (int)kSymS; ++s) {
+                const float scj = b.m * ri3[s];
+                fx[s] = __builtin_fmaf(scj, dx[s], fx[s]);
+                fy[s] = __builtin_fmaf(scj, dy[s], fy[s]);
+                fz[s] = __builtin_fmaf(scj, dz[s], fz[s]);
+            }
+            if constexpr (kReact) {
+                float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+#pragma unroll
+                for (int s = 0; s < (int)kSymS; ++s) {
+                    const float sci = mi[s] * ri3[s];
+                    sx = __builtin_fmaf(sci, dx[s], sx);
+                    sy = __builtin_fmaf(sci, dy[s], sy);
+                    sz = __builtin_fmaf(sci, dz[s], sz);
+                }
+                ax = rot_sub(ax, sx);
+                ay = rot_sub(ay, sy);
+                az = rot_sub(az, sz);
+            }
+        }
+        if constexpr (kReact) {
+            // lane l's FP32 sum belongs to source (l - u0 - 15) mod 64, the
+            // running sum of lane l - 16 to the same source
+            rx = __shfl(rx, from16) + (double)ax;
+            ry = __shfl(ry, from16) + (double)ay;
+            rz = __shfl(rz, from16) + (double)az;
         }
     }
 }
@@ -476,7 +491,7 @@ __global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict
                 float fx[kSymS], fy[kSymS], fz[kSymS];
 #pragma unroll
                 for (int s = 0; s < (int)kSymS; ++s) fx[s] = fy[s] = fz[s] = 0.0f;
-                float rx, ry, rz;
+                double rx, ry, rz;
                 const bool self = diag && bj == bi;
                 if (self)
                     sym_sweep<false>(txy, tzm, wave * 64u, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz);
@@ -491,9 +506,9 @@ __global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict
                 if (!self) {
                     const uint32_t k = wave * 64u + ((lane + 1u) & 63u);
                     double* ab = acc + (size_t)bj * 3 * kSymBlock;
-                    ab[k] += (double)rx;
-                    ab[kSymBlock + k] += (double)ry;
-                    ab[2 * kSymBlock + k] += (double)rz;
+                    ab[k] += rx;
+                    ab[kSymBlock + k] += ry;
+                    ab[2 * kSymBlock + k] += rz;
                 }
             }
             // sinks of block I: ((w0 + w1) + w2) + w3 through the tile buffer
