@@ -144,7 +144,14 @@ def profiled_traffic(kernel, workload_file, tuning):
     if not m or not t or kernel not in text:
         return None
     shape = (int(m[1]), int(m[2]), int(m[3]), int(m[4]) if m[4] else R.RD_PAIRS_ONE_SIDED)
-    if tuning is None or shape != (tuning.sinks_per_lane, tuning.chunk, tuning.blocks_per_cu, tuning.pair_mode):
+    if tuning is None:
+        return None
+    if kernel == "k_force_sym":
+        # symmetric items and slots depend on the disk alone: S, chunk and the
+        # grid size change neither (the workload has no one-sided pairs)
+        if shape[3] != tuning.pair_mode:
+            return None
+    elif shape != (tuning.sinks_per_lane, tuning.chunk, tuning.blocks_per_cu, tuning.pair_mode):
         return None
     return float(t[1])
 
