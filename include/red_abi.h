@@ -48,8 +48,11 @@ enum { RD_THRESHOLD_HIT_CENTRUM_DISTANCE = 0, RD_THRESHOLD_EJECTION_DISTANCE = 1
 enum { RD_EVENT_NONE = 0, RD_EVENT_HIT_CENTRUM = 1, RD_EVENT_EJECTION = 2, RD_EVENT_COLLISION = 3 };
 /* gas_decrease_t, red.type/red_type.h:105-111 */
 enum { RD_GAS_DENSITY_CONSTANT = 0, RD_GAS_DENSITY_DECREASE_LINEAR = 1, RD_GAS_DENSITY_DECREASE_EXPONENTIAL = 2 };
-/* integrator_type_t, red.type/red_type.h:136-143 (RK4 and RKF7(8) are built) */
-enum { RD_RK4 = 2, RD_RKF78 = 4 };
+/* integrator_type_t, red.type/red_type.h:136-143 (all but RKF5(6) are built).
+ * Euler and RK2 are fixed-step: their step is always the dt they were created
+ * with, and rd_set_dt_next does not change it (int_euler.cu:78-105,
+ * int_rungekutta2.cu:99-131). */
+enum { RD_EULER = 0, RD_RK2 = 1, RD_RK4 = 2, RD_RKF78 = 4 };
 
 /* Force precision of the pair sum. */
 enum {

@@ -828,7 +828,11 @@ or_integ* or_integ_create(or_disk* d, int method, double dt, int adaptive, doubl
     it->d = d; it->method = method; it->adaptive = adaptive; it->tol = tol;
     it->dt_try = dt; it->dt_next = dt;
     if (method == 4) { it->n_stage = adaptive ? 13 : 11; it->order = 7; }
-    else { it->n_stage = adaptive ? 5 : 4; it->order = 4; }
+    else if (method == 2) { it->n_stage = adaptive ? 5 : 4; it->order = 4; }
+    /* euler / rungekutta2 construct the base with adaptive = false
+       (int_euler.cu:48-54, int_rungekutta2.cu:49-55) */
+    else if (method == 1) { it->n_stage = 2; it->order = 2; it->adaptive = 0; }
+    else { it->n_stage = 1; it->order = 1; it->adaptive = 0; }
     integ_alloc(it, d->n);
     return it;
 }
@@ -998,9 +1002,67 @@ static int rk4_step(or_integ* it, double* dt_did)
     *dt_did = it->dt_did;
     return OR_OK;
 }
+/* y_out = y + fac * f over both halves (the sum_vector kernels of
+   int_euler.cu:26-36 and int_rungekutta2.cu:30-40) */
+static void sum_halves(or_integ* it, const or_var4* const y[2], or_var4* const out[2], int r, double fac)
+{
+    const uint32_t nvt = 4 * it->d->n;
+    for (int i = 0; i < 2; i++) {
+        const double* a = ccomp(y[i]);
+        const double* f = comp(it->k[i][r]);
+        double* o = comp(out[i]);
+        for (uint32_t t = 0; t < nvt; t++) o[t] = a[t] + fac * f[t];
+    }
+}
+/* int_euler.cu:78-105.  The step is always dt_try, the constructor's dt:
+   euler::step never takes dt_next, which it sets to dt_try. */
+static int euler_step(or_integ* it, double* dt_did)
+{
+    or_disk* d = it->d;
+    if (it->cap < d->n) integ_alloc(it, d->n);
+    int rc = dydx_pair(it, 0, d->t, d->r, d->v);
+    if (rc) return rc;
+    const or_var4* y[2] = {d->r, d->v};
+    or_var4* out[2] = {d->rout, d->vout};
+    sum_halves(it, y, out, 0, it->dt_try);
+    it->dt_did = it->dt_try;
+    it->dt_next = it->dt_try;
+    it->tried++; it->passed++;
+    d->t += it->dt_did;
+    swap_state(d);
+    *dt_did = it->dt_did;
+    return OR_OK;
+}
+/* int_rungekutta2.cu:99-131: the midpoint rule, a = {0, 1/2}, b = {0, 1},
+   c = {0, 1/2} (int_rungekutta2.cu:13-15); dt_try fixed as for Euler. */
+static int rk2_step(or_integ* it, double* dt_did)
+{
+    or_disk* d = it->d;
+    if (it->cap < d->n) integ_alloc(it, d->n);
+    int rc = dydx_pair(it, 0, d->t + 0.0 * it->dt_try, d->r, d->v);
+    if (rc) return rc;
+    const or_var4* y[2] = {d->r, d->v};
+    or_var4* tmp[2] = {it->ytemp[0], it->ytemp[1]};
+    sum_halves(it, y, tmp, 0, (1.0 / 2.0) * it->dt_try);
+    if ((rc = dydx_pair(it, 1, d->t + (1.0 / 2.0) * it->dt_try, it->ytemp[0], it->ytemp[1]))) return rc;
+    or_var4* out[2] = {d->rout, d->vout};
+    sum_halves(it, y, out, 1, 1.0 * it->dt_try);
+    it->dt_did = it->dt_try;
+    it->dt_next = it->dt_try;
+    it->tried++; it->passed++;
+    d->t += it->dt_did;
+    swap_state(d);
+    *dt_did = it->dt_did;
+    return OR_OK;
+}
 int or_step(or_integ* it, double* dt_did)
 {
-    return it->method == 4 ? rk8_step(it, dt_did) : rk4_step(it, dt_did);
+    switch (it->method) {
+    case 0: return euler_step(it, dt_did);
+    case 1: return rk2_step(it, dt_did);
+    case 4: return rk8_step(it, dt_did);
+    default: return rk4_step(it, dt_did);
+    }
 }
 
 /* src/red.cuda/main.cu:470-625 without output, dumps or device switching */

@@ -138,7 +138,10 @@ void     or_clear_event_counter(or_disk* d);
 int      or_rebuild_vectors(or_disk* d);
 uint32_t or_event_log(const or_disk* d, or_event* out, uint32_t cap);
 
-/* integrators (int_rungekutta4.cu, int_rungekutta8.cu, integrator.cu) */
+/* integrators (int_euler.cu, int_rungekutta2.cu, int_rungekutta4.cu,
+   int_rungekutta8.cu, integrator.cu); method as integrator_type_t
+   (red_type.h:136-143): 0 Euler, 1 RK2, 2 RK4, 4 RKF7(8).  Euler and RK2 are
+   fixed-step whatever `adaptive` says, as their constructors are. */
 or_integ* or_integ_create(or_disk* d, int method, double dt, int adaptive, double tol);
 void      or_integ_destroy(or_integ* it);
 int       or_step(or_integ* it, double* dt_did);

@@ -187,6 +187,7 @@ struct rd_disk {
 struct rd_integrator {
     rd_disk* d = nullptr;
     int device = -1;   // the disk's device, kept so that destroy never reads the disk
+    bool fixed_dt = false;   // Euler, RK2: every step is dt_try, dt_next follows it
     int method = RD_RKF78;
     int adaptive = 1;
     double tol = 0.0;
@@ -633,8 +634,10 @@ hipError_t enqueue_attempt(rd_integrator* it)
         StageArgs a = make_args(d, 0, it->adaptive, it->err_all);
         if ((e = enqueue_force(d, d->x, d->y, d->z, F_NEED_K0)) != hipSuccess) return e;
         if ((e = launch_stage(d->stream, it->method, 0, a)) != hipSuccess) return e;
-        StageArgs a1 = make_args(d, 0, it->adaptive, it->err_all);   // comb1 writes P[1]
-        if ((e = launch_comb1(d->stream, it->method, a1)) != hipSuccess) return e;
+        if (it->n_stage > 1) {
+            StageArgs a1 = make_args(d, 0, it->adaptive, it->err_all);   // comb1 writes P[1]
+            if ((e = launch_comb1(d->stream, it->method, a1)) != hipSuccess) return e;
+        }
     }
     for (int r = 1; r <= last_stage(it); ++r) {
         StageArgs a = make_args(d, r, it->adaptive, it->err_all);
@@ -1066,7 +1069,8 @@ int rd_integrator_create(rd_disk* d, int method, double dt, int adaptive, double
 {
     DeviceScope dev_scope(d ? d->device : -1);
     if (!d || !out) return fail(RD_EINVAL, "NULL argument");
-    if (method != RD_RK4 && method != RD_RKF78) return fail(RD_ENOTIMPL, "only RK4 and RKF78 are built");
+    if (method != RD_EULER && method != RD_RK2 && method != RD_RK4 && method != RD_RKF78)
+        return fail(RD_ENOTIMPL, "Euler, RK2, RK4 and RKF78 are built (RKF5(6) is not)");
     rd_integrator* it = new rd_integrator();
     it->d = d;
     it->device = d->device;
@@ -1074,7 +1078,15 @@ int rd_integrator_create(rd_disk* d, int method, double dt, int adaptive, double
     it->adaptive = adaptive ? 1 : 0;
     it->tol = tol;
     if (method == RD_RKF78) { it->n_stage = adaptive ? 13 : 11; it->order = 7; }
-    else { it->n_stage = adaptive ? 5 : 4; it->order = 4; }
+    else if (method == RD_RK4) { it->n_stage = adaptive ? 5 : 4; it->order = 4; }
+    else {
+        // euler / rungekutta2 construct the base with adaptive = false and step
+        // with the constructor's dt (int_euler.cu:48-54, int_rungekutta2.cu:49-55)
+        it->adaptive = 0;
+        it->fixed_dt = true;
+        it->n_stage = method == RD_RK2 ? 2 : 1;
+        it->order = method == RD_RK2 ? 2 : 1;
+    }
     it->dt_try.assign(d->n_disk, dt);
     it->dt_did.assign(d->n_disk, 0.0);
     it->dt_next.assign(d->n_disk, dt);
@@ -1206,7 +1218,7 @@ static int step_disks(rd_integrator* it, const std::vector<uint32_t>& want)
     std::vector<int> iter(nd, 0);
     std::vector<char> need_k0(nd, 1);
     for (uint32_t k = 0; k < nd; ++k)
-        if (pending[k]) it->dt_try[k] = it->dt_next[k];
+        if (pending[k] && !it->fixed_dt) it->dt_try[k] = it->dt_next[k];
     const double lambda = it->method == RD_RKF78 ? 41.0 / 840.0 : 1.0 / 10.0;
     uint32_t commit_flags_pending = 0;
     std::vector<uint32_t> commit(nd, 0);
@@ -1271,6 +1283,7 @@ static int step_disks(rd_integrator* it, const std::vector<uint32_t>& want)
                 it->tried[k] += iter[k];
                 it->failed[k] += iter[k] - 1;
                 it->passed[k]++;
+                if (it->fixed_dt) it->dt_next[k] = it->dt_try[k];
                 d->hd[k].t += it->dt_did[k];
                 pending[k] = 0;
                 remaining[k]--;
@@ -1318,7 +1331,8 @@ int rd_steps(rd_integrator* it, uint32_t n_steps)
         if (int rc = ensure_lists(d)) return rc;
         const uint32_t nd = d->n_disk;
         for (uint32_t k = 0; k < nd; ++k) {
-            it->dt_try[k] = it->dt_next[k];
+            if (!it->fixed_dt) it->dt_try[k] = it->dt_next[k];
+            it->dt_next[k] = it->dt_try[k];
             it->dt_did[k] = it->dt_try[k];
             d->h_ctl[k] = DiskCtl{d->hd[k].t, it->dt_try[k], F_ACTIVE | F_NEED_K0 | F_COMMIT, 0};
         }

@@ -961,6 +961,52 @@ __global__ __launch_bounds__(kBodyBlock) void k_stage_rk4(StageArgs a)
     if constexpr (R == 4) error_reduce(a, valid, d, e);
 }
 
+// Euler (int_euler.cu:56-105): y_{n+1} = y_n + h f(t, y_n), the position's
+// derivative being the velocity.
+template <int R>
+__global__ __launch_bounds__(kBodyBlock) void k_stage_euler(StageArgs a)
+{
+    static_assert(R == 0, "Euler has one stage");
+    const uint32_t i = blockIdx.x * kBodyBlock + threadIdx.x;
+    const int q = blockIdx.y;
+    if (i >= a.n_all) return;
+    const uint32_t d = a.disk_of[i];
+    const DiskCtl c = a.ctl[d];
+    if (!(c.flags & F_NEED_K0)) return;
+    const DiskInfo di = a.info[d];
+    const double y = a.pos[q][i], v = a.vel[q][i];
+    const double kRa = stage_accel(a, i, q, di, c.t, partial_sum(a, i, q, nchunk_of(di, i)), a.fpos, a.vel);
+    ka_w(a, 0, q)[i] = kRa;
+    a.out[q][i] = y + c.h * v;
+    a.out[3 + q][i] = v + c.h * kRa;
+}
+
+// Midpoint RK2 (int_rungekutta2.cu:13-15, 58-131): stage 0 is k0 (k_comb1<2>
+// then forms y + (h/2) k0, RK4's first combination); stage 1 evaluates there,
+// at t + h/2, and y_{n+1} = y_n + (1 h) k1.
+template <int R>
+__global__ __launch_bounds__(kBodyBlock) void k_stage_rk2(StageArgs a)
+{
+    const uint32_t i = blockIdx.x * kBodyBlock + threadIdx.x;
+    const int q = blockIdx.y;
+    if (i >= a.n_all) return;
+    const uint32_t d = a.disk_of[i];
+    const DiskCtl c = a.ctl[d];
+    if (!(c.flags & (R == 0 ? F_NEED_K0 : F_ACTIVE))) return;
+    const DiskInfo di = a.info[d];
+    const double h = c.h;
+    const double t = c.t + (R == 0 ? 0.0 : 1.0 / 2.0) * h;
+    const double* vR[3] = {kv_of(a, R, 0), kv_of(a, R, 1), kv_of(a, R, 2)};
+    const double kRa = stage_accel(a, i, q, di, t, partial_sum(a, i, q, nchunk_of(di, i)), a.fpos,
+                                   R == 0 ? a.vel : vR);
+    ka_w(a, R, q)[i] = kRa;
+    if constexpr (R == 1) {
+        const double fac = 1.0 * h;
+        a.out[q][i] = a.pos[q][i] + fac * kv_of(a, 1, q)[i];
+        a.out[3 + q][i] = a.vel[q][i] + fac * kRa;
+    }
+}
+
 // First combination of an attempt (stage 1 from k0), and clears the error
 // slot of each active disk.
 template <int M>
@@ -1033,6 +1079,16 @@ hipError_t launch_stage(hipStream_t st, int method, int r, const StageArgs& a)
             STAGE_CASE(rk8, 0) STAGE_CASE(rk8, 1) STAGE_CASE(rk8, 2) STAGE_CASE(rk8, 3) STAGE_CASE(rk8, 4)
             STAGE_CASE(rk8, 5) STAGE_CASE(rk8, 6) STAGE_CASE(rk8, 7) STAGE_CASE(rk8, 8) STAGE_CASE(rk8, 9)
             STAGE_CASE(rk8, 10) STAGE_CASE(rk8, 11) STAGE_CASE(rk8, 12)
+        default: return hipErrorInvalidValue;
+        }
+    } else if (method == 0) {
+        switch (r) {
+            STAGE_CASE(euler, 0)
+        default: return hipErrorInvalidValue;
+        }
+    } else if (method == 1) {
+        switch (r) {
+            STAGE_CASE(rk2, 0) STAGE_CASE(rk2, 1)
         default: return hipErrorInvalidValue;
         }
     } else {

@@ -161,10 +161,10 @@ Parameters load_parameters(const std::string& path)
             else throw std::string("Invalid data type: " + value);
         } else if (key == "integrator") {
             value = lower(value);
-            if (value == "runge-kutta4" || value == "rk4") P.method = RD_RK4;
+            if (value == "euler" || value == "e") P.method = RD_EULER;
+            else if (value == "runge-kutta2" || value == "rk2") P.method = RD_RK2;
+            else if (value == "runge-kutta4" || value == "rk4") P.method = RD_RK4;
             else if (value == "runge-kutta-fehlberg8" || value == "rkf8") P.method = RD_RKF78;
-            else if (value == "euler" || value == "e" || value == "runge-kutta2" || value == "rk2")
-                throw std::string("Integrator '" + value + "' is not part of this build (rk4 and rkf8 are).");
             else throw std::string("Invalid integrator type: " + value);
         } else if (key == "tolerance") {
             P.adaptive = true;
@@ -614,8 +614,9 @@ struct Run {
 std::string create_prefix(const Options& o, const Parameters& p)
 {
     if (!o.ef) return "";
-    // main.cu create_prefix: configuration, device, adaptive/fixed, integrator
-    return std::string("R_gpu_") + (p.adaptive ? "as" : "fs") + "_" + (p.method == RD_RK4 ? "RK4" : "RKF7") + "_";
+    // main.cu:100-131 create_prefix: configuration, device, adaptive/fixed, integrator
+    static const char* short_name[] = {"E", "RK2", "RK4", "RK5", "RKF7"};
+    return std::string("R_gpu_") + (p.adaptive ? "as" : "fs") + "_" + short_name[p.method] + "_";
 }
 
 int self_test(int id_dev)
@@ -714,9 +715,14 @@ int main(int argc, char** argv)
         double dt = info.dt;
         if (run.par.length < 0.0 && dt > 0.0) dt = -dt;   // backward integration
         run.dt_file = dt;
-        integrator* intgr = run.par.method == RD_RK4
-                                ? (integrator*)new rungekutta4(&ppd, dt, run.par.adaptive, run.par.tolerance)
-                                : (integrator*)new rungekutta8(&ppd, dt, run.par.adaptive, run.par.tolerance);
+        // options::create_integrator (options.cpp:253-282)
+        integrator* intgr = nullptr;
+        switch (run.par.method) {
+        case RD_EULER: intgr = new euler(&ppd, dt); break;
+        case RD_RK2: intgr = new rungekutta2(&ppd, dt); break;
+        case RD_RK4: intgr = new rungekutta4(&ppd, dt, run.par.adaptive, run.par.tolerance); break;
+        default: intgr = new rungekutta8(&ppd, dt, run.par.adaptive, run.par.tolerance); break;
+        }
         if (run.par.error_check_for_tp) intgr->set_error_check_for_tp(true);
 
         // run_simulation (main.cu:403-640)

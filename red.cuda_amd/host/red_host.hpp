@@ -234,6 +234,18 @@ class integrator {
     rd_integrator* h_ = nullptr;
 };
 
+// euler (src/red.cuda/int_euler.h): fixed step, first order
+class euler : public integrator {
+  public:
+    euler(pp_disk* ppd, double dt) : integrator(ppd, RD_EULER, dt, false, 0.0) {}
+};
+
+// rungekutta2, the midpoint rule (src/red.cuda/int_rungekutta2.h): fixed step
+class rungekutta2 : public integrator {
+  public:
+    rungekutta2(pp_disk* ppd, double dt) : integrator(ppd, RD_RK2, dt, false, 0.0) {}
+};
+
 // rungekutta4 (src/red.cuda/int_rungekutta4.h:7-28)
 class rungekutta4 : public integrator {
   public:

@@ -21,7 +21,7 @@ if HERE not in sys.path:
 
 from redamd_types import (VAR4, PARAM, BODY_MD, INTEGRAL, EVENT, GasParams, Tuning, DiskSpec,  # noqa: F401
                           RD_OK, RD_EINVAL, RD_EDEVICE, RD_ENOTIMPL, RD_EOVERFLOW, RD_ESTATE, RD_ENOMEM,
-                          RD_RK4, RD_RKF78, RD_PREC_FP64, RD_PREC_MIXED, RD_DRAG_NSI, RD_DRAG_ALL_MASSIVE,
+                          RD_EULER, RD_RK2, RD_RK4, RD_RKF78, RD_PREC_FP64, RD_PREC_MIXED, RD_DRAG_NSI, RD_DRAG_ALL_MASSIVE,
                           BODY_STAR, BODY_GIANT, BODY_ROCKY, BODY_PROTO, BODY_SUPERPL, BODY_PL, BODY_TP,
                           EVENT_NONE, EVENT_HIT_CENTRUM, EVENT_EJECTION, EVENT_COLLISION, K_GAUSS,
                           RD_EVENT_COUNTER_TOTAL, RD_EVENT_COUNTER_LAST_CLEAR, RD_EVENT_COUNTER_LAST_STEP,

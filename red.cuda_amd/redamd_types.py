@@ -30,7 +30,7 @@ EVENT = np.dtype({"names": ["event_name", "t", "d",
                   "itemsize": 432})
 
 RD_OK, RD_EINVAL, RD_EDEVICE, RD_ENOTIMPL, RD_EOVERFLOW, RD_ESTATE, RD_ENOMEM = range(7)
-RD_RK4, RD_RKF78 = 2, 4
+RD_EULER, RD_RK2, RD_RK4, RD_RKF78 = 0, 1, 2, 4
 RD_EVENT_COUNTER_TOTAL, RD_EVENT_COUNTER_LAST_CLEAR, RD_EVENT_COUNTER_LAST_STEP = 0, 1, 2
 RD_PREC_FP64, RD_PREC_MIXED = 0, 1
 RD_DRAG_NSI, RD_DRAG_ALL_MASSIVE = 0, 1

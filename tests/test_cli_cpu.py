@@ -54,7 +54,7 @@ def _dir(tmp_path, params, start=None):
 @pytest.mark.parametrize("params,msg", [
     ({"lenght": "1.0"}, "Invalid parameter: lenght"),
     ({"length": "one"}, "Invalid number at: length"),
-    ({"integrator": "euler"}, "not part of this build"),
+    ({"integrator": "rkf5"}, "Invalid integrator type"),
     ({"integrator": "leapfrog"}, "Invalid integrator type"),
     ({"odr": "hdf5"}, "Invalid data type"),
     ({"error check for tp": "yes"}, "Invalid value at: error check for tp"),

@@ -89,6 +89,32 @@ def test_cli_run_matches_oracle(tmp_path, binary, flags):
         assert "Rebuild the vectors" in log and "COLLISION" in log
 
 
+@pytest.mark.parametrize("name,method", [("euler", 0), ("rk2", 1), ("rk4", 2)])
+def test_cli_fixed_step_integrators(tmp_path, name, method):
+    # the ring with collisions, 0.1 yr at h = 0.5 d: the final snapshot equals
+    # the oracle's driver loop with the same integrator, bit for bit
+    r, v, p, b, nbt = _system()
+    dt = 0.5 * R.K_GAUSS
+    params = dict(PARAMS, integrator=name, length="0.1")
+    del params["tolerance"]
+    d = str(tmp_path)
+    CF.write_run(d, r, v, p, b, nbt, 0.0, dt, params)
+    _run(d)
+    thr = (float(params["hit centrum"]), float(params["ejection"]), float(params["radii enhance factor"]))
+    o = O.ODisk(r, v, p, b, nbt, thr=thr)
+    oi = O.OInteg(o, method, dt, adaptive=True, tol=1e-10)   # the parameter default: adaptive, 1e-10
+    rc, _ = oi.run(CF.internal_time(params["length"]), CF.internal_time(params["output interval"]), 2 ** 62)
+    assert rc == 0
+    last = sorted(glob.glob(os.path.join(d, "data_*[0-9].txt")))[-1]
+    _, ids, rr, vv, _ = CF.read_data(last, R.BODY_MD)
+    ot, orr, ovv, _, obb = o.state()
+    act = obb["id"] > 0
+    assert np.array_equal(ids, obb["id"][act])
+    assert np.array_equal(rr, np.stack([orr["x"], orr["y"], orr["z"]], 1)[act])
+    assert np.array_equal(vv, np.stack([ovv["x"], ovv["y"], ovv["z"]], 1)[act])
+    assert float(open(last.replace(".txt", ".info.txt")).read().split()[0]) == ot
+
+
 def test_cli_continues_from_its_start_file(tmp_path):
     r, v, p, b, nbt = _system()
     d = str(tmp_path)

@@ -131,6 +131,33 @@ def test_adaptive_bitexact_with_rejections(method):
     assert t == ot and bits_equal(rr, orr) and bits_equal(vv, ovv)
 
 
+@pytest.mark.parametrize("method", [R.RD_EULER, R.RD_RK2])
+def test_euler_rk2_bitexact(method):
+    """Euler (int_euler.cu) and midpoint RK2 (int_rungekutta2.cu) against the
+    oracle, bit for bit: single steps, a dt_next the methods must ignore, and
+    the chained fixed-step path of rd_steps."""
+    r, v, p, b, nbt = small_system(300, 61, mixed_types(300))
+    d, o = pair(r, v, p, b, nbt)
+    h = 0.5 * KDAY
+    it = R.Integrator(d, method, h, adaptive=True, tol=1e-10)   # fixed step whatever adaptive says
+    oi = O.OInteg(o, method, h, adaptive=True, tol=1e-10)
+    for _ in range(5):
+        assert it.step() == oi.step() == h
+    it.set_dt_next(h / 3)
+    O.lib.or_set_dt_next(oi.h, h / 3)
+    for _ in range(5):
+        assert it.step() == oi.step() == h
+    assert it.dt_next() == oi.dt_next() == h
+    it.steps(20)
+    for _ in range(20):
+        oi.step()
+    t, rr, vv, _, _ = d.state()
+    ot, orr, ovv, _, _ = o.state()
+    assert t == ot
+    assert bits_equal(rr, orr) and bits_equal(vv, ovv)
+    assert it.counters() == oi.counters() == (30, 30, 0)
+
+
 def test_graph_matches_direct_launch():
     out = []
     for g in (0, 1):

@@ -154,10 +154,13 @@ def kepler_binary(e=0.3):
     return r, v, p, b, nbt, P
 
 
-@pytest.mark.parametrize("method,order", [(2, 4), (4, 7)])
-def test_fixed_step_convergence_order(method, order):
+@pytest.mark.parametrize("method,order,steps", [(0, 1, (4096, 8192)), (1, 2, (512, 1024)), (2, 4, (64, 128)),
+                                               (4, 7, (64, 128))])
+def test_fixed_step_convergence_order(method, order, steps):
+    # Euler (int_euler.cu), midpoint RK2 (int_rungekutta2.cu), RK4, RKF7(8): the
+    # error after one period halves 2^order times when the step halves
     errs = []
-    for nsteps in (64, 128):
+    for nsteps in steps:
         r, v, p, b, nbt, P = kepler_binary()
         d = O.ODisk(r, v, p, b, nbt)
         it = O.OInteg(d, method, P / nsteps, adaptive=False)
@@ -167,6 +170,23 @@ def test_fixed_step_convergence_order(method, order):
         errs.append(np.hypot(rr["x"][1] - r["x"][1], rr["y"][1] - r["y"][1]))
     rate = np.log2(errs[0] / errs[1])
     assert rate > order - 0.6, (errs, rate)
+    assert rate < order + 0.6, (errs, rate)
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_euler_rk2_keep_their_step(method):
+    # euler::step and rungekutta2::step use dt_try, the constructor's dt, and
+    # set dt_next back to it (int_euler.cu:78-105, int_rungekutta2.cu:99-131):
+    # the driver's clamp to the output interval does not shorten their steps
+    r, v, p, b, nbt, P = kepler_binary()
+    d = O.ODisk(r, v, p, b, nbt)
+    h = P / 500
+    it = O.OInteg(d, method, h, adaptive=True, tol=1e-12)
+    assert it.step() == h
+    O.lib.or_set_dt_next(it.h, h / 7)
+    assert it.step() == h and it.dt_next() == h
+    assert it.counters() == (2, 2, 0)
+    assert d.t == h + h
 
 
 def test_rkf78_adaptive_closed_orbit_and_controller():
