@@ -71,3 +71,20 @@ def read_data(path, body_md_dtype):
 def internal_time(years_text):
     """parameter::set_param + transform_time: years -> day -> k day, in that order."""
     return float(years_text) * YEAR * K_GAUSS
+
+
+def read_run(directory, var4, param, body_md):
+    """(r, v, p, b, n_by_type, t, dt) of the text run named by start_files.txt."""
+    info_fn, data_fn = open(os.path.join(directory, "start_files.txt")).read().split()[:2]
+    info = open(os.path.join(directory, info_fn)).read().split()
+    t, dt, nbt = float(info[0]), float(info[1]), np.array([int(x) for x in info[3:10]], np.uint32)
+    rows = [line.split() for line in open(os.path.join(directory, data_fn))]
+    n = len(rows)
+    r, v, p, b = np.zeros(n, var4), np.zeros(n, var4), np.zeros(n, param), np.zeros(n, body_md)
+    for i, f in enumerate(rows):
+        b["id"][i], b["body_type"][i], b["mig_type"][i] = int(f[1]), int(f[2]), int(f[3])
+        b["mig_stop_at"][i] = float(f[4])
+        p["mass"][i], p["radius"][i], p["density"][i], p["cd"][i] = (float(x) for x in f[5:9])
+        r["x"][i], r["y"][i], r["z"][i] = (float(x) for x in f[9:12])
+        v["x"][i], v["y"][i], v["z"][i] = (float(x) for x in f[12:15])
+    return r, v, p, b, nbt, t, dt

@@ -115,6 +115,33 @@ def test_cli_fixed_step_integrators(tmp_path, name, method):
     assert float(open(last.replace(".txt", ".info.txt")).read().split()[0]) == ot
 
 
+def test_redinit_then_red(tmp_path):
+    # red's initial-condition generator writes a run directory that red reads:
+    # 10 giant planets (n_gp preset), 2 years of RKF7(8) against the oracle
+    gen = os.path.join(os.path.dirname(RED), "redinit")
+    res = subprocess.run([gen, "-preset", "n_gp", "-seed", "12345", "-odir", str(tmp_path)], capture_output=True,
+                         text=True, timeout=60)
+    assert res.returncode == 0, res.stderr
+    r, v, p, b, nbt, t0, dt = CF.read_run(str(tmp_path), R.VAR4, R.PARAM, R.BODY_MD)
+    params = {"name": "giants", "integrator": "rkf8", "tolerance": "1.0e-10", "length": "2.0",
+              "output interval": "0.5"}
+    with open(os.path.join(str(tmp_path), "parameters.txt"), "w") as f:
+        for k, val in params.items():
+            f.write(f"{k} = {val}\n")
+    _run(str(tmp_path))
+    o = O.ODisk(r, v, p, b, nbt, t=t0)
+    oi = O.OInteg(o, 4, dt, adaptive=True, tol=1e-10)
+    rc, steps = oi.run(CF.internal_time("2.0"), CF.internal_time("0.5"), 2 ** 62)
+    assert rc == 0 and steps > 10
+    last = sorted(glob.glob(os.path.join(str(tmp_path), "data_*[0-9].txt")))[-1]
+    names, ids, rr, vv, _ = CF.read_data(last, R.BODY_MD)
+    ot, orr, ovv, _, obb = o.state()
+    assert names[0] == "star" and names[1] == "giant1"
+    assert np.array_equal(ids, obb["id"])
+    assert np.array_equal(rr, np.stack([orr["x"], orr["y"], orr["z"]], 1))
+    assert np.array_equal(vv, np.stack([ovv["x"], ovv["y"], ovv["z"]], 1))
+
+
 def test_cli_continues_from_its_start_file(tmp_path):
     r, v, p, b, nbt = _system()
     d = str(tmp_path)
