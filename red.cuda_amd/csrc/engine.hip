@@ -142,6 +142,7 @@ struct rd_disk {
     double* scratch = nullptr;
     size_t scratch_len = 0;
     double* red_out = nullptr;     // 12 doubles for integrals
+    double* trash = nullptr;       // [3][kBodyBlock], written by single-disk stages that do not apply
     uint32_t* feat = nullptr;      // step-proposer features, 3 per disk
     double* esum = nullptr;        // Keplerian energy per disk
     double* eterm = nullptr;       // per-body Keplerian energy [n_all]
@@ -230,7 +231,7 @@ void free_buffers(rd_disk* d)
 {
     double** dp[] = {&d->x, &d->y, &d->z, &d->vx, &d->vy, &d->vz, &d->xo, &d->yo, &d->zo, &d->vxo, &d->vyo,
                      &d->vzo, &d->K, &d->part, &d->mass, &d->radius, &d->density, &d->cd, &d->m_ex, &d->scratch,
-                     &d->red_out, &d->eterm};
+                     &d->red_out, &d->eterm, &d->trash};
     for (auto p : dp) dev_free(*p);
     for (int b = 0; b < 2; ++b)
         for (int c = 0; c < 3; ++c) dev_free(d->P[b][c]);
@@ -279,6 +280,7 @@ int alloc_buffers(rd_disk* d)
     if (int rc = dev_alloc(&d->ctl, d->n_disk)) return rc;
     if (int rc = dev_alloc(&d->err, d->n_disk)) return rc;
     if (int rc = dev_alloc(&d->red_out, 16)) return rc;
+    if (int rc = dev_alloc(&d->trash, 3 * (size_t)kBodyBlock)) return rc;
     if (int rc = dev_alloc(&d->feat, 3 * (size_t)d->n_disk)) return rc;
     if (int rc = dev_alloc(&d->esum, d->n_disk)) return rc;
     if (int rc = dev_alloc(&d->fx, (size_t)RD_MLP_IN * d->n_disk)) return rc;
@@ -597,6 +599,9 @@ StageArgs make_args(rd_disk* d, int r, int adaptive, int err_all)
     a.gas = d->gas;
     a.mixed = d->precision == RD_PREC_MIXED;
     a.adaptive = adaptive;
+    a.one_disk = d->n_disk == 1;
+    if (a.one_disk && !d->h_info.empty()) a.info0 = d->h_info[0];
+    a.trash = d->trash;
     return a;
 }
 

@@ -17,7 +17,8 @@
 namespace rdamd {
 
 constexpr int kMaxStage = 13;
-constexpr int kBlock = 256;          // threads per block for every kernel
+constexpr int kBlock = 256;          // threads per block of the pair kernels
+constexpr int kBodyBlock = 64;       // per-body kernels: one wave per block, so small N still spreads over all CUs
 constexpr int kTile = 256;           // sources staged in LDS per tile
 // sinks per work item of the exact (FP64, reference-order) pair kernel
 constexpr uint32_t kExactSinks = 16;
@@ -95,6 +96,9 @@ struct StageArgs {
     GasDev gas;
     int mixed;
     int adaptive;
+    int one_disk;                // every body belongs to disk 0
+    DiskInfo info0;              // info[0] when one_disk (kernel arguments are scalar loads)
+    double* trash;               // [3][kBodyBlock]: stores of a single-disk stage that does not apply
 };
 
 // Launchers (kernels.hip).

@@ -211,10 +211,6 @@ struct alignas(16) SrcF {
 // touching), so the floor changes r^2 by < 1e-8 ulp.
 constexpr float kEps2 = 1.0e-24f;
 
-// Per-body kernels run one wave per block so that small N still spreads over
-// all CUs.
-constexpr int kBodyBlock = 64;
-
 // Mixed precision pair kernel.  A work item is 64*S sinks (S per lane) against
 // a chunk of sources; the four waves of the block split each 256-source LDS
 // tile in quarters.  Per pair: FP64 offset rounded to FP32, r^2 by FMA, one
@@ -708,33 +704,69 @@ __device__ double partial_sum(const StageArgs& a, uint32_t i, int q, uint32_t nc
     return s;
 }
 
-// Component q of the acceleration of body i at this stage from the summed
-// partials s: FP64 prefix in mixed mode, zero for inactive sinks, plus drag.
-// pos: force-input positions; vel: velocities of this stage.
-__device__ double stage_accel(const StageArgs& a, uint32_t i, int q, const DiskInfo& di, double t, double s,
+// partial_sum for a launch over one disk: the loop runs to ncmax, the
+// disk's largest slot count (uniform, from the kernel arguments), and each
+// lane adds its own nc slots in slot order.  The first batch of loads then
+// waits for nothing.  Slots in [nc, ncmax) lie inside the partial buffer,
+// which holds the largest slot count of any body.
+__device__ double partial_sum_upto(const StageArgs& a, uint32_t i, int q, uint32_t nc, uint32_t ncmax)
+{
+    const double* __restrict__ p = a.part + (size_t)q * a.n_all + i;
+    const size_t st = (size_t)3 * a.n_all;
+    double s = 0.0;
+    for (uint32_t c = 0; c < ncmax; c += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = (c + u < ncmax) ? p[(size_t)(c + u) * st] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) s = (c + u >= nc) ? s : ((c + u == 0) ? v[u] : s + v[u]);
+    }
+    return s;
+}
+
+// Mixed mode: the FP64 prefix sources' (the star block's) acceleration on
+// body i, component q, reference formula and order; 0 in FP64 mode.
+// pos: force-input positions; s0, if given: the first prefix source's
+// position, mass and id, loaded by the caller.
+struct PrefixBody {
+    double x, y, z, m;
+    int id;
+};
+
+__device__ double prefix_accel(const StageArgs& a, uint32_t i, int q, const DiskInfo& di,
+                               const double* __restrict__ const* pos, const PrefixBody* s0 = nullptr)
+{
+    if (!a.mixed) return 0.0;
+    const uint32_t li = i - di.off;
+    const int cls = li < di.n_si ? 0 : (li < di.n_massive ? 1 : 2);
+    const double xi = pos[0][i], yi = pos[1][i], zi = pos[2][i];
+    const uint32_t src_hi = cls == 1 ? di.n_si : di.n_massive;
+    const uint32_t hi = min(di.n_fp64, src_hi);
+    double b = 0.0;
+    for (uint32_t lj = 0; lj < hi; ++lj) {
+        const uint32_t j = di.off + lj;
+        const bool pre = s0 && lj == 0;
+        if (j == i || (pre ? s0->id : a.id[j]) < 0) continue;
+        const double dx = (pre ? s0->x : pos[0][j]) - xi;
+        const double dy = (pre ? s0->y : pos[1][j]) - yi;
+        const double dz = (pre ? s0->z : pos[2][j]) - zi;
+        const double r2 = SQR(dx) + SQR(dy) + SQR(dz);
+        const double d = sqrt(r2);
+        const double sc = (pre ? s0->m : a.mass[j]) / (d * r2);
+        b += sc * (q == 0 ? dx : (q == 1 ? dy : dz));
+    }
+    return b;
+}
+
+// Component q of the acceleration of body i at this stage from the FP64
+// prefix part b and the summed partials s: b + s in mixed mode, zero for
+// inactive sinks, plus drag.  pos: force-input positions; vel: velocities of
+// this stage.
+__device__ double stage_accel(const StageArgs& a, uint32_t i, int q, const DiskInfo& di, double t, double b, double s,
                               const double* __restrict__ const* pos, const double* __restrict__ const* vel)
 {
     const uint32_t li = i - di.off;
-    const int cls = li < di.n_si ? 0 : (li < di.n_massive ? 1 : 2);
-    if (a.mixed) {
-        // FP64 prefix sources (the star block), reference formula and order.
-        const double xi = pos[0][i], yi = pos[1][i], zi = pos[2][i];
-        const uint32_t src_hi = cls == 1 ? di.n_si : di.n_massive;
-        const uint32_t hi = min(di.n_fp64, src_hi);
-        double b = 0.0;
-        for (uint32_t lj = 0; lj < hi; ++lj) {
-            const uint32_t j = di.off + lj;
-            if (j == i || a.id[j] < 0) continue;
-            const double dx = pos[0][j] - xi;
-            const double dy = pos[1][j] - yi;
-            const double dz = pos[2][j] - zi;
-            const double r2 = SQR(dx) + SQR(dy) + SQR(dz);
-            const double d = sqrt(r2);
-            const double sc = a.mass[j] / (d * r2);
-            b += sc * (q == 0 ? dx : (q == 1 ? dy : dz));
-        }
-        s = b + s;
-    }
+    if (a.mixed) s = b + s;
     if (!(a.id[i] > 0)) s = 0.0;
     if (a.gas.on && li >= di.drag_lo && li < di.drag_hi) {
         const double decr = reduction_factor(a.gas, t);
@@ -753,7 +785,7 @@ __device__ inline void pin(double& v) { asm volatile("" : "+v"(v)); }
 __device__ inline uint32_t nchunk_of(const DiskInfo& di, uint32_t i)
 {
     const uint32_t li = i - di.off;
-    return di.nchunk[li < di.n_si ? 0 : (li < di.n_massive ? 1 : 2)];
+    return li < di.n_si ? di.nchunk[0] : (li < di.n_massive ? di.nchunk[1] : di.nchunk[2]);
 }
 
 // Column values of stage row R (int_rungekutta8.cu:82-246) for body i,
@@ -810,18 +842,107 @@ __device__ void error_reduce(const StageArgs& a, bool valid, uint32_t d, double 
     }
 }
 
-// RKF7(8) stage kernel for stage R: finishes k_R (partials, FP64 prefix,
-// drag) and forms what the next stage needs from body i's own k values.
-// Loads whose address depends only on i are issued before the per-disk
-// control is known.
+// RKF7(8) stage R for a launch over a single disk.  The disk's info comes in
+// the kernel arguments and its control load goes out with the per-body
+// loads; every value is then computed without a branch, and a stage that
+// does not apply to the disk (its flags) stores into a.trash instead of the
+// state.  With no branch to sink them into, the per-body loads, the control
+// load and the first batch of partials share one memory round trip (a branch
+// on the flags would put the control load's round trip in front of them).
 template <int R>
-__global__ __launch_bounds__(kBodyBlock) void k_stage_rk8(StageArgs a)
+__device__ __forceinline__ void stage_rk8_one(const StageArgs& a)
 {
     const uint32_t i0 = blockIdx.x * kBodyBlock + threadIdx.x;
     const bool in = i0 < a.n_all;
     const uint32_t i = in ? i0 : a.n_all - 1;
     const int q = blockIdx.y;
-    // round 1
+    const DiskInfo& di = a.info0;
+    const DiskCtl c = a.ctl[0];
+    // slot count of body i's class, as arithmetic on the uniform counts
+    const uint32_t li = i - di.off;
+    const uint32_t nc = di.nchunk[0] + (li >= di.n_si ? di.nchunk[1] - di.nchunk[0] : 0u) +
+                        (li >= di.n_massive ? di.nchunk[2] - di.nchunk[1] : 0u);
+    const uint32_t ncmax = max(di.nchunk[0], max(di.nchunk[1], di.nchunk[2]));
+    const uint32_t j0 = di.off;
+    const PrefixBody s0{a.fpos[0][j0], a.fpos[1][j0], a.fpos[2][j0], a.mass[j0], a.id[j0]};
+    const double yy = a.pos[q][i], vv = a.vel[q][i];
+    const double kk = kv_of(a, R, q)[i];
+    constexpr int NR = (R >= 1 && R <= 9) ? R + 1 : (R == 10 ? 11 : (R == 11 ? 12 : 1));
+    double kv[9], ka[9];
+    if constexpr (R >= 1 && R <= 11) rk8_load_row<NR, R>(a, i, q, kv, ka);
+    double y10v[6], y10a[6];
+    if constexpr (R == 10) {
+        const int cols[6] = {0, 5, 6, 7, 8, 9};
+#pragma unroll
+        for (int u = 0; u < 6; ++u) {
+            y10v[u] = kv_of(a, cols[u], q)[i];
+            y10a[u] = ka_of(a, cols[u], q)[i];
+        }
+    }
+    double e0v = 0, e10v = 0, e11v = 0, e0a = 0, e10a = 0, e11a = 0;
+    if constexpr (R == 12) {
+        e0v = kv_of(a, 0, q)[i]; e10v = kv_of(a, 10, q)[i]; e11v = kv_of(a, 11, q)[i];
+        e0a = ka_of(a, 0, q)[i]; e10a = ka_of(a, 10, q)[i]; e11a = ka_of(a, 11, q)[i];
+    }
+    const double ps = partial_sum_upto(a, i, q, nc, ncmax);
+    const double b = prefix_accel(a, i, q, di, a.fpos, &s0);
+    const bool valid = in && (c.flags & (R == 0 ? F_NEED_K0 : F_ACTIVE)) != 0;
+    const double h = c.h;
+    const double t = c.t + rk8_c(R) * h;
+    const double* vR[3] = {kv_of(a, R, 0), kv_of(a, R, 1), kv_of(a, R, 2)};
+    const double kRa = stage_accel(a, i, q, di, t, b, ps, a.fpos, vR);
+    double* const trash = a.trash + (size_t)q * kBodyBlock + threadIdx.x;
+    auto at = [&](double* p) { return valid ? p + i : trash; };
+    *at(ka_w(a, R, q)) = kRa;
+    if constexpr (R >= 1 && R <= 9) {
+        double p, w;
+        rk8_comb_q<NR, R>(yy, vv, h, kv, ka, kk, kRa, p, w);
+        *at(a.npos[q]) = p;
+        *at(kv_w(a, NR, q)) = w;
+    } else if constexpr (R == 10) {
+        // y_{n+1} = y + h (b0 k0 + b5 k5 + b6 (k6 + k7) + b8 (k8 + k9) + b10 k10) (int_rungekutta8.cu:264-274)
+        constexpr double b0 = 41.0 / 840.0, b5 = 34.0 / 105.0, b6 = 9.0 / 35.0, b8 = 9.0 / 280.0,
+                         b10 = 41.0 / 840.0;
+        *at(a.out[q]) = yy + h * (b0 * y10v[0] + b5 * y10v[1] + b6 * (y10v[2] + y10v[3]) +
+                                 b8 * (y10v[4] + y10v[5]) + b10 * kk);
+        *at(a.out[3 + q]) = vv + h * (b0 * y10a[0] + b5 * y10a[1] + b6 * (y10a[2] + y10a[3]) +
+                                     b8 * (y10a[4] + y10a[5]) + b10 * kRa);
+        if (a.adaptive) {
+            double p, w;
+            rk8_comb_q<11, R>(yy, vv, h, kv, ka, kk, kRa, p, w);
+            *at(a.npos[q]) = p;
+            *at(kv_w(a, 11, q)) = w;
+        }
+    } else if constexpr (R == 11) {
+        double p, w;
+        rk8_comb_q<12, R>(yy, vv, h, kv, ka, kk, kRa, p, w);
+        *at(a.npos[q]) = p;
+        *at(kv_w(a, 12, q)) = w;
+    } else if constexpr (R == 12) {
+        double e = 0.0;
+        // |f0 + f10 - f11 - f12| (int_rungekutta8.cu:250-260)
+        if (valid && (a.err_all || li < di.err_hi))
+            e = fmax(fabs(e0v + e10v - e11v - kk), fabs(e0a + e10a - e11a - kRa));
+        error_reduce(a, valid, 0, e);
+    }
+}
+
+// RKF7(8) stage kernel for stage R: finishes k_R (partials, FP64 prefix,
+// drag) and forms what the next stage needs from body i's own k values.
+// Loads whose address depends only on i are issued before the per-disk
+// control is known.
+template <int R, bool ONE>
+__global__ __launch_bounds__(kBodyBlock) void k_stage_rk8(StageArgs a)
+{
+    if constexpr (ONE) {
+        stage_rk8_one<R>(a);
+        return;
+    }
+    const uint32_t i0 = blockIdx.x * kBodyBlock + threadIdx.x;
+    const bool in = i0 < a.n_all;
+    const uint32_t i = in ? i0 : a.n_all - 1;
+    const int q = blockIdx.y;
+    // first the loads whose address depends on i only
     const uint32_t d = a.disk_of[i];
     const double y = a.pos[q][i], v = a.vel[q][i];
     const double kRv = kv_of(a, R, q)[i];
@@ -861,7 +982,7 @@ __global__ __launch_bounds__(kBodyBlock) void k_stage_rk8(StageArgs a)
     }
     double yy = y, vv = v, kk = kRv;
     pin(yy); pin(vv); pin(kk);
-    // round 2
+    // then the per-disk control
     const DiskCtl c = a.ctl[d];
     const DiskInfo di = a.info[d];
     const bool valid = in && (c.flags & (R == 0 ? F_NEED_K0 : F_ACTIVE)) != 0;
@@ -870,7 +991,8 @@ __global__ __launch_bounds__(kBodyBlock) void k_stage_rk8(StageArgs a)
         const double h = c.h;
         const double t = c.t + rk8_c(R) * h;
         const double* vR[3] = {kv_of(a, R, 0), kv_of(a, R, 1), kv_of(a, R, 2)};
-        const double kRa = stage_accel(a, i, q, di, t, partial_sum(a, i, q, nchunk_of(di, i)), a.fpos, vR);
+        const double b = prefix_accel(a, i, q, di, a.fpos);
+        const double kRa = stage_accel(a, i, q, di, t, b, partial_sum(a, i, q, nchunk_of(di, i)), a.fpos, vR);
         ka_w(a, R, q)[i] = kRa;
         if constexpr (R >= 1 && R <= 9) {
             double p, w;
@@ -936,7 +1058,8 @@ __global__ __launch_bounds__(kBodyBlock) void k_stage_rk4(StageArgs a)
         const double h = c.h;
         const double t = c.t + rk4_c(R) * h;
         const double* vR[3] = {kv_of(a, R, 0), kv_of(a, R, 1), kv_of(a, R, 2)};
-        const double kRa = stage_accel(a, i, q, di, t, partial_sum(a, i, q, nchunk_of(di, i)), a.fpos, vR);
+        const double b = prefix_accel(a, i, q, di, a.fpos);
+        const double kRa = stage_accel(a, i, q, di, t, b, partial_sum(a, i, q, nchunk_of(di, i)), a.fpos, vR);
         ka_w(a, R, q)[i] = kRa;
         if constexpr (R == 1 || R == 2) {
             const double fac = rk4_a(R + 1) * h;
@@ -975,7 +1098,8 @@ __global__ __launch_bounds__(kBodyBlock) void k_stage_euler(StageArgs a)
     if (!(c.flags & F_NEED_K0)) return;
     const DiskInfo di = a.info[d];
     const double y = a.pos[q][i], v = a.vel[q][i];
-    const double kRa = stage_accel(a, i, q, di, c.t, partial_sum(a, i, q, nchunk_of(di, i)), a.fpos, a.vel);
+    const double b = prefix_accel(a, i, q, di, a.fpos);
+    const double kRa = stage_accel(a, i, q, di, c.t, b, partial_sum(a, i, q, nchunk_of(di, i)), a.fpos, a.vel);
     ka_w(a, 0, q)[i] = kRa;
     a.out[q][i] = y + c.h * v;
     a.out[3 + q][i] = v + c.h * kRa;
@@ -997,7 +1121,8 @@ __global__ __launch_bounds__(kBodyBlock) void k_stage_rk2(StageArgs a)
     const double h = c.h;
     const double t = c.t + (R == 0 ? 0.0 : 1.0 / 2.0) * h;
     const double* vR[3] = {kv_of(a, R, 0), kv_of(a, R, 1), kv_of(a, R, 2)};
-    const double kRa = stage_accel(a, i, q, di, t, partial_sum(a, i, q, nchunk_of(di, i)), a.fpos,
+    const double b = prefix_accel(a, i, q, di, a.fpos);
+    const double kRa = stage_accel(a, i, q, di, t, b, partial_sum(a, i, q, nchunk_of(di, i)), a.fpos,
                                    R == 0 ? a.vel : vR);
     ka_w(a, R, q)[i] = kRa;
     if constexpr (R == 1) {
@@ -1059,13 +1184,22 @@ __global__ __launch_bounds__(kBodyBlock) void k_operator(StageArgs a, double t, 
     if (li >= di.n) return;
     const uint32_t i = di.off + li;
     const double* vel[3] = {vx, vy, vz};
-    const double s = stage_accel(a, i, q, di, t, partial_sum(a, i, q, nchunk_of(di, i)), a.fpos, vel);
+    const double b = prefix_accel(a, i, q, di, a.fpos);
+    const double s = stage_accel(a, i, q, di, t, b, partial_sum(a, i, q, nchunk_of(di, i)), a.fpos, vel);
     (q == 0 ? ax : (q == 1 ? ay : az))[i] = s;
 }
 
 #define STAGE_CASE(M, R)                                                                   \
     case R:                                                                                \
         hipLaunchKernelGGL(k_stage_##M<R>, grid, dim3(kBodyBlock), 0, st, a);              \
+        break;
+
+#define STAGE_CASE_RK8(R)                                                                  \
+    case R:                                                                                \
+        if (a.one_disk)                                                                    \
+            hipLaunchKernelGGL((k_stage_rk8<R, true>), grid, dim3(kBodyBlock), 0, st, a);  \
+        else                                                                               \
+            hipLaunchKernelGGL((k_stage_rk8<R, false>), grid, dim3(kBodyBlock), 0, st, a); \
         break;
 
 static dim3 body_grid(uint32_t n) { return dim3((n + kBodyBlock - 1) / kBodyBlock, 3); }
@@ -1076,9 +1210,9 @@ hipError_t launch_stage(hipStream_t st, int method, int r, const StageArgs& a)
     const dim3 grid = body_grid(a.n_all);
     if (method == 4) {
         switch (r) {
-            STAGE_CASE(rk8, 0) STAGE_CASE(rk8, 1) STAGE_CASE(rk8, 2) STAGE_CASE(rk8, 3) STAGE_CASE(rk8, 4)
-            STAGE_CASE(rk8, 5) STAGE_CASE(rk8, 6) STAGE_CASE(rk8, 7) STAGE_CASE(rk8, 8) STAGE_CASE(rk8, 9)
-            STAGE_CASE(rk8, 10) STAGE_CASE(rk8, 11) STAGE_CASE(rk8, 12)
+            STAGE_CASE_RK8(0) STAGE_CASE_RK8(1) STAGE_CASE_RK8(2) STAGE_CASE_RK8(3) STAGE_CASE_RK8(4)
+            STAGE_CASE_RK8(5) STAGE_CASE_RK8(6) STAGE_CASE_RK8(7) STAGE_CASE_RK8(8) STAGE_CASE_RK8(9)
+            STAGE_CASE_RK8(10) STAGE_CASE_RK8(11) STAGE_CASE_RK8(12)
         default: return hipErrorInvalidValue;
         }
     } else if (method == 0) {
