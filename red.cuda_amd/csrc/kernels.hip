@@ -842,13 +842,35 @@ __device__ void error_reduce(const StageArgs& a, bool valid, uint32_t d, double 
     }
 }
 
-// RKF7(8) stage R for a launch over a single disk.  The disk's info comes in
+// Stage kernels over a single disk (ONE = true).  The disk's info comes in
 // the kernel arguments and its control load goes out with the per-body
 // loads; every value is then computed without a branch, and a stage that
 // does not apply to the disk (its flags) stores into a.trash instead of the
 // state.  With no branch to sink them into, the per-body loads, the control
 // load and the first batch of partials share one memory round trip (a branch
 // on the flags would put the control load's round trip in front of them).
+// Stage 0 is the exception: it returns early when k0 is reused after a
+// rejected attempt, since it then has nothing to do.
+
+// Summed partials ps and FP64 prefix part b of body i's acceleration, for a
+// single-disk stage.  Called after the body's own loads are issued, so that
+// the first batch of partials goes out with them.
+__device__ __forceinline__ void one_disk_sums(const StageArgs& a, uint32_t i, int q, double& ps, double& b)
+{
+    const DiskInfo& di = a.info0;
+    // slot count of body i's class, as arithmetic on the uniform counts (a
+    // dynamic index into nchunk would put the info in LDS)
+    const uint32_t li = i - di.off;
+    const uint32_t nc = di.nchunk[0] + (li >= di.n_si ? di.nchunk[1] - di.nchunk[0] : 0u) +
+                        (li >= di.n_massive ? di.nchunk[2] - di.nchunk[1] : 0u);
+    const uint32_t ncmax = max(di.nchunk[0], max(di.nchunk[1], di.nchunk[2]));
+    const uint32_t j0 = di.off;
+    const PrefixBody s0{a.fpos[0][j0], a.fpos[1][j0], a.fpos[2][j0], a.mass[j0], a.id[j0]};
+    ps = partial_sum_upto(a, i, q, nc, ncmax);
+    b = prefix_accel(a, i, q, di, a.fpos, &s0);
+}
+
+// RKF7(8) stage R over a single disk.
 template <int R>
 __device__ __forceinline__ void stage_rk8_one(const StageArgs& a)
 {
@@ -858,13 +880,8 @@ __device__ __forceinline__ void stage_rk8_one(const StageArgs& a)
     const int q = blockIdx.y;
     const DiskInfo& di = a.info0;
     const DiskCtl c = a.ctl[0];
-    // slot count of body i's class, as arithmetic on the uniform counts
-    const uint32_t li = i - di.off;
-    const uint32_t nc = di.nchunk[0] + (li >= di.n_si ? di.nchunk[1] - di.nchunk[0] : 0u) +
-                        (li >= di.n_massive ? di.nchunk[2] - di.nchunk[1] : 0u);
-    const uint32_t ncmax = max(di.nchunk[0], max(di.nchunk[1], di.nchunk[2]));
-    const uint32_t j0 = di.off;
-    const PrefixBody s0{a.fpos[0][j0], a.fpos[1][j0], a.fpos[2][j0], a.mass[j0], a.id[j0]};
+    if constexpr (R == 0)
+        if (!(c.flags & F_NEED_K0)) return;
     const double yy = a.pos[q][i], vv = a.vel[q][i];
     const double kk = kv_of(a, R, q)[i];
     constexpr int NR = (R >= 1 && R <= 9) ? R + 1 : (R == 10 ? 11 : (R == 11 ? 12 : 1));
@@ -884,8 +901,8 @@ __device__ __forceinline__ void stage_rk8_one(const StageArgs& a)
         e0v = kv_of(a, 0, q)[i]; e10v = kv_of(a, 10, q)[i]; e11v = kv_of(a, 11, q)[i];
         e0a = ka_of(a, 0, q)[i]; e10a = ka_of(a, 10, q)[i]; e11a = ka_of(a, 11, q)[i];
     }
-    const double ps = partial_sum_upto(a, i, q, nc, ncmax);
-    const double b = prefix_accel(a, i, q, di, a.fpos, &s0);
+    double ps, b;
+    one_disk_sums(a, i, q, ps, b);
     const bool valid = in && (c.flags & (R == 0 ? F_NEED_K0 : F_ACTIVE)) != 0;
     const double h = c.h;
     const double t = c.t + rk8_c(R) * h;
@@ -919,6 +936,7 @@ __device__ __forceinline__ void stage_rk8_one(const StageArgs& a)
         *at(a.npos[q]) = p;
         *at(kv_w(a, 12, q)) = w;
     } else if constexpr (R == 12) {
+        const uint32_t li = i - di.off;
         double e = 0.0;
         // |f0 + f10 - f11 - f12| (int_rungekutta8.cu:250-260)
         if (valid && (a.err_all || li < di.err_hi))
@@ -1027,10 +1045,70 @@ __global__ __launch_bounds__(kBodyBlock) void k_stage_rk8(StageArgs a)
     if constexpr (R == 12) error_reduce(a, valid, d, e);
 }
 
-// Classical RK4 stage kernel (int_rungekutta4.cu:37-73, 130-247).
+// Classical RK4 stage R over a single disk.
 template <int R>
+__device__ __forceinline__ void stage_rk4_one(const StageArgs& a)
+{
+    const uint32_t i0 = blockIdx.x * kBodyBlock + threadIdx.x;
+    const bool in = i0 < a.n_all;
+    const uint32_t i = in ? i0 : a.n_all - 1;
+    const int q = blockIdx.y;
+    const DiskInfo& di = a.info0;
+    const DiskCtl c = a.ctl[0];
+    if constexpr (R == 0)
+        if (!(c.flags & F_NEED_K0)) return;
+    const double yy = a.pos[q][i], vv = a.vel[q][i];
+    const double kk = kv_of(a, R, q)[i];
+    double k0v = 0, k1v = 0, k2v = 0, k0a = 0, k1a = 0, k2a = 0, k3v = 0, k3a = 0;
+    if constexpr (R == 3) {
+        k0v = kv_of(a, 0, q)[i]; k1v = kv_of(a, 1, q)[i]; k2v = kv_of(a, 2, q)[i];
+        k0a = ka_of(a, 0, q)[i]; k1a = ka_of(a, 1, q)[i]; k2a = ka_of(a, 2, q)[i];
+    }
+    if constexpr (R == 4) {
+        k3v = kv_of(a, 3, q)[i];
+        k3a = ka_of(a, 3, q)[i];
+    }
+    double ps, b;
+    one_disk_sums(a, i, q, ps, b);
+    const bool valid = in && (c.flags & (R == 0 ? F_NEED_K0 : F_ACTIVE)) != 0;
+    const double h = c.h;
+    const double t = c.t + rk4_c(R) * h;
+    const double* vR[3] = {kv_of(a, R, 0), kv_of(a, R, 1), kv_of(a, R, 2)};
+    const double kRa = stage_accel(a, i, q, di, t, b, ps, a.fpos, vR);
+    double* const trash = a.trash + (size_t)q * kBodyBlock + threadIdx.x;
+    auto at = [&](double* p) { return valid ? p + i : trash; };
+    *at(ka_w(a, R, q)) = kRa;
+    if constexpr (R == 1 || R == 2) {
+        const double fac = rk4_a(R + 1) * h;
+        *at(a.npos[q]) = yy + fac * kk;
+        *at(kv_w(a, R + 1, q)) = vv + fac * kRa;
+    } else if constexpr (R == 3) {
+        // y + b0 (f1 + f4) + b1 (f2 + f3), b pre-multiplied by h (int_rungekutta4.cu:50-60, 150-171)
+        const double b0 = (1.0 / 6.0) * h, b1 = (1.0 / 3.0) * h;
+        const double yo = yy + b0 * (k0v + kk) + b1 * (k1v + k2v);
+        const double vo = vv + b0 * (k0a + kRa) + b1 * (k1a + k2a);
+        *at(a.out[q]) = yo;
+        *at(a.out[3 + q]) = vo;
+        if (a.adaptive) {
+            *at(a.npos[q]) = yo;
+            *at(kv_w(a, 4, q)) = vo;
+        }
+    } else if constexpr (R == 4) {
+        const uint32_t li = i - di.off;
+        double e = 0.0;
+        if (valid && (a.err_all || li < di.err_hi)) e = fmax(fabs(k3v - kk), fabs(k3a - kRa));   // |f4 - f5|
+        error_reduce(a, valid, 0, e);
+    }
+}
+
+// Classical RK4 stage kernel (int_rungekutta4.cu:37-73, 130-247).
+template <int R, bool ONE>
 __global__ __launch_bounds__(kBodyBlock) void k_stage_rk4(StageArgs a)
 {
+    if constexpr (ONE) {
+        stage_rk4_one<R>(a);
+        return;
+    }
     const uint32_t i0 = blockIdx.x * kBodyBlock + threadIdx.x;
     const bool in = i0 < a.n_all;
     const uint32_t i = in ? i0 : a.n_all - 1;
@@ -1194,12 +1272,12 @@ __global__ __launch_bounds__(kBodyBlock) void k_operator(StageArgs a, double t, 
         hipLaunchKernelGGL(k_stage_##M<R>, grid, dim3(kBodyBlock), 0, st, a);              \
         break;
 
-#define STAGE_CASE_RK8(R)                                                                  \
+#define STAGE_CASE_ONE(M, R)                                                               \
     case R:                                                                                \
         if (a.one_disk)                                                                    \
-            hipLaunchKernelGGL((k_stage_rk8<R, true>), grid, dim3(kBodyBlock), 0, st, a);  \
+            hipLaunchKernelGGL((k_stage_##M<R, true>), grid, dim3(kBodyBlock), 0, st, a);  \
         else                                                                               \
-            hipLaunchKernelGGL((k_stage_rk8<R, false>), grid, dim3(kBodyBlock), 0, st, a); \
+            hipLaunchKernelGGL((k_stage_##M<R, false>), grid, dim3(kBodyBlock), 0, st, a); \
         break;
 
 static dim3 body_grid(uint32_t n) { return dim3((n + kBodyBlock - 1) / kBodyBlock, 3); }
@@ -1210,9 +1288,10 @@ hipError_t launch_stage(hipStream_t st, int method, int r, const StageArgs& a)
     const dim3 grid = body_grid(a.n_all);
     if (method == 4) {
         switch (r) {
-            STAGE_CASE_RK8(0) STAGE_CASE_RK8(1) STAGE_CASE_RK8(2) STAGE_CASE_RK8(3) STAGE_CASE_RK8(4)
-            STAGE_CASE_RK8(5) STAGE_CASE_RK8(6) STAGE_CASE_RK8(7) STAGE_CASE_RK8(8) STAGE_CASE_RK8(9)
-            STAGE_CASE_RK8(10) STAGE_CASE_RK8(11) STAGE_CASE_RK8(12)
+            STAGE_CASE_ONE(rk8, 0) STAGE_CASE_ONE(rk8, 1) STAGE_CASE_ONE(rk8, 2) STAGE_CASE_ONE(rk8, 3)
+            STAGE_CASE_ONE(rk8, 4) STAGE_CASE_ONE(rk8, 5) STAGE_CASE_ONE(rk8, 6) STAGE_CASE_ONE(rk8, 7)
+            STAGE_CASE_ONE(rk8, 8) STAGE_CASE_ONE(rk8, 9) STAGE_CASE_ONE(rk8, 10) STAGE_CASE_ONE(rk8, 11)
+            STAGE_CASE_ONE(rk8, 12)
         default: return hipErrorInvalidValue;
         }
     } else if (method == 0) {
@@ -1227,7 +1306,8 @@ hipError_t launch_stage(hipStream_t st, int method, int r, const StageArgs& a)
         }
     } else {
         switch (r) {
-            STAGE_CASE(rk4, 0) STAGE_CASE(rk4, 1) STAGE_CASE(rk4, 2) STAGE_CASE(rk4, 3) STAGE_CASE(rk4, 4)
+            STAGE_CASE_ONE(rk4, 0) STAGE_CASE_ONE(rk4, 1) STAGE_CASE_ONE(rk4, 2) STAGE_CASE_ONE(rk4, 3)
+            STAGE_CASE_ONE(rk4, 4)
         default: return hipErrorInvalidValue;
         }
     }
