@@ -11,7 +11,8 @@ rejected attempt).  Inputs are resident in HBM before the timed region.
 Prints one JSON line:
   value         ordered pair interactions per second over the timed steps
                 (whole job: all stages, rejections, controller included)
-  rkf78_steps_per_s, energy error, force error vs the FP64 kernel
+  rkf78_steps_per_s, energy error, force error vs the FP64 kernel, and the
+                final-state difference vs FP64 after 50 fixed-h RKF7(8) steps
   roofline      pair kernel: 20 flop per pair over its HIP-event-timed
                 launch duration, against the 157.3 TFLOP/s FP32 peak
   cpu_baseline  the oracle's restatement of the reference CPU path
@@ -37,6 +38,7 @@ from proposer import batch_sizes  # noqa: E402
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
 FLOP_PER_PAIR = 20.0
+FS_STEPS = 50          # fixed-h steps of the final-state comparison against FP64 (config 2)
 GRAPH = 1   # replay each RK attempt from a captured hipGraph (--no-graph: direct launches)
 
 
@@ -476,6 +478,27 @@ def main():
         ge = np.stack([ae["x"], ae["y"], ae["z"]], 1)
         f_err = float((np.linalg.norm(gm - ge, axis=1) / np.linalg.norm(ge, axis=1))[1:].max())
 
+    # final-state difference against the FP64 kernel: the same fixed-h RKF7(8)
+    # steps (h = the timed run's mean accepted step) from the same initial state
+    fs_diff = None
+    if args.precision == "mixed":
+        h_fix = (t_end - t_start) / args.steps
+        ends = {}
+        for prec in (R.RD_PREC_MIXED, R.RD_PREC_FP64):
+            dd = R.Disk(r, v, p, b, nbt)
+            dd.set_precision(prec)
+            ii = R.Integrator(dd, R.RD_RKF78, h_fix, adaptive=False, tol=args.tol)
+            ii.steps(FS_STEPS)
+            dd.sync()
+            _, r_end, v_end, _, _ = dd.state()
+            ends[prec] = (np.stack([r_end["x"], r_end["y"], r_end["z"]], 1),
+                          np.stack([v_end["x"], v_end["y"], v_end["z"]], 1))
+            del ii, dd
+        (rm, vm), (re, ve) = ends[R.RD_PREC_MIXED], ends[R.RD_PREC_FP64]
+        fs_diff = {"steps": FS_STEPS, "h_days": h_fix / R.K_GAUSS,
+                   "max_rel_pos": float((np.linalg.norm(rm - re, axis=1) / np.linalg.norm(re, axis=1))[1:].max()),
+                   "max_rel_vel": float((np.linalg.norm(vm - ve, axis=1) / np.linalg.norm(ve, axis=1))[1:].max())}
+
     out = {
         "metric": "pair interactions/sec + RKF78 steps/sec at rel. energy error <=1e-9, 1 GPU",
         "value": value,
@@ -501,6 +524,7 @@ def main():
         "rel_energy_error": rel_e,
         "energy_ok": bool(rel_e <= 1e-9),
         "max_rel_force_err_vs_fp64": f_err,
+        "final_state_vs_fp64": fs_diff,
         "roofline": {"bound": "valu-fp32", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
                      "traffic": profiled_traffic(pair_kernel(args.precision, tuning),

@@ -1376,8 +1376,9 @@ __global__ __launch_bounds__(kBlock) void k_collision(const WorkItem* __restrict
                                                       double f2, uint32_t* pairs, uint32_t* counter, uint32_t cap)
 {
     __shared__ double4 tile[kTile];
-    __shared__ int32_t tid_ok[kTile];
     const uint32_t tid = threadIdx.x;
+    // an inactive source's radius is staged as NaN: r2 < f2 (ri + NaN)^2 is false
+    const double no_radius = __longlong_as_double(0x7ff8000000000000LL);
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         const WorkItem w = items[it];
         if (!(ctl[w.disk].flags & F_CHECK)) continue;
@@ -1388,16 +1389,15 @@ __global__ __launch_bounds__(kBlock) void k_collision(const WorkItem* __restrict
         for (uint32_t t0 = w.src_lo; t0 < w.src_hi; t0 += kTile) {
             __syncthreads();
             const uint32_t j = t0 + tid;
-            if (j < w.src_hi) {
-                tile[tid] = make_double4(x[j], y[j], z[j], radius[j]);
-                tid_ok[tid] = id[j] > 0;
-            }
+            if (j < w.src_hi) tile[tid] = make_double4(x[j], y[j], z[j], id[j] > 0 ? radius[j] : no_radius);
             __syncthreads();
             const uint32_t nt = min((uint32_t)kTile, w.src_hi - t0);
             if (valid) {
-                for (uint32_t k = 0; k < nt; ++k) {
+                // sources j > i only: the loop starts past i on a tile that holds it
+                const uint32_t k0 = t0 <= i ? i - t0 + 1 : 0u;
+#pragma unroll 4
+                for (uint32_t k = k0; k < nt; ++k) {
                     const uint32_t jj = t0 + k;
-                    if (jj <= i || !tid_ok[k]) continue;
                     const double4 s = tile[k];
                     const double dx = s.x - xi;
                     const double dy = s.y - yi;
