@@ -143,6 +143,8 @@ struct rd_disk {
     size_t scratch_len = 0;
     double* red_out = nullptr;     // 12 doubles for integrals
     double* trash = nullptr;       // [3][kBodyBlock], written by single-disk stages that do not apply
+    unsigned long long* errw = nullptr;     // single disk: per-wave error bits of the last stage
+    unsigned long long* h_errw = nullptr;   // pinned copy of errw
     uint32_t* feat = nullptr;      // step-proposer features, 3 per disk
     double* esum = nullptr;        // Keplerian energy per disk
     double* eterm = nullptr;       // per-body Keplerian energy [n_all]
@@ -239,6 +241,7 @@ void free_buffers(rd_disk* d)
     dev_free(d->type);
     dev_free(d->m32);
     dev_free(d->disk_of);
+    dev_free(d->errw);
     dev_free(d->info);
     dev_free(d->ctl);
     dev_free(d->err);
@@ -253,13 +256,16 @@ void free_buffers(rd_disk* d)
     dev_free(d->fx);
     dev_free(d->fy);
     for (void** h : {(void**)&d->h_ctl, (void**)&d->h_err, (void**)&d->h_feat, (void**)&d->h_esum, (void**)&d->h_fx,
-                     (void**)&d->h_fy}) {
+                     (void**)&d->h_fy, (void**)&d->h_errw}) {
         if (*h) (void)hipHostFree(*h);
         *h = nullptr;
     }
     d->part_len = 0;
     d->scratch_len = 0;
 }
+
+// Waves of a per-body launch (3 components): the single-disk error slots.
+size_t n_errw(const rd_disk* d) { return 3 * (((size_t)d->n_all + kBodyBlock - 1) / kBodyBlock) + 1; }
 
 int alloc_buffers(rd_disk* d)
 {
@@ -281,6 +287,7 @@ int alloc_buffers(rd_disk* d)
     if (int rc = dev_alloc(&d->err, d->n_disk)) return rc;
     if (int rc = dev_alloc(&d->red_out, 16)) return rc;
     if (int rc = dev_alloc(&d->trash, 3 * (size_t)kBodyBlock)) return rc;
+    if (int rc = dev_alloc(&d->errw, n_errw(d))) return rc;
     if (int rc = dev_alloc(&d->feat, 3 * (size_t)d->n_disk)) return rc;
     if (int rc = dev_alloc(&d->esum, d->n_disk)) return rc;
     if (int rc = dev_alloc(&d->fx, (size_t)RD_MLP_IN * d->n_disk)) return rc;
@@ -290,6 +297,7 @@ int alloc_buffers(rd_disk* d)
     if (int rc = dev_alloc(&d->ev_counter, 1)) return rc;
     HIP_TRY(hipHostMalloc((void**)&d->h_ctl, sizeof(DiskCtl) * std::max<uint32_t>(1, d->n_disk)));
     HIP_TRY(hipHostMalloc((void**)&d->h_err, sizeof(unsigned long long) * std::max<uint32_t>(1, d->n_disk)));
+    HIP_TRY(hipHostMalloc((void**)&d->h_errw, sizeof(unsigned long long) * n_errw(d)));
     HIP_TRY(hipHostMalloc((void**)&d->h_feat, 3 * sizeof(uint32_t) * std::max<uint32_t>(1, d->n_disk)));
     HIP_TRY(hipHostMalloc((void**)&d->h_esum, sizeof(double) * std::max<uint32_t>(1, d->n_disk)));
     HIP_TRY(hipHostMalloc((void**)&d->h_fx, RD_MLP_IN * sizeof(float) * std::max<uint32_t>(1, d->n_disk)));
@@ -602,6 +610,7 @@ StageArgs make_args(rd_disk* d, int r, int adaptive, int err_all)
     a.one_disk = d->n_disk == 1;
     if (a.one_disk && !d->h_info.empty()) a.info0 = d->h_info[0];
     a.trash = d->trash;
+    a.errw = d->errw;
     return a;
 }
 
@@ -1256,11 +1265,19 @@ static int step_disks(rd_integrator* it, const std::vector<uint32_t>& want)
             n_new += need_k0[k];
         }
         it->force_evals += (uint64_t)(last_stage(it)) + (n_new ? 1 : 0);
-        if (it->adaptive) {
+        const size_t nw = 3 * (((size_t)d->n_all + kBodyBlock - 1) / kBodyBlock);
+        if (it->adaptive && nd == 1)   // single disk: one error slot per wave of the last stage
+            HIP_TRY(hipMemcpyAsync(d->h_errw, d->errw, nw * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                   d->stream));
+        else if (it->adaptive)
             HIP_TRY(hipMemcpyAsync(d->h_err, d->err, nd * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                    d->stream));
-        }
         HIP_TRY(hipStreamSynchronize(d->stream));
+        if (it->adaptive && nd == 1) {
+            unsigned long long m = 0;
+            for (size_t w = 0; w < nw; ++w) m = std::max(m, d->h_errw[w]);
+            d->h_err[0] = m;
+        }
         // pair count of this attempt: active disks' full evaluation per stage
         for (uint32_t k = 0; k < nd; ++k) {
             if (!pending[k]) continue;

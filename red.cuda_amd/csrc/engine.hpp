@@ -99,6 +99,7 @@ struct StageArgs {
     int one_disk;                // every body belongs to disk 0
     DiskInfo info0;              // info[0] when one_disk (kernel arguments are scalar loads)
     double* trash;               // [3][kBodyBlock]: stores of a single-disk stage that does not apply
+    unsigned long long* errw;    // single disk: error bits per wave of the last stage, [3][blocks]
 };
 
 // Launchers (kernels.hip).

@@ -842,6 +842,20 @@ __device__ void error_reduce(const StageArgs& a, bool valid, uint32_t d, double 
     }
 }
 
+// Single-disk form of error_reduce: the wave's max goes to its own slot,
+// a.errw[blockIdx.y * gridDim.x + blockIdx.x] (one wave per block), and the
+// host takes the max of the slots.  One same-address atomic per wave would
+// serialise hundreds of waves on one word.
+__device__ void error_store_wave(const StageArgs& a, bool valid, double e)
+{
+    unsigned long long bits = valid ? (unsigned long long)__double_as_longlong(e) : 0ull;
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(bits, off, 64);
+        bits = o > bits ? o : bits;
+    }
+    if ((threadIdx.x & 63u) == 0) a.errw[blockIdx.y * gridDim.x + blockIdx.x] = bits;
+}
+
 // Stage kernels over a single disk (ONE = true).  The disk's info comes in
 // the kernel arguments and its control load goes out with the per-body
 // loads; every value is then computed without a branch, and a stage that
@@ -941,7 +955,7 @@ __device__ __forceinline__ void stage_rk8_one(const StageArgs& a)
         // |f0 + f10 - f11 - f12| (int_rungekutta8.cu:250-260)
         if (valid && (a.err_all || li < di.err_hi))
             e = fmax(fabs(e0v + e10v - e11v - kk), fabs(e0a + e10a - e11a - kRa));
-        error_reduce(a, valid, 0, e);
+        error_store_wave(a, valid, e);
     }
 }
 
@@ -1097,7 +1111,7 @@ __device__ __forceinline__ void stage_rk4_one(const StageArgs& a)
         const uint32_t li = i - di.off;
         double e = 0.0;
         if (valid && (a.err_all || li < di.err_hi)) e = fmax(fabs(k3v - kk), fabs(k3a - kRa));   // |f4 - f5|
-        error_reduce(a, valid, 0, e);
+        error_store_wave(a, valid, e);
     }
 }
 
