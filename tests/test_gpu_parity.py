@@ -131,6 +131,26 @@ def test_adaptive_bitexact_with_rejections(method):
     assert t == ot and bits_equal(rr, orr) and bits_equal(vv, ovv)
 
 
+@pytest.mark.parametrize("method", [R.RD_RKF78, R.RD_RK4])
+def test_adaptive_bitexact_many_waves(method):
+    """The error norm of a single disk is reduced over ~140 waves (one slot per
+    wave, maximum on the host) and stage 0 is skipped on k0 reuse: the step
+    and dt sequence still match the oracle bit for bit."""
+    r, v, p, b, nbt = R.make_disk(3000, 5)
+    d, o = pair(r, v, p, b, nbt)
+    h0 = 40.0 * KDAY                    # too large: rejections first
+    it = R.Integrator(d, method, h0, adaptive=True, tol=1e-11)
+    oi = O.OInteg(o, method, h0, adaptive=True, tol=1e-11)
+    for _ in range(3):
+        assert it.step() == oi.step()
+        assert it.dt_next() == oi.dt_next()
+    assert it.counters() == oi.counters()
+    assert it.counters()[2] > 0
+    t, rr, vv, _, _ = d.state()
+    ot, orr, ovv, _, _ = o.state()
+    assert t == ot and bits_equal(rr, orr) and bits_equal(vv, ovv)
+
+
 @pytest.mark.parametrize("method", [R.RD_EULER, R.RD_RK2])
 def test_euler_rk2_bitexact(method):
     """Euler (int_euler.cu) and midpoint RK2 (int_rungekutta2.cu) against the
