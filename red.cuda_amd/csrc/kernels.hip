@@ -353,12 +353,13 @@ __device__ inline float rot_sub(float acc, float v)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(acc), 0x13C, 0xF, 0xF, false)) - v;
 }
 
-// One wave's sweep of its 64-source quarter: sink sums into f*, reactions
-// (when kReact) into r* (lane l: source (l + 1) mod 64 of the quarter).  The
-// FP32 reaction sums are folded into FP64 every 16 steps (64 terms, as on the
-// sink side): the FP64 running sum is rotated 16 lanes at each fold so that
-// it stays with the sources the FP32 sum in the same lane belongs to.
-template <bool kReact>
+// One wave's sweep of its 64-source quarter over the first kS sink rows
+// (lane l, row s: sink l + 64 s): sink sums into f*, reactions (when kReact)
+// into r* (lane l: source (l + 1) mod 64 of the quarter).  The FP32 reaction
+// sums are folded into FP64 every 16 steps (64 terms, as on the sink side):
+// the FP64 running sum is rotated 16 lanes at each fold so that it stays with
+// the sources the FP32 sum in the same lane belongs to.
+template <bool kReact, int kS>
 __device__ inline void sym_sweep(const double2* __restrict__ txy, const SrcZM* __restrict__ tzm, uint32_t qb,
                                  uint32_t lane, const double* xi, const double* yi, const double* zi,
                                  const float* mi, float* fx, float* fy, float* fz, double& rx, double& ry,
@@ -376,25 +377,25 @@ __device__ inline void sym_sweep(const double2* __restrict__ txy, const SrcZM* _
             const uint32_t k = qb + ((lane - u) & 63u);
             const double2 a = txy[k];
             const SrcZM b = tzm[k];
-            float dx[kSymS], dy[kSymS], dz[kSymS], r2[kSymS], ri3[kSymS];
+            float dx[kS], dy[kS], dz[kS], r2[kS], ri3[kS];
 #pragma unroll
-            for (int s = 0; s < (int)kSymS; ++s) {
+            for (int s = 0; s < kS; ++s) {
                 dx[s] = (float)(a.x - xi[s]);
                 dy[s] = (float)(a.y - yi[s]);
                 dz[s] = (float)(b.z - zi[s]);
             }
 #pragma unroll
-            for (int s = 0; s < (int)kSymS; ++s) r2[s] = __builtin_fmaf(dz[s], dz[s], kEps2);
+            for (int s = 0; s < kS; ++s) r2[s] = __builtin_fmaf(dz[s], dz[s], kEps2);
 #pragma unroll
-            for (int s = 0; s < (int)kSymS; ++s) r2[s] = __builtin_fmaf(dy[s], dy[s], r2[s]);
+            for (int s = 0; s < kS; ++s) r2[s] = __builtin_fmaf(dy[s], dy[s], r2[s]);
 #pragma unroll
-            for (int s = 0; s < (int)kSymS; ++s) r2[s] = __builtin_fmaf(dx[s], dx[s], r2[s]);
+            for (int s = 0; s < kS; ++s) r2[s] = __builtin_fmaf(dx[s], dx[s], r2[s]);
 #pragma unroll
-            for (int s = 0; s < (int)kSymS; ++s) ri3[s] = __builtin_amdgcn_rsqf(r2[s]);
+            for (int s = 0; s < kS; ++s) ri3[s] = __builtin_amdgcn_rsqf(r2[s]);
 #pragma unroll
-            for (int s = 0; s < (int)kSymS; ++s) ri3[s] = (ri3[s] * ri3[s]) * ri3[s];
+            for (int s = 0; s < kS; ++s) ri3[s] = (ri3[s] * ri3[s]) * ri3[s];
 #pragma unroll
-            for (int s = 0; s < (int)kSymS; ++s) {
+            for (int s = 0; s < kS; ++s) {
                 const float scj = b.m * ri3[s];
                 fx[s] = __builtin_fmaf(scj, dx[s], fx[s]);
                 fy[s] = __builtin_fmaf(scj, dy[s], fy[s]);
@@ -403,7 +404,7 @@ __device__ inline void sym_sweep(const double2* __restrict__ txy, const SrcZM* _
             if constexpr (kReact) {
                 float sx = 0.0f, sy = 0.0f, sz = 0.0f;
 #pragma unroll
-                for (int s = 0; s < (int)kSymS; ++s) {
+                for (int s = 0; s < kS; ++s) {
                     const float sci = mi[s] * ri3[s];
                     sx = __builtin_fmaf(sci, dx[s], sx);
                     sy = __builtin_fmaf(sci, dy[s], sy);
@@ -421,6 +422,22 @@ __device__ inline void sym_sweep(const double2* __restrict__ txy, const SrcZM* _
             ry = __shfl(ry, from16) + (double)ay;
             rz = __shfl(rz, from16) + (double)az;
         }
+    }
+}
+
+// sym_sweep over the first nS sink rows: rows past them hold no valid sink
+// (a disk's last block), and skipping them only drops terms that are exactly 0.
+template <bool kReact>
+__device__ inline void sym_sweep_rows(uint32_t nS, const double2* __restrict__ txy, const SrcZM* __restrict__ tzm,
+                                      uint32_t qb, uint32_t lane, const double* xi, const double* yi,
+                                      const double* zi, const float* mi, float* fx, float* fy, float* fz,
+                                      double& rx, double& ry, double& rz)
+{
+    switch (nS) {
+    case 1: sym_sweep<kReact, 1>(txy, tzm, qb, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
+    case 2: sym_sweep<kReact, 2>(txy, tzm, qb, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
+    case 3: sym_sweep<kReact, 3>(txy, tzm, qb, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
+    default: sym_sweep<kReact, kSymS>(txy, tzm, qb, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
     }
 }
 
@@ -456,6 +473,8 @@ __global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict
         for (uint32_t q = tid; q < nJ * 3 * kSymBlock; q += kBlock) acc[q] = 0.0;
         for (uint32_t bi = 0; bi < nI; ++bi) {
             const uint32_t i0 = w.sink_lo + bi * kSymBlock;
+            // sink rows holding at least one valid sink (fewer only in a disk's last block)
+            const uint32_t nS = min(kSymS, (min(kSymBlock, w.sink_hi - i0) + 63u) / 64u);
             double xi[kSymS], yi[kSymS], zi[kSymS], ax[kSymS], ay[kSymS], az[kSymS];
             float mi[kSymS];
 #pragma unroll
@@ -487,12 +506,16 @@ __global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict
                 float fx[kSymS], fy[kSymS], fz[kSymS];
 #pragma unroll
                 for (int s = 0; s < (int)kSymS; ++s) fx[s] = fy[s] = fz[s] = 0.0f;
-                double rx, ry, rz;
+                double rx = 0.0, ry = 0.0, rz = 0.0;
                 const bool self = diag && bj == bi;
-                if (self)
-                    sym_sweep<false>(txy, tzm, wave * 64u, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz);
-                else
-                    sym_sweep<true>(txy, tzm, wave * 64u, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz);
+                // a wave whose quarter holds no source (a disk's last block) has
+                // nothing to add: its sources are padding with mass 0
+                if (j0 + wave * 64u < w.src_hi) {
+                    if (self)
+                        sym_sweep_rows<false>(nS, txy, tzm, wave * 64u, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz);
+                    else
+                        sym_sweep_rows<true>(nS, txy, tzm, wave * 64u, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz);
+                }
 #pragma unroll
                 for (int s = 0; s < (int)kSymS; ++s) {
                     ax[s] += (double)fx[s];
