@@ -279,6 +279,35 @@ def test_batch_matches_single_disks(prec):
         assert bit.counters(k) == sit.counters()
 
 
+def test_config4_batch_matches_single_disks():
+    """Config 4 at its full size: the 512-disk batch (N ~ U[64, 4096], disk k
+    seeded 4000 + k, 1.06 M bodies, mixed precision, general stage kernels)
+    against standalone runs (single-disk stage kernels) of eight of its disks
+    after 3 accepted RKF7(8) steps: times, states and counters bit for bit."""
+    from proposer import batch_sizes
+    sizes = batch_sizes()
+    parts = [R.make_disk(n, 4000 + k) for k, n in enumerate(sizes)]
+    cat = [np.concatenate([q[i] for q in parts]) for i in range(4)]
+    nbt = np.stack([q[4] for q in parts])
+    bd = R.Disk(*cat, nbt, batch=True)
+    bd.set_precision(R.RD_PREC_MIXED)
+    h0, tol = 0.1 * KDAY, 1e-10
+    bit = R.Integrator(bd, R.RD_RKF78, h0, adaptive=True, tol=tol)
+    bit.steps(3)
+    order = np.argsort(sizes)
+    picks = sorted({int(order[0]), int(order[1]), int(order[-1]), int(order[-2]), 0, 137, 256, 511})
+    for k in picks:
+        sd = R.Disk(*parts[k])
+        sd.set_precision(R.RD_PREC_MIXED)
+        sit = R.Integrator(sd, R.RD_RKF78, h0, adaptive=True, tol=tol)
+        sit.steps(3)
+        a, b_ = bd.state(k), sd.state()
+        assert a[0] == b_[0], k
+        assert bits_equal(a[1], b_[1]) and bits_equal(a[2], b_[2]), k
+        assert bit.counters(k) == sit.counters(), k
+        del sit, sd
+
+
 @pytest.mark.parametrize("prec", [R.RD_PREC_FP64, R.RD_PREC_MIXED])
 def test_batch_events_match_single_disks(prec):
     """Batched driver loop (one launch per stage over all disks, batch-wide
