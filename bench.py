@@ -17,7 +17,7 @@ Prints one JSON line:
                 launch duration, against the 157.3 TFLOP/s FP32 peak
   cpu_baseline  the oracle's restatement of the reference CPU path
                 (symmetric FP64 loop, pp_disk.cu:822-865) on one core, timed
-                over one RKF7(8) step of the same disk
+                over the first RKF7(8) steps of the same disk (at least 10 s)
 """
 import argparse
 import json
@@ -145,17 +145,19 @@ def profiled_traffic(kernel, workload_file, tuning):
             text = f.read()
     except OSError:
         return None
-    m = re.search(r"--tuning (\d+),(\d+),(\d+)(?:,(\d+))?", text)
+    m = re.search(r"--tuning (\d+),(\d+),(\d+)(?:,(\d+))?(?:,(\d+))?", text)
     t = re.search(r"Traffic per launch: ([0-9.e+]+) B", text)
     if not m or not t or kernel not in text:
         return None
     shape = (int(m[1]), int(m[2]), int(m[3]), int(m[4]) if m[4] else R.RD_PAIRS_ONE_SIDED)
+    chunk_min = int(m[5]) if m[5] else 1
     if tuning is None:
         return None
     if kernel == "k_force_sym":
-        # symmetric items and slots depend on the disk alone: S, chunk and the
-        # grid size change neither (the workload has no one-sided pairs)
-        if shape[3] != tuning.pair_mode:
+        # symmetric items and slots depend on the disk and the minimum chunk
+        # alone: S, chunk and the grid size change neither (the workload has
+        # no one-sided pairs)
+        if shape[3] != tuning.pair_mode or chunk_min != max(1, tuning.sym_chunk_min):
             return None
     elif shape != (tuning.sinks_per_lane, tuning.chunk, tuning.blocks_per_cu, tuning.pair_mode):
         return None
@@ -171,14 +173,16 @@ def pair_kernel(precision, tuning):
 
 def tuning_json(t, **extra):
     return {"sinks_per_lane": t.sinks_per_lane, "chunk": t.chunk, "blocks_per_cu": t.blocks_per_cu,
-            "pairs": "symmetric" if t.pair_mode == R.RD_PAIRS_SYMMETRIC else "one-sided", **extra}
+            "pairs": "symmetric" if t.pair_mode == R.RD_PAIRS_SYMMETRIC else "one-sided",
+            "sym_chunk_min": t.sym_chunk_min, **extra}
 
 
 def set_tuning_arg(d, text):
-    """--tuning S,CHUNK,BLOCKS_PER_CU[,PAIR_MODE] (pair mode 1 = symmetric, 0 = one-sided)."""
+    """--tuning S,CHUNK,BLOCKS_PER_CU[,PAIR_MODE[,SYM_CHUNK_MIN]] (pair mode 1 = symmetric, 0 = one-sided)."""
     f = [int(x) for x in text.split(",")]
     d.set_tuning(sinks_per_lane=f[0], chunk=f[1], blocks_per_cu=f[2],
-                 pair_mode=f[3] if len(f) > 3 else R.RD_PAIRS_SYMMETRIC)
+                 pair_mode=f[3] if len(f) > 3 else R.RD_PAIRS_SYMMETRIC,
+                 sym_chunk_min=f[4] if len(f) > 4 else 1)
     return d.tuning()
 
 
@@ -396,8 +400,8 @@ def main():
                     help="config 2: skip the final-state comparison against FP64 (for profiling runs)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch the attempt's kernels directly (no hipGraph)")
-    ap.add_argument("--tuning", default="", help="S,CHUNK,BLOCKS_PER_CU[,PAIR_MODE] for the mixed kernel "
-                                                 "(skips autotune; pair mode 1 symmetric, 0 one-sided)")
+    ap.add_argument("--tuning", default="", help="S,CHUNK,BLOCKS_PER_CU[,PAIR_MODE[,SYM_CHUNK_MIN]] for the mixed "
+                                                 "kernel (skips autotune; pair mode 1 symmetric, 0 one-sided)")
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5],
                     help="1 RK4 N=256, 2 RKF78 N=16384 (default), 3 gas+collisions N=131072, "
                          "4 batch of 512 disks, 5 learned step proposer")

@@ -100,6 +100,9 @@ typedef struct rd_tuning {
     uint32_t blocks_per_cu;    /* persistent grid size = blocks_per_cu * CUs      */
     uint32_t use_graph;        /* capture one RK attempt in a hipGraph            */
     uint32_t pair_mode;        /* mixed kernel, SI x SI pairs: RD_PAIRS_*         */
+    uint32_t sym_chunk_min;    /* symmetric kernel: 256-body blocks per chunk at  */
+                               /* least (1..8; 0 reads as 1).  Fewer partial-sum  */
+                               /* slots per body, fewer and larger work items.    */
 } rd_tuning;
 
 /* Mixed precision: RD_PAIRS_SYMMETRIC evaluates each SI x SI pair once and

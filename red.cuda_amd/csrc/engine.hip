@@ -181,7 +181,7 @@ struct rd_disk {
 
     int precision = RD_PREC_FP64;
     int drag_mode = RD_DRAG_NSI;
-    rd_tuning tuning{2, 1024, 4, 0, 1};
+    rd_tuning tuning{2, 1024, 4, 0, 1, 1};
     double thr[3] = {0, 0, 0};
     GasDev gas{};
     uint64_t generation = 0;       // bumps when buffers / lists change (invalidates graphs)
@@ -463,8 +463,10 @@ int build_lists(rd_disk* d)
                 if (sym && cls == 0) {
                     const uint32_t nblk = (b[1] + kSymBlockBodies - 1) / kSymBlockBodies;
                     // kSymChunks..2 kSymChunks - 1 chunks once the disk is that large:
-                    // enough items to fill the device, few partial-sum slots
-                    const uint32_t R = std::min<uint32_t>(kSymMaxR, std::max<uint32_t>(1, nblk / kSymChunks));
+                    // enough items to fill the device, few partial-sum slots; at least
+                    // sym_chunk_min blocks (many small disks fill the device by themselves)
+                    const uint32_t rmin = std::max<uint32_t>(1, d->tuning.sym_chunk_min);
+                    const uint32_t R = std::min<uint32_t>(kSymMaxR, std::max<uint32_t>(rmin, nblk / kSymChunks));
                     const uint32_t nc = (nblk + R - 1) / R, span = R * kSymBlockBodies;
                     d->sym_R = std::max(d->sym_R, R);
                     for (uint32_t a = 0; a < nc; ++a)
@@ -838,7 +840,9 @@ int rd_disk_set_tuning(rd_disk* d, const rd_tuning* t)
     if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return fail(RD_EINVAL, "blocks_per_cu must be in [1, 64]");
     if (t->pair_mode != RD_PAIRS_ONE_SIDED && t->pair_mode != RD_PAIRS_SYMMETRIC)
         return fail(RD_EINVAL, "pair_mode must be RD_PAIRS_ONE_SIDED or RD_PAIRS_SYMMETRIC");
+    if (t->sym_chunk_min > kSymMaxR) return fail(RD_EINVAL, "sym_chunk_min must be in [0, 8]");
     d->tuning = *t;
+    if (d->tuning.sym_chunk_min == 0) d->tuning.sym_chunk_min = 1;
     d->lists_valid = false;
     return build_lists(d);
 }
@@ -1044,9 +1048,11 @@ int rd_autotune(rd_disk* d, rd_tuning* best)
     const uint32_t B_opts[] = {1, 2, 4, 8};
     uint32_t max_n = 0;
     bool one_sided_rest = false;   // pairs the symmetric kernel leaves to the one-sided one
+    bool small_disks = false;      // a disk whose chunking sym_chunk_min changes
     for (auto& h : d->hd) {
         max_n = std::max(max_n, h.n);
         one_sided_rest |= h.n_massive() > h.n_si() || h.n > h.n_massive();
+        small_disks |= (h.n_si() + kSymBlockBodies - 1) / kSymBlockBodies < 2 * kSymChunks && h.n_si() > kSymBlockBodies;
     }
     auto trial = [&](const rd_tuning& t) -> int {
         d->tuning = t;
@@ -1063,14 +1069,17 @@ int rd_autotune(rd_disk* d, rd_tuning* best)
                 // with every pair symmetric, S and C shape nothing: one pass
                 if (mode == RD_PAIRS_SYMMETRIC && !one_sided_rest && (S != keep.sinks_per_lane || C != keep.chunk))
                     continue;
-                for (uint32_t B : B_opts) {
-                    rd_tuning t = keep;
-                    t.sinks_per_lane = S;
-                    t.chunk = C;
-                    t.blocks_per_cu = B;
-                    t.pair_mode = mode;
-                    if (int rc = trial(t)) return rc;
-                }
+                for (uint32_t B : B_opts)
+                    for (uint32_t M : {1u, 2u}) {
+                        if (M != 1 && (mode != RD_PAIRS_SYMMETRIC || !small_disks)) continue;
+                        rd_tuning t = keep;
+                        t.sinks_per_lane = S;
+                        t.chunk = C;
+                        t.blocks_per_cu = B;
+                        t.pair_mode = mode;
+                        t.sym_chunk_min = M;
+                        if (int rc = trial(t)) return rc;
+                    }
             }
     d->tuning = win;
     if (int rc = build_lists(d)) return rc;

@@ -215,7 +215,8 @@ class Disk:
         check(lib.rd_disk_get_tuning(self.h, C.byref(t)))
         return t
 
-    def set_tuning(self, sinks_per_lane=None, chunk=None, blocks_per_cu=None, use_graph=None, pair_mode=None):
+    def set_tuning(self, sinks_per_lane=None, chunk=None, blocks_per_cu=None, use_graph=None, pair_mode=None,
+                   sym_chunk_min=None):
         t = self.tuning()
         if sinks_per_lane is not None:
             t.sinks_per_lane = sinks_per_lane
@@ -227,6 +228,8 @@ class Disk:
             t.use_graph = use_graph
         if pair_mode is not None:
             t.pair_mode = pair_mode
+        if sym_chunk_min is not None:
+            t.sym_chunk_min = sym_chunk_min
         check(lib.rd_disk_set_tuning(self.h, C.byref(t)))
 
     def autotune(self):

@@ -53,7 +53,8 @@ class GasParams(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("sinks_per_lane", C.c_uint32), ("chunk", C.c_uint32),
-                ("blocks_per_cu", C.c_uint32), ("use_graph", C.c_uint32), ("pair_mode", C.c_uint32)]
+                ("blocks_per_cu", C.c_uint32), ("use_graph", C.c_uint32), ("pair_mode", C.c_uint32),
+                ("sym_chunk_min", C.c_uint32)]
 
 
 class DiskSpec(C.Structure):

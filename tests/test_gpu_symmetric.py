@@ -93,3 +93,36 @@ def test_symmetric_batch_matches_single_disks():
         a = bd.calc_dydx(1, 0, 0.0, r, v, k=k)
         batch = np.stack([a["x"], a["y"], a["z"]], 1)
         assert np.array_equal(batch.view(np.uint64), single.view(np.uint64)), k
+
+
+def test_chunk_min_bounds_and_batch_identity():
+    # sym_chunk_min = 2: small disks in chunks of two 256-body blocks (fewer
+    # partial-sum slots, items of up to 4 tiles, reactions summed in LDS).
+    # Within the FP32 bound of the FP64 kernel, and a batch with that tuning
+    # is bit-identical to standalone disks with that tuning.
+    sizes = [(700, 51), (5000, 52), (129, 53), (2000, 54)]
+    parts = [R.make_disk(n, s) for n, s in sizes]
+    cat = [np.concatenate([q[i] for q in parts]) for i in range(4)]
+    bd = R.Disk(*cat, np.stack([q[4] for q in parts]), batch=True)
+    bd.set_precision(R.RD_PREC_MIXED)
+    bd.set_tuning(pair_mode=R.RD_PAIRS_SYMMETRIC, sym_chunk_min=2)
+    assert bd.tuning().sym_chunk_min == 2
+    for k, (r, v, p, b, nbt) in enumerate(parts):
+        ref, mut = fp64_and_mutual(r, v, p, b, nbt)
+        d = R.Disk(r, v, p, b, nbt)
+        d.set_precision(R.RD_PREC_MIXED)
+        d.set_tuning(pair_mode=R.RD_PAIRS_SYMMETRIC, sym_chunk_min=2)
+        single = acc(d, r, v)
+        check_bounds(single, ref, mut)
+        a = bd.calc_dydx(1, 0, 0.0, r, v, k=k)
+        batch = np.stack([a["x"], a["y"], a["z"]], 1)
+        assert np.array_equal(batch.view(np.uint64), single.view(np.uint64)), k
+
+
+def test_chunk_min_rejects_out_of_range():
+    r, v, p, b, nbt = R.make_disk(300, 55)
+    d = R.Disk(r, v, p, b, nbt)
+    with pytest.raises(R.RedError):
+        d.set_tuning(sym_chunk_min=9)
+    d.set_tuning(sym_chunk_min=0)          # 0 reads as 1
+    assert d.tuning().sym_chunk_min == 1
