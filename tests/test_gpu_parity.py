@@ -258,20 +258,25 @@ def test_ejection_and_hit_centrum():
     assert bits_equal(rr, orr) and bits_equal(vv, ovv) and bits_equal(pp, opp) and bits_equal(bb, obb)
 
 
+@pytest.mark.parametrize("method", [R.RD_RKF78, R.RD_RK4, R.RD_RK2, R.RD_EULER])
 @pytest.mark.parametrize("prec", [R.RD_PREC_FP64, R.RD_PREC_MIXED])
-def test_batch_matches_single_disks(prec):
+def test_batch_matches_single_disks(prec, method):
+    """Batch (general stage kernels) against standalone disks (single-disk
+    stage kernels), every integrator, both precisions."""
     sizes = [64, 300, 1000]
     parts = [R.make_disk(n - 1, 100 + k) for k, n in enumerate(sizes)]
     cat = [np.concatenate([q[i] for q in parts]) for i in range(4)]
     nbt = np.stack([q[4] for q in parts])
     bd = R.Disk(*cat, nbt, batch=True)
     bd.set_precision(prec)
-    bit = R.Integrator(bd, R.RD_RKF78, 3 * KDAY, adaptive=True, tol=1e-11)
+    adaptive = method in (R.RD_RKF78, R.RD_RK4)
+    h = 3 * KDAY if adaptive else 0.2 * KDAY
+    bit = R.Integrator(bd, method, h, adaptive=adaptive, tol=1e-11)
     bit.steps(4)
     for k, q in enumerate(parts):
         sd = R.Disk(*q)
         sd.set_precision(prec)
-        sit = R.Integrator(sd, R.RD_RKF78, 3 * KDAY, adaptive=True, tol=1e-11)
+        sit = R.Integrator(sd, method, h, adaptive=adaptive, tol=1e-11)
         sit.steps(4)
         a, b_ = bd.state(k), sd.state()
         assert a[0] == b_[0]
