@@ -22,7 +22,8 @@
 extern "C" {
 #endif
 
-#define RD_ABI_VERSION 1
+/* 2: rd_tuning gained sym_chunk_min */
+#define RD_ABI_VERSION 2
 
 /* Status codes.  The reference throws std::string (red.type/red_macro.h:37-95,
  * src/red.cuda/pp_disk.cu:688, 974-982); here each throw site maps to a code. */

@@ -44,6 +44,10 @@ class pp_disk {
             const std::vector<rd_body_md>& bmd, const uint32_t n_by_type[RD_BODY_TYPE_N], double t0,
             const double threshold[RD_THRESHOLD_N], const rd_gas_params* gas, int id_dev)
     {
+        // the structs of this header must match the library's
+        if (rd_version() != RD_ABI_VERSION)
+            throw std::string("librdamd.so ABI version ") + std::to_string(rd_version()) + ", header " +
+                std::to_string(RD_ABI_VERSION) + ": rebuild";
         rd_check(rd_disk_create((uint32_t)r.size(), n_by_type, r.data(), v.data(), p.data(), bmd.data(), t0, threshold,
                                 gas, id_dev, &h_),
                  "pp_disk");
