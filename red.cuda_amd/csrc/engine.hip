@@ -156,8 +156,12 @@ struct rd_disk {
     uint32_t n_items_force = 0, grid_force = 0;
     WorkItem* items_sym = nullptr;   // mixed precision, symmetric SI x SI pairs
     uint32_t n_items_sym = 0, grid_sym = 0, sym_R = 1;
-    WorkItem* items_coll = nullptr;
-    uint32_t n_items_coll = 0;
+    // collision grid: bucket heads, per-body list links and cells, largest radius
+    uint32_t* coll_head = nullptr;
+    uint32_t* coll_next = nullptr;
+    long long* coll_cell = nullptr;
+    unsigned long long* coll_rmax = nullptr;
+    uint32_t coll_table = 0;
     WorkItem* items_feat = nullptr;
     uint32_t n_items_feat = 0;
     bool exact_lane = false;       // FP64 pair kernel: one sink per lane (see build_lists)
@@ -247,7 +251,11 @@ void free_buffers(rd_disk* d)
     dev_free(d->err);
     dev_free(d->items_force);
     dev_free(d->items_sym);
-    dev_free(d->items_coll);
+    dev_free(d->coll_head);
+    dev_free(d->coll_next);
+    dev_free(d->coll_cell);
+    dev_free(d->coll_rmax);
+    d->coll_table = 0;
     dev_free(d->items_feat);
     dev_free(d->ev_buf);
     dev_free(d->ev_counter);
@@ -295,6 +303,12 @@ int alloc_buffers(rd_disk* d)
     d->ev_cap = std::max<uint32_t>(4096u, (uint32_t)n);
     if (int rc = dev_alloc(&d->ev_buf, 2 * (size_t)d->ev_cap)) return rc;
     if (int rc = dev_alloc(&d->ev_counter, 1)) return rc;
+    d->coll_table = 1024;
+    while (d->coll_table < 2 * (size_t)n) d->coll_table *= 2;
+    if (int rc = dev_alloc(&d->coll_head, d->coll_table)) return rc;
+    if (int rc = dev_alloc(&d->coll_next, n)) return rc;
+    if (int rc = dev_alloc(&d->coll_cell, 3 * n)) return rc;
+    if (int rc = dev_alloc(&d->coll_rmax, 1)) return rc;
     HIP_TRY(hipHostMalloc((void**)&d->h_ctl, sizeof(DiskCtl) * std::max<uint32_t>(1, d->n_disk)));
     HIP_TRY(hipHostMalloc((void**)&d->h_err, sizeof(unsigned long long) * std::max<uint32_t>(1, d->n_disk)));
     HIP_TRY(hipHostMalloc((void**)&d->h_errw, sizeof(unsigned long long) * n_errw(d)));
@@ -495,15 +509,6 @@ int build_lists(rd_disk* d)
         }
     }
     d->pairs_per_eval = pairs;
-    // collision work: upper triangle in (256 sinks) x (4096 sources) tiles
-    std::vector<WorkItem> coll;
-    for (auto& o : order) {
-        const HostDisk& h = d->hd[o.second];
-        for (uint32_t s = 0; s < h.n; s += kBlock)
-            for (uint32_t j = s; j < h.n; j += 4096)
-                coll.push_back(WorkItem{o.second, h.off + s, h.off + std::min(h.n, s + kBlock), h.off + j,
-                                        h.off + std::min(h.n, j + 4096), 0, 0, 0});
-    }
     // proposer-feature work: the pairs j > i in (128 sinks) x (source chunk)
     // tiles, the chunk cut so that the launch has about 16 items per CU
     std::vector<WorkItem> feat;
@@ -534,24 +539,19 @@ int build_lists(rd_disk* d)
     for (auto& q : sym_items) symv.push_back(q.second);
     dev_free(d->items_force);
     dev_free(d->items_sym);
-    dev_free(d->items_coll);
     dev_free(d->items_feat);
     if (int rc = dev_alloc(&d->items_sym, symv.size())) return rc;
     if (int rc = dev_alloc(&d->items_force, items.size())) return rc;
-    if (int rc = dev_alloc(&d->items_coll, coll.size())) return rc;
     if (int rc = dev_alloc(&d->items_feat, feat.size())) return rc;
     d->n_items_force = (uint32_t)items.size();
     d->n_items_sym = (uint32_t)symv.size();
     if (!symv.empty())
         HIP_TRY(hipMemcpy(d->items_sym, symv.data(), symv.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
-    d->n_items_coll = (uint32_t)coll.size();
     d->n_items_feat = (uint32_t)feat.size();
     if (!feat.empty())
         HIP_TRY(hipMemcpy(d->items_feat, feat.data(), feat.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
     if (!items.empty())
         HIP_TRY(hipMemcpy(d->items_force, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
-    if (!coll.empty())
-        HIP_TRY(hipMemcpy(d->items_coll, coll.data(), coll.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
     const uint32_t maxgrid = std::max<uint32_t>(1, d->tuning.blocks_per_cu) * (uint32_t)d->n_cu;
     // the exact kernel's blocks use 26 KB of LDS: 6 fit on a CU
     const uint32_t grid_cap = d->precision == RD_PREC_FP64 && !d->exact_lane ? 6u * (uint32_t)d->n_cu : maxgrid;
@@ -1666,9 +1666,9 @@ int detect_collisions(rd_disk* d, const std::vector<char>& want)
     if (int rc = ensure_lists(d)) return rc;
     const double f = d->thr[RD_THRESHOLD_RADII_ENHANCE_FACTOR];
     if (int rc = flag_check(d, want)) return rc;
-    HIP_TRY(launch_collision(d->stream, d->items_coll, d->n_items_coll,
-                             std::min<uint32_t>(d->n_items_coll, 8u * (uint32_t)d->n_cu), d->x, d->y, d->z,
-                             d->radius, d->id, d->ctl, f * f, d->ev_buf, d->ev_counter, d->ev_cap));
+    HIP_TRY(launch_collision_grid(d->stream, d->n_all, d->x, d->y, d->z, d->radius, d->id, d->disk_of, d->ctl, f,
+                                  d->coll_table, d->coll_head, d->coll_next, d->coll_cell, d->coll_rmax, d->ev_buf,
+                                  d->ev_counter, d->ev_cap));
     std::vector<std::pair<uint32_t, uint32_t>> pr;
     if (int rc = read_pairs(d, pr)) return rc;
     std::sort(pr.begin(), pr.end());

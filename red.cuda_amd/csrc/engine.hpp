@@ -122,10 +122,11 @@ hipError_t launch_comb1(hipStream_t s, int method, const StageArgs& a);
 hipError_t launch_commit(hipStream_t s, const StageArgs& a, int advance_time, DiskCtl* ctl_rw);
 hipError_t launch_operator(hipStream_t s, const StageArgs& a, double t, uint32_t disk, double* ax, double* ay,
                            double* az, const double* vx, const double* vy, const double* vz);
-hipError_t launch_collision(hipStream_t s, const WorkItem* items, uint32_t n_items, uint32_t grid,
-                            const double* x, const double* y, const double* z, const double* radius,
-                            const int32_t* id, const DiskCtl* ctl, double f2, uint32_t* pairs, uint32_t* counter,
-                            uint32_t cap);
+hipError_t launch_collision_grid(hipStream_t s, uint32_t n_all, const double* x, const double* y, const double* z,
+                                 const double* radius, const int32_t* id, const uint32_t* disk_of,
+                                 const DiskCtl* ctl, double f, uint32_t table, uint32_t* head, uint32_t* next,
+                                 long long* cell, unsigned long long* rmax, uint32_t* pairs, uint32_t* counter,
+                                 uint32_t cap);
 hipError_t launch_ejection(hipStream_t s, uint32_t n_all, const double* x, const double* y, const double* z,
                            const int32_t* id, const int32_t* type, const uint32_t* disk_of, const DiskCtl* ctl,
                            double ed2, double hcd2, uint32_t* out, uint32_t* counter, uint32_t cap);

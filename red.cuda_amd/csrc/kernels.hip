@@ -1402,66 +1402,135 @@ hipError_t launch_set_sources(hipStream_t st, uint32_t n_all, const double* mass
 // ---------------------------------------------------------------------------
 // Events
 // ---------------------------------------------------------------------------
-// Collision test d^2 < f^2 (R_i + R_j)^2 for active i < j (pp_disk.cu:321-371;
-// the star takes part as i, as in the reference kernel).  Pairs are appended
-// in any order; the host sorts them into (i, j) order.
-__global__ __launch_bounds__(kBlock) void k_collision(const WorkItem* __restrict__ items, uint32_t n_items,
-                                                      const double* __restrict__ x, const double* __restrict__ y,
-                                                      const double* __restrict__ z,
-                                                      const double* __restrict__ radius,
-                                                      const int32_t* __restrict__ id, const DiskCtl* __restrict__ ctl,
-                                                      double f2, uint32_t* pairs, uint32_t* counter, uint32_t cap)
+// Collision test d^2 < f^2 (R_i + R_j)^2 for active i < j of one disk
+// (pp_disk.cu:321-371; the star takes part as i, as in the reference kernel),
+// on a hashed grid.  A colliding pair is closer than f (R_i + R_j) <= 2 f R_max,
+// so with cells at least that wide the two bodies lie in the same or in
+// neighbouring cells.  The active bodies of the disks flagged F_CHECK are
+// hashed by (disk, cell) into per-bucket linked lists; each body then walks the
+// lists of its 27 neighbouring cells and tests the bodies j > i of its own disk
+// found in those cells (a body is found once: in the list of its own cell).
+// Every pair within reach gets the reference's test, in O(N) rather than the
+// reference's O(N^2).  Pairs are appended in any order; the host sorts them
+// into (i, j) order.
+
+// Largest radius of a checked active body, as the bits of a non-negative
+// double (their unsigned order is the numeric order).
+__global__ __launch_bounds__(kBlock) void k_coll_rmax(uint32_t n_all, const double* __restrict__ radius,
+                                                      const int32_t* __restrict__ id,
+                                                      const uint32_t* __restrict__ disk_of,
+                                                      const DiskCtl* __restrict__ ctl, unsigned long long* rmax)
 {
-    __shared__ double4 tile[kTile];
-    const uint32_t tid = threadIdx.x;
-    // an inactive source's radius is staged as NaN: r2 < f2 (ri + NaN)^2 is false
-    const double no_radius = __longlong_as_double(0x7ff8000000000000LL);
-    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
-        const WorkItem w = items[it];
-        if (!(ctl[w.disk].flags & F_CHECK)) continue;
-        const uint32_t i = w.sink_lo + tid;
-        const bool valid = i < w.sink_hi && id[i] > 0;
-        const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0, zi = valid ? z[i] : 0.0;
-        const double ri = valid ? radius[i] : 0.0;
-        for (uint32_t t0 = w.src_lo; t0 < w.src_hi; t0 += kTile) {
-            __syncthreads();
-            const uint32_t j = t0 + tid;
-            if (j < w.src_hi) tile[tid] = make_double4(x[j], y[j], z[j], id[j] > 0 ? radius[j] : no_radius);
-            __syncthreads();
-            const uint32_t nt = min((uint32_t)kTile, w.src_hi - t0);
-            if (valid) {
-                // sources j > i only: the loop starts past i on a tile that holds it
-                const uint32_t k0 = t0 <= i ? i - t0 + 1 : 0u;
-#pragma unroll 4
-                for (uint32_t k = k0; k < nt; ++k) {
-                    const uint32_t jj = t0 + k;
-                    const double4 s = tile[k];
-                    const double dx = s.x - xi;
-                    const double dy = s.y - yi;
-                    const double dz = s.z - zi;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    double r = 0.0;
+    if (i < n_all && id[i] > 0 && (ctl[disk_of[i]].flags & F_CHECK)) r = fmax(radius[i], 0.0);
+    unsigned long long bits = (unsigned long long)__double_as_longlong(r);
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(bits, off, 64);
+        bits = o > bits ? o : bits;
+    }
+    if ((threadIdx.x & 63u) == 0 && bits) atomicMax(rmax, bits);
+}
+
+// Cell width 2 f R_max, widened by 1/1024 so that the rounding of x / width
+// (below 1e-4 of a cell for |x| / width < 1e12) cannot put two bodies closer
+// than the contact distance two cells apart.  0: no pair can collide.
+__device__ inline double coll_cell_inv(const unsigned long long* rmax, double f)
+{
+    const double w = 2.0 * f * __longlong_as_double((long long)*rmax) * (1.0 + 1.0 / 1024.0);
+    return w > 0.0 ? 1.0 / w : 0.0;
+}
+
+__device__ inline uint32_t coll_hash(uint32_t disk, long long cx, long long cy, long long cz, uint32_t mask)
+{
+    uint64_t h = (uint64_t)cx * 0x9E3779B97F4A7C15ull ^ (uint64_t)cy * 0xC2B2AE3D27D4EB4Full ^
+                 (uint64_t)cz * 0x165667B19E3779F9ull ^ (uint64_t)disk * 0xD6E8FEB86659FD93ull;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 29;
+    return (uint32_t)h & mask;
+}
+
+__global__ __launch_bounds__(kBlock) void k_coll_insert(uint32_t n_all, const double* __restrict__ x,
+                                                        const double* __restrict__ y, const double* __restrict__ z,
+                                                        const int32_t* __restrict__ id,
+                                                        const uint32_t* __restrict__ disk_of,
+                                                        const DiskCtl* __restrict__ ctl,
+                                                        const unsigned long long* rmax, double f, uint32_t mask,
+                                                        uint32_t* head, uint32_t* next, long long* cell)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_all) return;
+    const uint32_t d = disk_of[i];
+    if (!(id[i] > 0 && (ctl[d].flags & F_CHECK))) return;
+    const double inv = coll_cell_inv(rmax, f);
+    if (inv == 0.0) return;
+    const long long cx = (long long)floor(x[i] * inv), cy = (long long)floor(y[i] * inv),
+                    cz = (long long)floor(z[i] * inv);
+    cell[3 * (size_t)i] = cx;
+    cell[3 * (size_t)i + 1] = cy;
+    cell[3 * (size_t)i + 2] = cz;
+    next[i] = atomicExch(&head[coll_hash(d, cx, cy, cz, mask)], i);
+}
+
+__global__ __launch_bounds__(kBlock) void k_coll_query(uint32_t n_all, const double* __restrict__ x,
+                                                       const double* __restrict__ y, const double* __restrict__ z,
+                                                       const double* __restrict__ radius,
+                                                       const int32_t* __restrict__ id,
+                                                       const uint32_t* __restrict__ disk_of,
+                                                       const DiskCtl* __restrict__ ctl,
+                                                       const unsigned long long* rmax, double f, double f2,
+                                                       uint32_t mask, const uint32_t* __restrict__ head,
+                                                       const uint32_t* __restrict__ next,
+                                                       const long long* __restrict__ cell, uint32_t* pairs,
+                                                       uint32_t* counter, uint32_t cap)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_all) return;
+    const uint32_t d = disk_of[i];
+    if (!(id[i] > 0 && (ctl[d].flags & F_CHECK))) return;
+    if (coll_cell_inv(rmax, f) == 0.0) return;
+    const double xi = x[i], yi = y[i], zi = z[i], ri = radius[i];
+    const long long cx = cell[3 * (size_t)i], cy = cell[3 * (size_t)i + 1], cz = cell[3 * (size_t)i + 2];
+    for (int oz = -1; oz <= 1; ++oz)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                const long long nx = cx + ox, ny = cy + oy, nz = cz + oz;
+                for (uint32_t j = head[coll_hash(d, nx, ny, nz, mask)]; j != ~0u; j = next[j]) {
+                    if (j <= i || disk_of[j] != d) continue;
+                    if (cell[3 * (size_t)j] != nx || cell[3 * (size_t)j + 1] != ny || cell[3 * (size_t)j + 2] != nz)
+                        continue;
+                    const double dx = x[j] - xi;
+                    const double dy = y[j] - yi;
+                    const double dz = z[j] - zi;
                     const double r2 = SQR(dx) + SQR(dy) + SQR(dz);
-                    if (r2 < f2 * SQR((ri + s.w))) {
+                    if (r2 < f2 * SQR((ri + radius[j]))) {
                         const uint32_t slot = atomicAdd(counter, 1u);
                         if (slot < cap) {
                             pairs[2 * slot] = i;
-                            pairs[2 * slot + 1] = jj;
+                            pairs[2 * slot + 1] = j;
                         }
                     }
                 }
             }
-        }
-        __syncthreads();
-    }
 }
 
-hipError_t launch_collision(hipStream_t st, const WorkItem* items, uint32_t n_items, uint32_t grid,
-                            const double* x, const double* y, const double* z, const double* radius,
-                            const int32_t* id, const DiskCtl* ctl, double f2, uint32_t* pairs, uint32_t* counter,
-                            uint32_t cap)
+hipError_t launch_collision_grid(hipStream_t st, uint32_t n_all, const double* x, const double* y, const double* z,
+                                 const double* radius, const int32_t* id, const uint32_t* disk_of,
+                                 const DiskCtl* ctl, double f, uint32_t table, uint32_t* head, uint32_t* next,
+                                 long long* cell, unsigned long long* rmax, uint32_t* pairs, uint32_t* counter,
+                                 uint32_t cap)
 {
-    if (n_items == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_collision, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, radius, id, ctl, f2,
-                       pairs, counter, cap);
+    if (n_all == 0) return hipSuccess;
+    if (table == 0 || (table & (table - 1))) return hipErrorInvalidValue;
+    const uint32_t grid = (n_all + kBlock - 1) / kBlock;
+    if (hipError_t e = hipMemsetAsync(rmax, 0, sizeof(unsigned long long), st)) return e;
+    if (hipError_t e = hipMemsetAsync(head, 0xff, (size_t)table * sizeof(uint32_t), st)) return e;
+    hipLaunchKernelGGL(k_coll_rmax, dim3(grid), dim3(kBlock), 0, st, n_all, radius, id, disk_of, ctl, rmax);
+    hipLaunchKernelGGL(k_coll_insert, dim3(grid), dim3(kBlock), 0, st, n_all, x, y, z, id, disk_of, ctl, rmax, f,
+                       table - 1, head, next, cell);
+    hipLaunchKernelGGL(k_coll_query, dim3(grid), dim3(kBlock), 0, st, n_all, x, y, z, radius, id, disk_of, ctl, rmax,
+                       f, f * f, table - 1, head, next, cell, pairs, counter, cap);
     return hipGetLastError();
 }
 

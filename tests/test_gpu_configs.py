@@ -98,3 +98,34 @@ def test_cpp_driver_smoke():
                          capture_output=True, text=True, timeout=120, check=True).stdout
     rec = json.loads(out.strip().splitlines()[-1])
     assert rec["steps"] == 20 and rec["tried"] >= 20 and rec["rel_energy_error"] < 1e-9
+
+
+def test_collision_grid_full_size():
+    """The collision check at config-3 size (N = 131072, radii 1-100 km) with
+    the radius factor raised to 5000 so that hundreds of pairs touch: the
+    engine's pairs (hashed grid) equal an independent search -- a scipy k-d
+    tree over the same FP64 positions, then the reference's test
+    d^2 < f^2 (R_i + R_j)^2 evaluated in the same operation order."""
+    from scipy.spatial import cKDTree
+    r, v, p, b, nbt = R.make_disk(131072, 3, size_mode=1, lo=1.0, hi=100.0, cd=1.0, r_star=1e-5)
+    f = 5000.0
+    d = R.Disk(r, v, p, b, nbt, threshold=(0.0, 0.0, f))
+    n = d.check_collision()
+    ev = d.events()
+    got = {(int(min(a, c)), int(max(a, c))) for a, c in zip(ev["idx1"], ev["idx2"])}
+    assert len(got) == n
+    x = as3(r)
+    rad = p["radius"].astype(np.float64)
+    act = b["id"] > 0
+    reach = 2.0 * f * rad[act].max() * (1.0 + 1e-6)
+    cand = cKDTree(x).query_pairs(reach, output_type="ndarray")
+    i, j = cand.min(1), cand.max(1)
+    keep = act[i] & act[j]
+    i, j = i[keep], j[keep]
+    dx, dy, dz = (x[j, 0] - x[i, 0]), (x[j, 1] - x[i, 1]), (x[j, 2] - x[i, 2])
+    r2 = dx * dx + dy * dy + dz * dz
+    s = rad[i] + rad[j]
+    hit = r2 < (f * f) * (s * s)
+    want = {(int(a), int(c)) for a, c in zip(i[hit], j[hit])}
+    assert len(want) > 100
+    assert got == want
