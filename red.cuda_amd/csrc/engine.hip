@@ -1148,7 +1148,7 @@ static int after_accept(rd_integrator* it, const std::vector<char>& acc, const s
         // that would just have met the tolerance
         if (it->logging && H.has_feat) {
             const double er = std::max(err_ratio[k], 1.0e-12);
-            const double target = std::log(it->dt_did[k]) + std::log(1.0 / er) / 8.0;
+            const double target = std::log(std::fabs(it->dt_did[k])) + std::log(1.0 / er) / 8.0;
             it->log_x.insert(it->log_x.end(), H.feat, H.feat + RD_MLP_IN);
             it->log_t.push_back((float)target);
         }
@@ -1185,14 +1185,16 @@ static int after_accept(rd_integrator* it, const std::vector<char>& acc, const s
             H.ek0_set = true;
         }
         const double tenc = 0.5 * std::log(std::max((double)r2v2, 1e-60));
+        // step sizes enter as magnitudes, so a backward run (dt < 0) sees the
+        // features of the forward one; the sign goes back on the proposal
         float* F = H.feat;
-        F[0] = (float)std::log(H.h1);
+        F[0] = (float)std::log(std::fabs(H.h1));
         F[1] = (float)std::log(std::max(H.e1, 1e-30));
         F[2] = (float)tenc;
         F[3] = (float)std::log(std::max((double)qmin, 1e-30));
         F[4] = (float)std::log((double)d->hd[k].n);
         F[5] = (float)std::log10(std::fabs(es[k] - H.ek0) / std::max(std::fabs(H.ek0), 1e-300) + 1e-20);
-        F[6] = (float)std::log(H.h2);
+        F[6] = (float)std::log(std::fabs(H.h2));
         F[7] = (float)std::log(std::max(H.e2, 1e-30));
         F[8] = (float)H.rej;
         F[9] = (float)std::log(std::fabs(it->dt_next[k]));

@@ -115,6 +115,29 @@ def test_cli_fixed_step_integrators(tmp_path, name, method):
     assert float(open(last.replace(".txt", ".info.txt")).read().split()[0]) == ot
 
 
+def test_cli_backward_run_matches_oracle(tmp_path):
+    # a negative length integrates backward: the start step is negated
+    # (options.cpp:244-248) and the step is clamped to minus the output
+    # interval (main.cu:535-538); collisions and merges still happen
+    r, v, p, b, nbt = _system()
+    dt = 1.0 * R.K_GAUSS
+    params = dict(PARAMS, length="-0.3")
+    d = str(tmp_path)
+    CF.write_run(d, r, v, p, b, nbt, 0.0, dt, params)
+    _run(d)
+    o, steps = _oracle(r, v, p, b, nbt, -dt, params)
+    last = sorted(glob.glob(os.path.join(d, "data_*[0-9].txt")))[-1]
+    _, ids, rr, vv, _ = CF.read_data(last, R.BODY_MD)
+    ot, orr, ovv, _, obb = o.state()
+    assert ot < 0.0
+    act = obb["id"] > 0
+    assert np.array_equal(ids, obb["id"][act])
+    assert np.array_equal(rr, np.stack([orr["x"], orr["y"], orr["z"]], 1)[act])
+    assert np.array_equal(vv, np.stack([ovv["x"], ovv["y"], ovv["z"]], 1)[act])
+    assert float(open(last.replace(".txt", ".info.txt")).read().split()[0]) == ot
+    assert len(o.event_log()) > 0
+
+
 def test_redinit_then_red(tmp_path):
     # red's initial-condition generator writes a run directory that red reads:
     # 10 giant planets (n_gp preset), 2 years of RKF7(8) against the oracle

@@ -166,6 +166,50 @@ def test_proposer_run_is_deterministic_and_holds_tolerance(R):
     assert c1[1] == 40 and dE1 < 1e-9
 
 
+def test_backward_proposer_mirrors_forward(R):
+    """Step sizes enter the features and the logged targets as magnitudes and
+    the proposal takes the sign of the controller's step, so a backward run
+    from (r, v) logs the same samples and takes the mirrored steps of the
+    forward run from (r, -v)."""
+    r, v, p, b, nbt = R.make_disk(1024, 43)
+    vm = v.copy()
+    for c in "xyz":
+        vm[c] = -v[c]
+    h0 = 0.05 * R.K_GAUSS
+
+    def logged(vel, h):
+        d = R.Disk(r, vel, p, b, nbt)
+        d.set_precision(R.RD_PREC_MIXED)
+        it = R.Integrator(d, R.RD_RKF78, h, adaptive=True, tol=1e-10)
+        it.set_logging(True)
+        it.steps(30)
+        return it.log()
+
+    Xf, Tf = logged(vm, h0)
+    Xb, Tb = logged(v, -h0)
+    assert np.isfinite(Xb).all() and np.isfinite(Tb).all()
+    assert np.array_equal(Xf, Xb) and np.array_equal(Tf, Tb)
+    m = R.Mlp(13)
+    m.set_norm(Xf.mean(0), Xf.std(0) + 1e-3, float(Tf.mean()), float(Tf.std()) + 1e-6)
+    m.train(Xf, Tf, epochs=20, batch=32, lr=1e-3)
+
+    def run(vel, h):
+        d = R.Disk(r, vel, p, b, nbt)
+        d.set_precision(R.RD_PREC_MIXED)
+        it = R.Integrator(d, R.RD_RKF78, h, adaptive=True, tol=1e-10)
+        it.set_proposer(m, 0.9)
+        hs = [it.step() for _ in range(20)]
+        return hs, it.counters(), d.state()
+
+    hf, cf, sf = run(vm, h0)
+    hb, cb, sb = run(v, -h0)
+    assert hb == [-h for h in hf] and cf == cb
+    assert sb[0] == -sf[0]
+    for c in "xyz":
+        assert np.array_equal(sf[1][c], sb[1][c])
+        assert np.array_equal(sf[2][c], -sb[2][c])
+
+
 def test_proposer_ablation_gain(R):
     """What the trained network gains over the classical controller alone:
     proposer fitted on 64 seeded disks, run on 64 held-out disks of the same

@@ -151,6 +151,61 @@ def test_adaptive_bitexact_many_waves(method):
     assert t == ot and bits_equal(rr, orr) and bits_equal(vv, ovv)
 
 
+@pytest.mark.parametrize("method", [R.RD_RKF78, R.RD_RK4])
+def test_backward_integration_bitexact(method):
+    """A negative step (options.cpp:244-248 negates dt for a negative
+    simulation length): the error norm is |lambda dt err|
+    (int_rungekutta8.cu:696), so rejections shrink the step and it stays
+    negative; bit for bit with the oracle."""
+    r, v, p, b, nbt = R.make_disk(300, 7)
+    d, o = pair(r, v, p, b, nbt)
+    h0 = -30.0 * KDAY                   # too large: rejections first
+    it = R.Integrator(d, method, h0, adaptive=True, tol=1e-11)
+    oi = O.OInteg(o, method, h0, adaptive=True, tol=1e-11)
+    for _ in range(6):
+        h = it.step()
+        assert h == oi.step() and h < 0.0
+        assert it.dt_next() == oi.dt_next() < 0.0
+    assert it.counters() == oi.counters()
+    assert it.counters()[2] > 0
+    t, rr, vv, _, _ = d.state()
+    ot, orr, ovv, _, _ = o.state()
+    assert t == ot < 0.0 and bits_equal(rr, orr) and bits_equal(vv, ovv)
+
+
+def mirrored(v):
+    m = v.copy()
+    for c in "xyz":
+        m[c] = -v[c]
+    return m
+
+
+@pytest.mark.parametrize("prec", [R.RD_PREC_FP64, R.RD_PREC_MIXED])
+def test_backward_mirrors_forward_full_size(prec):
+    """At N = 16384, where no oracle run is quick: without drag the right-hand
+    side is odd in v, so the backward run from (r, v) is the forward run from
+    (r, -v) with t and v mirrored.  Every operation on the path (the pair
+    kernels, the stage sums, the error norm, the controller) is symmetric
+    under a sign flip, so the mirror holds bit for bit in both precisions."""
+    r, v, p, b, nbt = R.make_disk(16384, 2)
+    fwd = R.Disk(r, mirrored(v), p, b, nbt)
+    bwd = R.Disk(r, v, p, b, nbt)
+    fwd.set_precision(prec)
+    bwd.set_precision(prec)
+    h0 = 20.0 * KDAY
+    fi = R.Integrator(fwd, R.RD_RKF78, h0, adaptive=True, tol=1e-10)
+    bi = R.Integrator(bwd, R.RD_RKF78, -h0, adaptive=True, tol=1e-10)
+    for _ in range(4):
+        hf = fi.step()
+        assert hf > 0.0 and bi.step() == -hf
+        assert bi.dt_next() == -fi.dt_next()
+    assert fi.counters() == bi.counters()
+    tf, rf, vf, _, _ = fwd.state()
+    tb, rb, vb, _, _ = bwd.state()
+    assert tb == -tf
+    assert bits_equal(rf, rb) and bits_equal(vf, mirrored(vb))
+
+
 @pytest.mark.parametrize("method", [R.RD_EULER, R.RD_RK2])
 def test_euler_rk2_bitexact(method):
     """Euler (int_euler.cu) and midpoint RK2 (int_rungekutta2.cu) against the

@@ -201,6 +201,52 @@ def test_rkf78_adaptive_closed_orbit_and_controller():
     assert np.isfinite(E0)
 
 
+@pytest.mark.parametrize("method", [2, 4])
+def test_backward_step_mirrors_forward(method):
+    """dt < 0 (main.cu sets it for a negative simulation length): without drag
+    the right-hand side is odd in v, so a backward run from (r, v) is the
+    forward run from (r, -v) with r, t and v mirrored.  Every operation on the
+    path is symmetric under a sign flip, so the mirror is exact; the error
+    norm takes |lambda dt err| (int_rungekutta8.cu:696) and the controller's
+    next step keeps the sign."""
+    r, v, p, b, nbt = small_system(48, 17)
+    vm = v.copy()
+    for c in "xyz":
+        vm[c] = -v[c]
+    fwd = O.ODisk(r, vm, p, b, nbt)
+    bwd = O.ODisk(r, v, p, b, nbt)
+    h = 0.05
+    fi = O.OInteg(fwd, method, h, adaptive=True, tol=1e-12)
+    bi = O.OInteg(bwd, method, -h, adaptive=True, tol=1e-12)
+    for _ in range(6):
+        hf = fi.step()
+        assert bi.step() == -hf and hf > 0.0
+        assert bi.dt_next() == -fi.dt_next()
+    assert fi.counters() == bi.counters()
+    tf, rf, vf, _, _ = fwd.state()
+    tb, rb, vb, _, _ = bwd.state()
+    assert tb == -tf
+    for c in "xyz":
+        assert np.array_equal(rf[c], rb[c])
+        assert np.array_equal(vf[c], -vb[c])
+
+
+def test_backward_rkf78_retraces_the_orbit():
+    # one period forward then one period back along the same eccentric orbit
+    r, v, p, b, nbt, P = kepler_binary(0.3)
+    d = O.ODisk(r, v, p, b, nbt)
+    it = O.OInteg(d, 4, P / 128, adaptive=False)
+    for _ in range(128):
+        it.step()
+    back = O.OInteg(d, 4, -P / 128, adaptive=False)
+    for _ in range(128):
+        back.step()
+    _, rr, vv, _, _ = d.state()
+    assert abs(d.t) < 1e-12 * P
+    assert np.hypot(rr["x"][1] - r["x"][1], rr["y"][1] - r["y"][1]) < 1e-10
+    assert np.hypot(vv["x"][1] - v["x"][1], vv["y"][1] - v["y"][1]) < 1e-10
+
+
 def test_rkf78_tableau_row_sums():
     # int_rungekutta8.cu:30-48: sum_j a_rj = c_r, sum b = 1
     from fractions import Fraction as F
