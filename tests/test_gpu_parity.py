@@ -315,9 +315,10 @@ def test_ejection_and_hit_centrum():
 
 @pytest.mark.parametrize("method", [R.RD_RKF78, R.RD_RK4, R.RD_RK2, R.RD_EULER])
 @pytest.mark.parametrize("prec", [R.RD_PREC_FP64, R.RD_PREC_MIXED])
-def test_batch_matches_single_disks(prec, method):
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_batch_matches_single_disks(prec, method, sign):
     """Batch (general stage kernels) against standalone disks (single-disk
-    stage kernels), every integrator, both precisions."""
+    stage kernels), every integrator, both precisions, forward and backward."""
     sizes = [64, 300, 1000]
     parts = [R.make_disk(n - 1, 100 + k) for k, n in enumerate(sizes)]
     cat = [np.concatenate([q[i] for q in parts]) for i in range(4)]
@@ -325,7 +326,7 @@ def test_batch_matches_single_disks(prec, method):
     bd = R.Disk(*cat, nbt, batch=True)
     bd.set_precision(prec)
     adaptive = method in (R.RD_RKF78, R.RD_RK4)
-    h = 3 * KDAY if adaptive else 0.2 * KDAY
+    h = sign * (3 * KDAY if adaptive else 0.2 * KDAY)
     bit = R.Integrator(bd, method, h, adaptive=adaptive, tol=1e-11)
     bit.steps(4)
     for k, q in enumerate(parts):
@@ -334,7 +335,7 @@ def test_batch_matches_single_disks(prec, method):
         sit = R.Integrator(sd, method, h, adaptive=adaptive, tol=1e-11)
         sit.steps(4)
         a, b_ = bd.state(k), sd.state()
-        assert a[0] == b_[0]
+        assert a[0] == b_[0] and a[0] * sign > 0.0
         assert bits_equal(a[1], b_[1]) and bits_equal(a[2], b_[2])
         assert bit.counters(k) == sit.counters()
 
