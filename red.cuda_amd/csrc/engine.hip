@@ -428,6 +428,7 @@ void class_bounds(const HostDisk& h, int cls, uint32_t b[4])
 
 int build_lists(rd_disk* d)
 {
+    d->lists_valid = false;   // until this build completes: a failure part-way forces a rebuild
     fill_info(d);
     std::vector<WorkItem> items;
     std::vector<std::pair<double, uint32_t>> order;   // (cost, disk)
@@ -560,6 +561,7 @@ int build_lists(rd_disk* d)
     const size_t need = max_chunks * 3 * (size_t)d->n_all;
     if (need > d->part_len) {
         dev_free(d->part);
+        d->part_len = 0;   // a failed allocation leaves no stale size behind
         if (int rc = dev_alloc(&d->part, need)) return rc;
         d->part_len = need;
         HIP_TRY(hipMemsetAsync(d->part, 0, need * sizeof(double), d->stream));
@@ -1887,6 +1889,7 @@ int rd_calc_integral(rd_disk* d, uint32_t k, rd_integral* out)
     const size_t need = std::max<size_t>(11 * (size_t)h.n, nch * h.n);
     if (need > d->scratch_len) {
         dev_free(d->scratch);
+        d->scratch_len = 0;
         if (int rc = dev_alloc(&d->scratch, need)) return rc;
         d->scratch_len = need;
     }
