@@ -386,6 +386,42 @@ def bench_config5(args):
     emit(out)
 
 
+def energy_drift(args):
+    """Long-horizon energy check of the default disk: RKF7(8) at args.tol in
+    args.precision until args.energy_orbits periods of the disk's inner edge,
+    the relative energy error (rd_calc_integral) at every tenth of the run.
+    Not a throughput figure: the run prints its checkpoints and one JSON line."""
+    r, v, p, b, nbt = R.make_disk(args.n, args.seed)
+    a_min = np.min(np.hypot(r["x"][1:], r["y"][1:]))
+    period = 2 * np.pi * np.sqrt(a_min ** 3 / 1.0)
+    d = R.Disk(r, v, p, b, nbt)
+    d.set_precision(R.RD_PREC_MIXED if args.precision == "mixed" else R.RD_PREC_FP64)
+    d.set_tuning(use_graph=GRAPH)
+    if args.precision == "mixed" and not args.no_autotune:
+        d.autotune()
+        d.set_tuning(use_graph=GRAPH)
+    it = R.Integrator(d, R.RD_RKF78, period / 1000.0, adaptive=True, tol=args.tol)
+    E0 = d.integral()["E"]
+    t0 = time.perf_counter()
+    checkpoints, mark = [], 0.1
+    while True:
+        it.steps(1000)
+        t = d.state()[0]
+        if t >= mark * args.energy_orbits * period:
+            tried, passed, failed = it.counters()
+            cp = {"orbits": t / period, "steps": passed, "attempts": tried,
+                  "rel_energy_error": abs(d.integral()["E"] / E0 - 1.0), "wall_s": time.perf_counter() - t0}
+            checkpoints.append(cp)
+            print(f"# {json.dumps(cp)}", file=sys.stderr, flush=True)
+            while mark * args.energy_orbits * period <= t:
+                mark += 0.1
+        if t >= args.energy_orbits * period:
+            break
+    print(json.dumps({"energy_drift": {"n_bodies": int(len(r)), "seed": args.seed, "tolerance": args.tol,
+                                       "precision": args.precision, "inner_period_days": period / R.K_GAUSS,
+                                       "checkpoints": checkpoints}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -412,6 +448,8 @@ def main():
     ap.add_argument("--c5-batch-length", type=float, default=2.9, help="config 5: held-out batch run length")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run the byte-counter calibration copy of this size and exit")
+    ap.add_argument("--energy-orbits", type=float, default=0.0,
+                    help="only run the default disk for this many inner-edge orbits and report the energy drift")
     args = ap.parse_args()
     global GRAPH
     GRAPH = 0 if args.no_graph else 1
@@ -424,6 +462,8 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one GPU)",
               file=sys.stderr)
         sys.exit(2)
+    if args.energy_orbits > 0.0:
+        return energy_drift(args)
     if args.config == 1:
         return bench_config1(args)
     if args.config == 3:
