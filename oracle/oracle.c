@@ -16,12 +16,20 @@
  *                    closed form, Newton-3 invariant, symmetric ring
  *   gravity (CPU)    pp_disk.cu:822-865      vs GPU formula (1-ulp rounding of
  *                    m/(d d^2) vs m*(1/(d d^2)))
- *   drag             pp_disk.cu:60-155,486   closed forms: decr<1e-6 cut,
- *                    co-moving gas => zero drag, f*|u| scaling
+ *   drag             pp_disk.cu:60-155,486   quadratic drag law 3 C_D rho_g |u| u
+ *                    / (8 R rho_p) with rho_g and v_gas from the disk model in
+ *                    long double (inside / outside the taper, off the
+ *                    midplane); decrease laws; decr<1e-6 cut; the gas
+ *                    parameters integrate over z to the stated Sigma
  *   RK4 / RKF78      int_rungekutta4.cu, int_rungekutta8.cu: order of
- *                    convergence on Kepler; tableau row sums == c
- *   controller       int_rungekutta8.cu:679-708: zero-error doubling case,
- *                    accept/reject counters
+ *                    convergence on Kepler; the oracle's tableau against the
+ *                    published fractions, row sums == c, quadrature order 7
+ *   error estimates  int_rungekutta8.cu:250-260, 693-696: lambda |dt| max|E|
+ *                    equals the true local error of the 7th-order step from an
+ *                    exact Kepler propagation (ratio -> 1, O(dt^8)); RK4's
+ *                    embedded estimate O(dt^4)
+ *   controller       int_rungekutta8.cu:697-708: err/tol settles at 0.9^7
+ *                    (0.9^4 for RK4) on a circular orbit; zero error doubles
  *   events           pp_disk.cu:284-371,1220-1368: constructed collision,
  *                    ejection and hit-centrum scenarios
  *   integrals        tools.cpp:545-700       2-body closed form
@@ -794,6 +802,12 @@ static const int rk8_cols[13][9] = {
     {0, 3, 4, 5, 6, 7}, {0, 3, 4, 5, 6, 7, 8}, {0, 3, 4, 5, 6, 7, 8, 9},
     {0, 5, 6, 7, 8, 9}, {0, 3, 4, 5, 6, 7, 8, 9, 11} };
 static const int rk8_row[13] = { 0, 1, 2, 4, 7, 11, 16, 22, 29, 37, 46, 56, 67 };
+void or_rk8_tableau(double c[13], double a[79], double b[11])
+{
+    memcpy(c, rk8_c, sizeof(rk8_c));
+    memcpy(a, rk8_a, sizeof(rk8_a));
+    memcpy(b, rk8_b, sizeof(rk8_b));
+}
 /* int_rungekutta4.cu:26-31 */
 static const double rk4_a[8] = { 0.0, 1.0/2.0, 1.0/2.0, 1.0, 1.0/6.0, 1.0/3.0, 1.0/3.0, 1.0/6.0 };
 static const double rk4_b[5] = { 1.0/6.0, 1.0/3.0, 1.0/3.0, 1.0/6.0 - 1.0/10.0, 1.0/10.0 };
@@ -802,7 +816,7 @@ static const double rk4_c[5] = { 0.0, 1.0/2.0, 1.0/2.0, 1.0, 1.0 };
 struct or_integ {
     or_disk* d;
     int method, adaptive, n_stage, order, error_check_for_tp;
-    double tol, dt_try, dt_did, dt_next;
+    double tol, dt_try, dt_did, dt_next, last_err;
     uint64_t tried, passed, failed;
     or_var4* k[2][13];
     or_var4* ytemp[2];
@@ -852,6 +866,22 @@ void or_set_error_check_for_tp(or_integ* it, int on) { it->error_check_for_tp = 
 
 static inline double* comp(or_var4* a) { return (double*)a; }
 static inline const double* ccomp(const or_var4* a) { return (const double*)a; }
+double or_last_error(const or_integ* it) { return it->last_err; }
+uint32_t or_last_error_terms(const or_integ* it, double* out, uint32_t cap)
+{
+    if (!it->adaptive || (it->method != 4 && it->method != 2)) return 0;
+    const uint32_t nv = 4 * it->d->n;
+    for (int i = 0; i < 2; i++)
+        for (uint32_t t = 0; t < nv && i * nv + t < cap; t++) {
+            double e;
+            if (it->method == 4)
+                e = fabs(comp(it->k[i][0])[t] + comp(it->k[i][10])[t] - comp(it->k[i][11])[t] - comp(it->k[i][12])[t]);
+            else
+                e = fabs(comp(it->k[i][3])[t] - comp(it->k[i][4])[t]);
+            out[i * nv + t] = e;
+        }
+    return 2 * nv;
+}
 
 /* RKF78 ytemp for stage r: ytemp = y_n + dt * (sum_j a_rj f_j), left to right
  * over the nonzero columns (int_rungekutta8.cu:82-246) */
@@ -937,6 +967,7 @@ static int rk8_step(or_integ* it, double* dt_did)
             uint32_t nmass = n_si_of(d->playing) + n_nsi_of(d->playing);
             uint32_t nv = 4 * (it->error_check_for_tp ? d->n : nmass);
             max_err = fabs(41.0 / 840.0 * it->dt_try * rk8_max_err(it, nv));
+            it->last_err = max_err;
             if (0.0 < max_err) it->dt_next = it->dt_try *= 0.9 * pow(it->tol / max_err, 1.0 / (it->order));
             else it->dt_next = it->dt_try *= 2.0;
         }
@@ -991,6 +1022,7 @@ static int rk4_step(or_integ* it, double* dt_did)
                 for (uint32_t t = 0; t < nv; t++) { double e = fabs(f4[t] - f5[t]); if (m < e) m = e; }
             }
             max_err = fabs(1.0 / 10.0 * it->dt_try * m);
+            it->last_err = max_err;
             if (0.0 < max_err) it->dt_next = it->dt_try *= 0.9 * pow(it->tol / max_err, 1.0 / (it->order));
             else it->dt_next = it->dt_try *= 2.0;
         }

@@ -149,6 +149,18 @@ double    or_get_dt_next(const or_integ* it);
 void      or_set_dt_next(or_integ* it, double dt);
 void      or_counters(const or_integ* it, uint64_t out[3]);   /* tried, passed, failed */
 void      or_set_error_check_for_tp(or_integ* it, int on);
+/* The controller's error of the last attempt, |lambda * dt * max_i |E_i||
+ * (int_rungekutta8.cu:693-696, int_rungekutta4.cu:226-229), and the terms
+ * |E_i| it is the maximum of, for every component of every body (positions
+ * then velocities, 4 doubles per body): E = k0 + k10 - k11 - k12 for RKF7(8),
+ * E = k3 - k4 for adaptive RK4.  Returns the number of terms (8 n), or 0 for
+ * a method without an error estimate.  Test hooks: they read what step() kept. */
+double    or_last_error(const or_integ* it);
+/* The Fehlberg 7(8) tableau the RKF7(8) step uses: c[13], the packed lower
+ * triangle a[79] (a[0] = 0, then row r >= 1 holds its r entries from index
+ * 1 + r (r - 1) / 2) and the 7th-order b[11]. */
+void      or_rk8_tableau(double c[13], double a[79], double b[11]);
+uint32_t  or_last_error_terms(const or_integ* it, double* out, uint32_t cap);
 
 /* driver loop semantics (src/red.cuda/main.cu:470-625) without file output */
 int or_run(or_disk* d, or_integ* it, double length, double output_interval, uint64_t max_steps, uint64_t* steps_done);

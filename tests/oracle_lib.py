@@ -79,6 +79,9 @@ _SIGS = {
     "or_set_dt_next": (None, [_P, _f64]),
     "or_counters": (None, [_P, _P]),
     "or_set_error_check_for_tp": (None, [_P, C.c_int]),
+    "or_last_error": (_f64, [_P]),
+    "or_rk8_tableau": (None, [_P, _P, _P]),
+    "or_last_error_terms": (_u32, [_P, _P, _u32]),
     "or_run": (C.c_int, [_P, _P, _f64, _f64, _u64, C.POINTER(_u64)]),
 }
 for _n, (_r, _a) in _SIGS.items():
@@ -182,6 +185,13 @@ class OInteg:
 
     def dt_next(self):
         return lib.or_get_dt_next(self.h)
+
+    def last_error(self):
+        """(lambda |dt| max|E|, |E| per component: positions then velocities)."""
+        n = 8 * lib.or_disk_n(self.disk.h)
+        terms = np.zeros(n, np.float64)
+        got = lib.or_last_error_terms(self.h, ptr(terms), n)
+        return lib.or_last_error(self.h), terms[:got]
 
     def run(self, length, output_interval, max_steps):
         steps = _u64()

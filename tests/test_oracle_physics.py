@@ -248,7 +248,10 @@ def test_backward_rkf78_retraces_the_orbit():
 
 
 def test_rkf78_tableau_row_sums():
-    # int_rungekutta8.cu:30-48: sum_j a_rj = c_r, sum b = 1
+    """The oracle's own Fehlberg 7(8) table (int_rungekutta8.cu:26-48) against
+    the published fractions (Fehlberg, NASA TR R-287), plus what the method
+    fixes on it: sum_j a_rj = c_r, and the quadrature conditions
+    sum_i b_i c_i^k = 1/(k+1) for k = 0..7 of a 7th-order formula."""
     from fractions import Fraction as F
     rows = {1: [F(2, 27)], 2: [F(1, 36), F(1, 12)], 3: [F(1, 24), 0, F(1, 8)],
             4: [F(5, 12), 0, F(-25, 16), F(25, 16)], 5: [F(1, 20), 0, 0, F(1, 4), F(1, 5)],
@@ -261,8 +264,19 @@ def test_rkf78_tableau_row_sums():
             11: [F(3, 205), 0, 0, 0, 0, F(-6, 41), F(-3, 205), F(-3, 41), F(3, 41), F(6, 41), 0],
             12: [F(-1777, 4100), 0, 0, F(-341, 164), F(4496, 1025), F(-289, 82), F(2193, 4100), F(51, 82),
                  F(33, 164), F(12, 41), 0, 1]}
-    c = [0, F(2, 27), F(1, 9), F(1, 6), F(5, 12), F(1, 2), F(5, 6), F(1, 6), F(2, 3), F(1, 3), 1, 0, 1]
-    for rr, row in rows.items():
-        assert sum(row) == c[rr]
-    b = [F(41, 840), 0, 0, 0, 0, F(34, 105), F(9, 35), F(9, 35), F(9, 280), F(9, 280), F(41, 840)]
-    assert sum(b) == 1
+    cp = [0, F(2, 27), F(1, 9), F(1, 6), F(5, 12), F(1, 2), F(5, 6), F(1, 6), F(2, 3), F(1, 3), 1, 0, 1]
+    bp = [F(41, 840), 0, 0, 0, 0, F(34, 105), F(9, 35), F(9, 35), F(9, 280), F(9, 280), F(41, 840)]
+    c, a, b = np.zeros(13), np.zeros(79), np.zeros(11)
+    O.lib.or_rk8_tableau(O.ptr(c), O.ptr(a), O.ptr(b))
+    assert list(c) == [float(x) for x in cp]
+    assert list(b) == [float(x) for x in bp]
+    for r in range(1, 13):
+        row = a[1 + r * (r - 1) // 2: 1 + r * (r - 1) // 2 + r]
+        assert list(row) == [float(x) for x in rows[r]], r
+        assert sum(rows[r]) == cp[r]
+        assert abs(row.sum() - c[r]) <= 4e-16 * max(1.0, np.abs(row).sum()), r
+    for k in range(8):
+        assert sum(x * y ** k for x, y in zip(bp, cp)) == F(1, k + 1), k
+        assert abs((b * c[:11] ** k).sum() - 1.0 / (k + 1)) < 1e-15, k
+    # ... and not k = 8: the 7th-order weights are not 8th order
+    assert sum(x * y ** 8 for x, y in zip(bp, cp)) != F(1, 9)
