@@ -195,6 +195,20 @@ int rd_set_dt_next(rd_integrator* it, uint32_t k, double dt);
 int rd_counters(rd_integrator* it, uint32_t k, uint64_t* tried, uint64_t* passed, uint64_t* failed);
 /* integrator::error_check_for_tp (integrator.h:32) */
 int rd_set_error_check_for_tp(rd_integrator* it, int on);
+/* Opt-in bound on the adaptive retry loop.  red.cuda's loop runs until
+ * err <= tol (int_rungekutta8.cu:645-709); src/red_v2/int_rungekutta7.cu:
+ * 176-185 bounds it by an iteration count and a minimum step.  A step fails
+ * when it is still rejected after max_attempts attempts, or when an attempt
+ * would use |dt| <= min_dt (the attempt is not made; this also stops a run
+ * whose step has collapsed to 0, which the reference accepts over and over:
+ * zero step, zero error), or (nonfinite != 0) when an error term is NaN or
+ * infinite -- under the reference's max (integrator.cu:248-263) a NaN term is
+ * never the maximum, so a NaN state is accepted with its step doubled.  0
+ * turns each bound off; all off is the default.  A failing disk keeps y_n and
+ * counts the attempts it made as failed; the other disks of a batch finish
+ * their step, then rd_step / rd_steps / rd_run return RD_ESTATE naming the
+ * first failing disk. */
+int rd_integrator_set_guard(rd_integrator* it, uint32_t max_attempts, double min_dt, int nonfinite);
 /* Number of pair-kernel launches and the ordered pairs they summed so far. */
 int rd_work_counters(rd_integrator* it, uint64_t* force_evals, double* pairs);
 

@@ -818,6 +818,9 @@ struct or_integ {
     int method, adaptive, n_stage, order, error_check_for_tp;
     double tol, dt_try, dt_did, dt_next, last_err;
     uint64_t tried, passed, failed;
+    uint32_t max_attempts;
+    double min_dt;
+    int nonfinite, last_bad;
     or_var4* k[2][13];
     or_var4* ytemp[2];
     uint32_t cap;
@@ -867,6 +870,26 @@ void or_set_error_check_for_tp(or_integ* it, int on) { it->error_check_for_tp = 
 static inline double* comp(or_var4* a) { return (double*)a; }
 static inline const double* ccomp(const or_var4* a) { return (const double*)a; }
 double or_last_error(const or_integ* it) { return it->last_err; }
+void or_set_guard(or_integ* it, uint32_t max_attempts, double min_dt, int nonfinite)
+{
+    it->max_attempts = max_attempts;
+    it->min_dt = min_dt;
+    it->nonfinite = nonfinite;
+}
+/* the guard's verdict on the attempt just judged (iter counts it) */
+static int guard_trips(const or_integ* it, double max_err, int iter)
+{
+    if (it->nonfinite && (it->last_bad || !isfinite(max_err))) return 1;
+    if (max_err > it->tol && it->max_attempts && (uint32_t)iter >= it->max_attempts) return 1;
+    return 0;
+}
+/* the guard's verdict on the attempt about to be made (iter made so far) */
+static int guard_refuses_dt(or_integ* it, int iter)
+{
+    if (!(it->adaptive && 0.0 < it->min_dt && fabs(it->dt_try) <= it->min_dt)) return 0;
+    it->tried += iter; it->failed += iter;
+    return 1;
+}
 uint32_t or_last_error_terms(const or_integ* it, double* out, uint32_t cap)
 {
     if (!it->adaptive || (it->method != 4 && it->method != 2)) return 0;
@@ -918,11 +941,13 @@ static void rk8_ynp1(or_integ* it, uint32_t nv)
 static double rk8_max_err(or_integ* it, uint32_t nv)
 {
     double m[2] = {0.0, 0.0};
+    it->last_bad = 0;
     for (int i = 0; i < 2; i++) {
         const double *f0 = comp(it->k[i][0]), *f10 = comp(it->k[i][10]), *f11 = comp(it->k[i][11]), *f12 = comp(it->k[i][12]);
         for (uint32_t t = 0; t < nv; t++) {
             double e = fabs(f0[t] + f10[t] - f11[t] - f12[t]);
-            if (m[i] < e) m[i] = e;
+            if (m[i] < e) m[i] = e;          /* a NaN term never compares greater */
+            if (!(e <= DBL_MAX)) it->last_bad = 1;
         }
     }
     return m[0] > m[1] ? m[0] : m[1];
@@ -951,6 +976,7 @@ static int rk8_step(or_integ* it, double* dt_did)
     double max_err = 0.0;
     int iter = 0;
     do {
+        if (guard_refuses_dt(it, iter)) return OR_ESTATE;
         it->dt_did = it->dt_try;
         for (int r = 1; r <= 10; r++) {
             ttemp = d->t + rk8_c[r] * it->dt_try;
@@ -970,6 +996,10 @@ static int rk8_step(or_integ* it, double* dt_did)
             it->last_err = max_err;
             if (0.0 < max_err) it->dt_next = it->dt_try *= 0.9 * pow(it->tol / max_err, 1.0 / (it->order));
             else it->dt_next = it->dt_try *= 2.0;
+            if (guard_trips(it, max_err, iter + 1)) {
+                it->tried += iter + 1; it->failed += iter + 1;
+                return OR_ESTATE;
+            }
         }
         iter++;
     } while (it->adaptive && max_err > it->tol);
@@ -992,6 +1022,7 @@ static int rk4_step(or_integ* it, double* dt_did)
     double max_err = 0.0;
     int iter = 0;
     do {
+        if (guard_refuses_dt(it, iter)) return OR_ESTATE;
         it->dt_did = it->dt_try;
         for (int s = 1; s < 4; s++) {
             ttemp = d->t + rk4_c[s] * it->dt_try;
@@ -1017,14 +1048,23 @@ static int rk4_step(or_integ* it, double* dt_did)
             uint32_t nmass = n_si_of(d->playing) + n_nsi_of(d->playing);
             uint32_t nv = 4 * (it->error_check_for_tp ? d->n : nmass);
             double m = 0.0;
+            it->last_bad = 0;
             for (int i = 0; i < 2; i++) {
                 const double *f4 = comp(it->k[i][3]), *f5 = comp(it->k[i][4]);
-                for (uint32_t t = 0; t < nv; t++) { double e = fabs(f4[t] - f5[t]); if (m < e) m = e; }
+                for (uint32_t t = 0; t < nv; t++) {
+                    double e = fabs(f4[t] - f5[t]);
+                    if (m < e) m = e;
+                    if (!(e <= DBL_MAX)) it->last_bad = 1;
+                }
             }
             max_err = fabs(1.0 / 10.0 * it->dt_try * m);
             it->last_err = max_err;
             if (0.0 < max_err) it->dt_next = it->dt_try *= 0.9 * pow(it->tol / max_err, 1.0 / (it->order));
             else it->dt_next = it->dt_try *= 2.0;
+            if (guard_trips(it, max_err, iter + 1)) {
+                it->tried += iter + 1; it->failed += iter + 1;
+                return OR_ESTATE;
+            }
         }
         iter++;
     } while (it->adaptive && max_err > it->tol);

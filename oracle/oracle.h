@@ -57,7 +57,7 @@ enum { OR_THR_HC = 0, OR_THR_EJ = 1, OR_THR_RAD = 2 };
 enum { OR_EV_NONE = 0, OR_EV_HIT_CENTRUM, OR_EV_EJECTION, OR_EV_COLLISION };
 enum { OR_FORMULA_GPU = 0, OR_FORMULA_CPU_SYM = 1 };
 enum { OR_DRAG_NSI = 0, OR_DRAG_ALL = 1 };
-enum { OR_OK = 0, OR_EINVAL = 1, OR_ENOTIMPL = 3, OR_EOVERFLOW = 4 };
+enum { OR_OK = 0, OR_EINVAL = 1, OR_ENOTIMPL = 3, OR_EOVERFLOW = 4, OR_ESTATE = 5 };
 
 /* ---- n_objects_t interaction bounds (red_type.h:622-700) ---------------- */
 void or_bound(const uint32_t playing[OR_NTYPE], int which, uint32_t out[4]);
@@ -149,6 +149,12 @@ double    or_get_dt_next(const or_integ* it);
 void      or_set_dt_next(or_integ* it, double dt);
 void      or_counters(const or_integ* it, uint64_t out[3]);   /* tried, passed, failed */
 void      or_set_error_check_for_tp(or_integ* it, int on);
+/* Opt-in bound on the retry loop (src/red_v2/int_rungekutta7.cu:176-185 has
+ * such a bound; red.cuda has none): or_step returns OR_ESTATE, keeping y_n and
+ * counting the step's attempts as failed, when a step is still rejected after
+ * max_attempts attempts, when an attempt would use |dt| <= min_dt (it is not
+ * made), or (nonfinite) when an error term is NaN or infinite.  0 = off. */
+void      or_set_guard(or_integ* it, uint32_t max_attempts, double min_dt, int nonfinite);
 /* The controller's error of the last attempt, |lambda * dt * max_i |E_i||
  * (int_rungekutta8.cu:693-696, int_rungekutta4.cu:226-229), and the terms
  * |E_i| it is the maximum of, for every component of every body (positions

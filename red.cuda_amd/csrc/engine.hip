@@ -172,6 +172,13 @@ struct rd_disk {
     uint32_t* ev_buf = nullptr;
     uint32_t* ev_counter = nullptr;
     uint32_t ev_cap = 0;
+    // records of the bodies an event touches, staged for one gather / scatter
+    uint32_t* ev_sidx = nullptr;
+    int32_t* ev_sids = nullptr;
+    double* ev_srec = nullptr;
+    uint32_t ev_scap = 0;
+    // integrators over this disk: their graphs are released before the stream
+    std::vector<rd_integrator*> integrators;
 
     // pinned host mirrors
     DiskCtl* h_ctl = nullptr;
@@ -204,6 +211,10 @@ struct rd_integrator {
     std::vector<uint64_t> tried, passed, failed;
     uint64_t force_evals = 0;
     double pairs = 0.0;
+    // opt-in bound on the retry loop (0 = off: the reference's unbounded loop)
+    uint32_t max_attempts = 0;
+    double min_dt = 0.0;
+    bool reject_nonfinite = false;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     uint64_t graph_gen = ~0ull;
@@ -230,6 +241,26 @@ namespace {
 // --------------------------------------------------------------------------
 // layout helpers
 // --------------------------------------------------------------------------
+// An integrator's captured attempt is destroyed only once the stream it was
+// launched on has drained, and always before that stream is destroyed.
+// An integrator whose disk was destroyed refuses every call that needs the disk.
+int check_live(const rd_integrator* it)
+{
+    if (!it) return fail(RD_EINVAL, "NULL integrator");
+    if (!it->d) return fail(RD_ESTATE, "the integrator's disk was destroyed");
+    return RD_OK;
+}
+
+void release_graph(rd_integrator* it)
+{
+    if (!it->gexec && !it->graph) return;
+    if (it->d) (void)hipStreamSynchronize(it->d->stream);
+    if (it->gexec) (void)hipGraphExecDestroy(it->gexec);
+    if (it->graph) (void)hipGraphDestroy(it->graph);
+    it->gexec = nullptr;
+    it->graph = nullptr;
+}
+
 double* kv_ptr(rd_disk* d, int r, int c) { return d->K + ((size_t)r * 6 + c) * d->n_all; }
 double* ka_ptr(rd_disk* d, int r, int c) { return d->K + ((size_t)r * 6 + 3 + c) * d->n_all; }
 
@@ -259,6 +290,10 @@ void free_buffers(rd_disk* d)
     dev_free(d->items_feat);
     dev_free(d->ev_buf);
     dev_free(d->ev_counter);
+    dev_free(d->ev_sidx);
+    dev_free(d->ev_sids);
+    dev_free(d->ev_srec);
+    d->ev_scap = 0;
     dev_free(d->feat);
     dev_free(d->esum);
     dev_free(d->fx);
@@ -292,7 +327,7 @@ int alloc_buffers(rd_disk* d)
     if (int rc = dev_alloc(&d->disk_of, n)) return rc;
     if (int rc = dev_alloc(&d->info, d->n_disk)) return rc;
     if (int rc = dev_alloc(&d->ctl, d->n_disk)) return rc;
-    if (int rc = dev_alloc(&d->err, d->n_disk)) return rc;
+    if (int rc = dev_alloc(&d->err, 2 * (size_t)d->n_disk)) return rc;   // max bits, then non-finite flags
     if (int rc = dev_alloc(&d->red_out, 16)) return rc;
     if (int rc = dev_alloc(&d->trash, 3 * (size_t)kBodyBlock)) return rc;
     if (int rc = dev_alloc(&d->errw, n_errw(d))) return rc;
@@ -310,7 +345,7 @@ int alloc_buffers(rd_disk* d)
     if (int rc = dev_alloc(&d->coll_cell, 3 * n)) return rc;
     if (int rc = dev_alloc(&d->coll_rmax, 1)) return rc;
     HIP_TRY(hipHostMalloc((void**)&d->h_ctl, sizeof(DiskCtl) * std::max<uint32_t>(1, d->n_disk)));
-    HIP_TRY(hipHostMalloc((void**)&d->h_err, sizeof(unsigned long long) * std::max<uint32_t>(1, d->n_disk)));
+    HIP_TRY(hipHostMalloc((void**)&d->h_err, 2 * sizeof(unsigned long long) * std::max<uint32_t>(1, d->n_disk)));
     HIP_TRY(hipHostMalloc((void**)&d->h_errw, sizeof(unsigned long long) * n_errw(d)));
     HIP_TRY(hipHostMalloc((void**)&d->h_feat, 3 * sizeof(uint32_t) * std::max<uint32_t>(1, d->n_disk)));
     HIP_TRY(hipHostMalloc((void**)&d->h_esum, sizeof(double) * std::max<uint32_t>(1, d->n_disk)));
@@ -606,6 +641,7 @@ StageArgs make_args(rd_disk* d, int r, int adaptive, int err_all)
     a.info = d->info;
     a.ctl = d->ctl;
     a.err = d->err;
+    a.errf = d->err + d->n_disk;
     a.mass = d->mass; a.radius = d->radius; a.density = d->density; a.cd = d->cd;
     a.id = d->id;
     a.gas = d->gas;
@@ -671,10 +707,7 @@ int run_attempt(rd_integrator* it, bool use_graph)
     if (use_graph) {
         const int kind = it->method * 16 + it->adaptive * 2 + it->err_all;
         if (!it->gexec || it->graph_gen != d->generation || it->graph_kind != kind) {
-            if (it->gexec) (void)hipGraphExecDestroy(it->gexec);
-            if (it->graph) (void)hipGraphDestroy(it->graph);
-            it->gexec = nullptr;
-            it->graph = nullptr;
+            release_graph(it);
             HIP_TRY(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
             hipError_t e = enqueue_attempt(it);
             hipGraph_t g = nullptr;
@@ -807,6 +840,12 @@ int rd_disk_destroy(rd_disk* d)
 {
     DeviceScope dev_scope(d ? d->device : -1);
     if (!d) return RD_OK;
+    // integrators that outlive the disk keep no graph and no disk pointer
+    for (rd_integrator* it : d->integrators) {
+        release_graph(it);
+        it->d = nullptr;
+    }
+    d->integrators.clear();
     (void)hipStreamSynchronize(d->stream);
     free_buffers(d);
     (void)hipStreamDestroy(d->stream);
@@ -1119,6 +1158,7 @@ int rd_integrator_create(rd_disk* d, int method, double dt, int adaptive, double
     it->passed.assign(d->n_disk, 0);
     it->failed.assign(d->n_disk, 0);
     it->hist.assign(d->n_disk, rd_integrator::Hist{});
+    d->integrators.push_back(it);
     *out = it;
     return RD_OK;
 }
@@ -1126,11 +1166,15 @@ int rd_integrator_create(rd_disk* d, int method, double dt, int adaptive, double
 int rd_integrator_destroy(rd_integrator* it)
 {
     // may run after the disk is gone (destroy order is the caller's; garbage
-    // collectors pick one): touch nothing but the integrator
+    // collectors pick one): the disk then already released the graph and
+    // cleared it->d
     DeviceScope dev_scope(it ? it->device : -1);
     if (!it) return RD_OK;
-    if (it->gexec) (void)hipGraphExecDestroy(it->gexec);
-    if (it->graph) (void)hipGraphDestroy(it->graph);
+    release_graph(it);
+    if (it->d) {
+        auto& v = it->d->integrators;
+        v.erase(std::remove(v.begin(), v.end(), it), v.end());
+    }
     delete it;
     return RD_OK;
 }
@@ -1251,9 +1295,19 @@ static int step_disks(rd_integrator* it, const std::vector<uint32_t>& want)
     std::vector<uint32_t> commit(nd, 0);
     std::vector<char> acc(nd, 0);
     std::vector<double> err_ratio(nd, 0.0);
+    std::string guard_msg;   // first disk whose step the guard failed
     for (;;) {
         bool any = false;
         for (uint32_t k = 0; k < nd; ++k) {
+            // opt-in guard: an attempt with |dt| <= min_dt is not made
+            if (pending[k] && it->adaptive && 0.0 < it->min_dt && std::fabs(it->dt_try[k]) <= it->min_dt) {
+                it->tried[k] += iter[k];
+                it->failed[k] += iter[k];
+                pending[k] = 0;
+                remaining[k] = 0;
+                if (guard_msg.empty())
+                    guard_msg = "disk " + std::to_string(k) + ": The stepsize is smaller than the limit.";
+            }
             uint32_t fl = commit[k];
             if (pending[k]) {
                 fl |= F_ACTIVE | (need_k0[k] ? F_NEED_K0 : 0u);
@@ -1282,14 +1336,18 @@ static int step_disks(rd_integrator* it, const std::vector<uint32_t>& want)
         if (it->adaptive && nd == 1)   // single disk: one error slot per wave of the last stage
             HIP_TRY(hipMemcpyAsync(d->h_errw, d->errw, nw * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                    d->stream));
-        else if (it->adaptive)
-            HIP_TRY(hipMemcpyAsync(d->h_err, d->err, nd * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+        else if (it->adaptive)   // max bits and non-finite flags of every disk
+            HIP_TRY(hipMemcpyAsync(d->h_err, d->err, 2 * nd * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                    d->stream));
         HIP_TRY(hipStreamSynchronize(d->stream));
         if (it->adaptive && nd == 1) {
-            unsigned long long m = 0;
-            for (size_t w = 0; w < nw; ++w) m = std::max(m, d->h_errw[w]);
+            unsigned long long m = 0, bad = 0;
+            for (size_t w = 0; w < nw; ++w) {
+                m = std::max(m, d->h_errw[w] & ~kErrNonFinite);
+                bad |= d->h_errw[w] & kErrNonFinite;
+            }
             d->h_err[0] = m;
+            d->h_err[1] = bad;
         }
         // pair count of this attempt: active disks' full evaluation per stage
         for (uint32_t k = 0; k < nd; ++k) {
@@ -1312,6 +1370,21 @@ static int step_disks(rd_integrator* it, const std::vector<uint32_t>& want)
             }
             iter[k]++;
             err_ratio[k] = it->adaptive ? max_err / it->tol : 1.0;
+            // opt-in guard: the step of disk k fails instead of retrying
+            // without end or accepting a non-finite estimate; y_n is kept
+            const char* why = nullptr;
+            if (it->adaptive && it->reject_nonfinite && (d->h_err[nd + k] || !std::isfinite(max_err)))
+                why = "The error estimate is not finite.";
+            else if (it->adaptive && max_err > it->tol && it->max_attempts && (uint32_t)iter[k] >= it->max_attempts)
+                why = "The number of iteration exceeded the limit.";
+            if (why) {
+                it->tried[k] += iter[k];
+                it->failed[k] += iter[k];
+                pending[k] = 0;
+                remaining[k] = 0;
+                if (guard_msg.empty()) guard_msg = "disk " + std::to_string(k) + ": " + why;
+                continue;
+            }
             if (it->adaptive && max_err > it->tol) {
                 need_k0[k] = 0;       // k0 is reused (int_rungekutta8.cu:648-656)
             } else {
@@ -1342,13 +1415,14 @@ static int step_disks(rd_integrator* it, const std::vector<uint32_t>& want)
         }
     }
     HIP_TRY(hipStreamSynchronize(d->stream));
+    if (!guard_msg.empty()) return fail(RD_ESTATE, guard_msg);
     return RD_OK;
 }
 
 int rd_step(rd_integrator* it, double* dt_did)
 {
     DeviceScope dev_scope(it ? it->device : -1);
-    if (!it) return fail(RD_EINVAL, "NULL integrator");
+    if (int rc = check_live(it)) return rc;
     std::vector<uint32_t> want(it->d->n_disk, 1u);
     if (int rc = step_disks(it, want)) return rc;
     if (dt_did) *dt_did = it->dt_did[0];
@@ -1358,7 +1432,7 @@ int rd_step(rd_integrator* it, double* dt_did)
 int rd_steps(rd_integrator* it, uint32_t n_steps)
 {
     DeviceScope dev_scope(it ? it->device : -1);
-    if (!it) return fail(RD_EINVAL, "NULL integrator");
+    if (int rc = check_live(it)) return rc;
     rd_disk* d = it->d;
     if (!it->adaptive) {
         // Fixed step: every attempt is accepted, so attempts are chained on the
@@ -1377,7 +1451,9 @@ int rd_steps(rd_integrator* it, uint32_t n_steps)
             if (int rc = run_attempt(it, d->tuning.use_graph != 0)) return rc;
             HIP_TRY(launch_commit(d->stream, a, 1, d->ctl));
             it->force_evals += (uint64_t)it->n_stage;
-            if ((s & 255u) == 255u) HIP_TRY(hipStreamSynchronize(d->stream));
+            // at most 64 attempts (each a graph launch and a commit) queued
+            // between host waits
+            if ((s & 63u) == 63u) HIP_TRY(hipStreamSynchronize(d->stream));
         }
         for (uint32_t k = 0; k < nd; ++k) {
             const HostDisk& h = d->hd[k];
@@ -1402,7 +1478,8 @@ int rd_steps(rd_integrator* it, uint32_t n_steps)
 int rd_get_dt_next(rd_integrator* it, uint32_t k, double* dt)
 {
     DeviceScope dev_scope(it ? it->device : -1);
-    if (!it || !dt || k >= it->d->n_disk) return fail(RD_EINVAL, "bad argument");
+    if (int rc = check_live(it)) return rc;
+    if (!dt || k >= it->d->n_disk) return fail(RD_EINVAL, "bad argument");
     *dt = it->dt_next[k];
     return RD_OK;
 }
@@ -1410,7 +1487,8 @@ int rd_get_dt_next(rd_integrator* it, uint32_t k, double* dt)
 int rd_set_dt_next(rd_integrator* it, uint32_t k, double dt)
 {
     DeviceScope dev_scope(it ? it->device : -1);
-    if (!it || k >= it->d->n_disk) return fail(RD_EINVAL, "bad argument");
+    if (int rc = check_live(it)) return rc;
+    if (k >= it->d->n_disk) return fail(RD_EINVAL, "bad argument");
     it->dt_next[k] = dt;
     return RD_OK;
 }
@@ -1418,7 +1496,8 @@ int rd_set_dt_next(rd_integrator* it, uint32_t k, double dt)
 int rd_counters(rd_integrator* it, uint32_t k, uint64_t* tried, uint64_t* passed, uint64_t* failed)
 {
     DeviceScope dev_scope(it ? it->device : -1);
-    if (!it || k >= it->d->n_disk) return fail(RD_EINVAL, "bad argument");
+    if (int rc = check_live(it)) return rc;
+    if (k >= it->d->n_disk) return fail(RD_EINVAL, "bad argument");
     if (tried) *tried = it->tried[k];
     if (passed) *passed = it->passed[k];
     if (failed) *failed = it->failed[k];
@@ -1433,10 +1512,22 @@ int rd_set_error_check_for_tp(rd_integrator* it, int on)
     return RD_OK;
 }
 
+int rd_integrator_set_guard(rd_integrator* it, uint32_t max_attempts, double min_dt, int nonfinite)
+{
+    DeviceScope dev_scope(it ? it->device : -1);
+    if (!it) return fail(RD_EINVAL, "NULL integrator");
+    if (!(min_dt >= 0.0)) return fail(RD_EINVAL, "min_dt must be >= 0");
+    it->max_attempts = max_attempts;
+    it->min_dt = min_dt;
+    it->reject_nonfinite = nonfinite != 0;
+    return RD_OK;
+}
+
 int rd_integrator_set_proposer(rd_integrator* it, rd_mlp* m, float safety)
 {
     DeviceScope dev_scope(it ? it->device : -1);
-    if (!it || !(safety > 0.0f)) return fail(RD_EINVAL, "bad argument");
+    if (int rc = check_live(it)) return rc;
+    if (!(safety > 0.0f)) return fail(RD_EINVAL, "bad argument");
     if (m && !it->adaptive) return fail(RD_ESTATE, "the proposer needs an adaptive integrator");
     if (m && mlp_device(m) != it->d->device) return fail(RD_EINVAL, "proposer and disk are on different devices");
     it->proposer = m;
@@ -1504,8 +1595,95 @@ int fetch(rd_disk* d, uint32_t k, HostView& hv)
     return download_bodies(d, h.off, h.n, hv.r.data(), hv.v.data(), hv.p.data());
 }
 
+// The records (r, v, p) of the few bodies an event touches, by local index.
+// Events move only these over PCIe: one gather kernel and one copy out, one
+// copy in and one scatter kernel back (the reference copies the three records
+// of each merge, pp_disk.cu:1358-1363, and one metadata record, 1240).
+struct BodyRec {
+    rd_var4 r, v;
+    rd_param p;
+};
+struct SparseView {
+    std::vector<uint32_t> idx;   // local indices, ascending
+    std::vector<BodyRec> rec;
+    BodyRec& at(uint32_t i) { return rec[std::lower_bound(idx.begin(), idx.end(), i) - idx.begin()]; }
+    const BodyRec& at(uint32_t i) const { return rec[std::lower_bound(idx.begin(), idx.end(), i) - idx.begin()]; }
+};
+
+BodyArrays body_arrays(rd_disk* d)
+{
+    return BodyArrays{{d->x, d->y, d->z, d->vx, d->vy, d->vz, d->mass, d->radius, d->density, d->cd}};
+}
+
+int ensure_stage(rd_disk* d, size_t n)
+{
+    if (n <= d->ev_scap) return RD_OK;
+    const uint32_t cap = (uint32_t)std::max<size_t>(256, 2 * n);
+    dev_free(d->ev_sidx);
+    dev_free(d->ev_sids);
+    dev_free(d->ev_srec);
+    d->ev_scap = 0;
+    if (int rc = dev_alloc(&d->ev_sidx, cap)) return rc;
+    if (int rc = dev_alloc(&d->ev_sids, cap)) return rc;
+    if (int rc = dev_alloc(&d->ev_srec, (size_t)kBodyRecord * cap)) return rc;
+    d->ev_scap = cap;
+    return RD_OK;
+}
+
+// Records of disk k's bodies `want` (local indices, any order, repeats allowed).
+int fetch_records(rd_disk* d, uint32_t k, std::vector<uint32_t> want, SparseView& sv)
+{
+    std::sort(want.begin(), want.end());
+    want.erase(std::unique(want.begin(), want.end()), want.end());
+    const uint32_t off = d->hd[k].off, n = (uint32_t)want.size();
+    sv.idx = want;
+    sv.rec.assign(n, BodyRec{});
+    if (n == 0) return RD_OK;
+    if (int rc = ensure_stage(d, n)) return rc;
+    std::vector<uint32_t> g(n);
+    for (uint32_t q = 0; q < n; ++q) g[q] = off + want[q];
+    std::vector<double> buf((size_t)kBodyRecord * n);
+    HIP_TRY(hipMemcpyAsync(d->ev_sidx, g.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(launch_gather_bodies(d->stream, n, d->ev_sidx, body_arrays(d), d->ev_srec));
+    HIP_TRY(hipMemcpyAsync(buf.data(), d->ev_srec, buf.size() * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    for (uint32_t q = 0; q < n; ++q) {
+        const double* b = &buf[(size_t)kBodyRecord * q];
+        sv.rec[q] = BodyRec{rd_var4{b[0], b[1], b[2], 0.0}, rd_var4{b[3], b[4], b[5], 0.0},
+                            rd_param{b[6], b[7], b[8], b[9]}};
+    }
+    return RD_OK;
+}
+
+// Writes the records of sv and the bodies' current ids back to disk k, and
+// refreshes the source masses the pair kernels read (a device pass).
+int push_records(rd_disk* d, uint32_t k, const SparseView& sv)
+{
+    const uint32_t off = d->hd[k].off, n = (uint32_t)sv.idx.size();
+    if (n == 0) return RD_OK;
+    if (int rc = ensure_stage(d, n)) return rc;
+    std::vector<uint32_t> g(n);
+    std::vector<int32_t> ids(n);
+    std::vector<double> buf((size_t)kBodyRecord * n);
+    for (uint32_t q = 0; q < n; ++q) {
+        g[q] = off + sv.idx[q];
+        ids[q] = d->bmd[off + sv.idx[q]].id;
+        const BodyRec& b = sv.rec[q];
+        const double v[kBodyRecord] = {b.r.x, b.r.y, b.r.z, b.v.x, b.v.y, b.v.z, b.p.mass, b.p.radius, b.p.density,
+                                       b.p.cd};
+        std::memcpy(&buf[(size_t)kBodyRecord * q], v, sizeof(v));
+    }
+    HIP_TRY(hipMemcpyAsync(d->ev_sidx, g.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipMemcpyAsync(d->ev_sids, ids.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipMemcpyAsync(d->ev_srec, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(launch_scatter_bodies(d->stream, n, d->ev_sidx, d->ev_srec, d->ev_sids, body_arrays(d), d->id));
+    HIP_TRY(launch_set_sources(d->stream, d->n_all, d->mass, d->id, d->disk_of, d->info, d->m_ex, d->m32));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    return RD_OK;
+}
+
 // pp_disk_utility::store_event_data (pp_disk.cu:196-253) without the printf.
-rd_event make_event(int name, double t, double dist, uint32_t idx1, uint32_t idx2, const HostView& hv,
+rd_event make_event(int name, double t, double dist, uint32_t idx1, uint32_t idx2, const SparseView& sv,
                     const rd_body_md* bmd)
 {
     rd_event e;
@@ -1515,13 +1693,13 @@ rd_event make_event(int name, double t, double dist, uint32_t idx1, uint32_t idx
     e.t = t;
     e.id0 = bmd[0].id; e.id1 = bmd[idx1].id; e.id2 = bmd[idx2].id;
     e.idx0 = 0; e.idx1 = idx1; e.idx2 = idx2;
-    e.r0 = hv.r[0]; e.v0 = hv.v[0];
-    e.r1 = hv.r[idx1]; e.v1 = hv.v[idx1];
-    e.r2 = hv.r[idx2]; e.v2 = hv.v[idx2];
-    e.p0 = hv.p[0];
+    e.r0 = sv.at(0).r; e.v0 = sv.at(0).v;
+    e.r1 = sv.at(idx1).r; e.v1 = sv.at(idx1).v;
+    e.r2 = sv.at(idx2).r; e.v2 = sv.at(idx2).v;
+    e.p0 = sv.at(0).p;
     if (RD_EVENT_EJECTION == name) {
-        e.p1 = hv.p[idx1];
-        e.p2 = hv.p[idx2];
+        e.p1 = sv.at(idx1).p;
+        e.p2 = sv.at(idx2).p;
         e.rs = e.r1;
         e.vs = e.v1;
         e.ps = e.p1;
@@ -1535,14 +1713,25 @@ double calc_radius(double m, double density)   // tools.cpp:455-460
     return std::pow(1.0 / four_pi_over_three * m / density, 1.0 / 3.0);
 }
 
+// The reference's refusal (pp_disk.cu:1333-1337), checked for every pair of
+// an event batch before any of them is merged, so a refused batch changes
+// nothing (the reference throws part-way through, after earlier merges).
+int refuse_superpl(const std::vector<rd_event>& ev, const rd_body_md* bmd, bool skip_ejections)
+{
+    for (const rd_event& e : ev) {
+        if (skip_ejections && RD_EVENT_EJECTION == e.event_name) continue;
+        if (RD_BODY_SUPERPLANETESIMAL == bmd[e.idx2].body_type)
+            return fail(RD_ENOTIMPL, "Collision between a massive body and a super-planetesimal is not yet implemented.");
+    }
+    return RD_OK;
+}
+
 // handle_collision_pair (pp_disk.cu:1328-1368) with tools.cpp:480-507
-int merge_pair(HostView& hv, const rd_body_md* bmd, rd_event& c)
+void merge_pair(SparseView& sv, rd_event& c)
 {
     const uint32_t s = c.idx1, m = c.idx2;
-    if (RD_BODY_SUPERPLANETESIMAL == bmd[m].body_type)
-        return fail(RD_ENOTIMPL, "Collision between a massive body and a super-planetesimal is not yet implemented.");
-    c.p1 = hv.p[s];
-    c.p2 = hv.p[m];
+    c.p1 = sv.at(s).p;
+    c.p2 = sv.at(m).p;
     const double m1 = c.p1.mass, m2 = c.p2.mass, M = m1 + m2;
     c.rs.x = (m1 * c.r1.x + m2 * c.r2.x) / M;
     c.rs.y = (m1 * c.r1.y + m2 * c.r2.y) / M;
@@ -1552,26 +1741,15 @@ int merge_pair(HostView& hv, const rd_body_md* bmd, rd_event& c)
     c.vs.y = (m1 * c.v1.y + m2 * c.v2.y) / M;
     c.vs.z = (m1 * c.v1.z + m2 * c.v2.z) / M;
     c.vs.w = 0.0;
-    hv.r[s] = c.rs;
-    hv.v[s] = c.vs;
+    sv.at(s).r = c.rs;
+    sv.at(s).v = c.vs;
     const double R1 = c.p1.radius, R2 = c.p2.radius;
     const double volume = 4.188790204786391 * (R1 * R1 * R1 + R2 * R2 * R2);
     c.ps.mass = c.p1.mass + c.p2.mass;
     c.ps.density = c.ps.mass / volume;
     c.ps.radius = calc_radius(c.ps.mass, c.ps.density);
     c.ps.cd = c.p1.cd;
-    hv.p[s] = c.ps;
-    return RD_OK;
-}
-
-int push_back_disk(rd_disk* d, uint32_t k, const HostView& hv)
-{
-    const HostDisk& h = d->hd[k];
-    if (int rc = upload_bodies(d, h.off, h.n, hv.r.data(), hv.v.data(), hv.p.data(), d->bmd.data() + h.off))
-        return rc;
-    HIP_TRY(launch_set_sources(d->stream, d->n_all, d->mass, d->id, d->disk_of, d->info, d->m_ex, d->m32));
-    HIP_TRY(hipStreamSynchronize(d->stream));
-    return RD_OK;
+    sv.at(s).p = c.ps;
 }
 
 // pp_disk::increment_event_counter (pp_disk.cu:1431-1440); kind 0 collision,
@@ -1647,18 +1825,21 @@ int detect_ejections(rd_disk* d, const std::vector<char>& want)
     std::sort(ev.begin(), ev.end(), [](auto& a, auto& b) {
         return a.first != b.first ? a.first < b.first : a.second > b.second;
     });
-    uint32_t cur = ~0u;
-    HostView hv;
-    for (auto& q : ev) {
-        const uint32_t k = disk_of_body(d, q.first);
+    for (size_t q0 = 0; q0 < ev.size();) {   // one disk's events at a time
+        const uint32_t k = disk_of_body(d, ev[q0].first);
         HostDisk& h = d->hd[k];
-        if (k != cur) {
-            if (int rc = fetch(d, k, hv)) return rc;
-            cur = k;
+        size_t q1 = q0;
+        std::vector<uint32_t> who{0};
+        for (; q1 < ev.size() && ev[q1].first < h.off + h.n; ++q1) who.push_back(ev[q1].first - h.off);
+        SparseView sv;
+        if (int rc = fetch_records(d, k, who, sv)) return rc;
+        for (size_t q = q0; q < q1; ++q) {
+            const uint32_t li = ev[q].first - h.off;
+            const rd_var4& r = sv.at(li).r;
+            const double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+            h.pending.push_back(make_event((int)ev[q].second, h.t, std::sqrt(r2), 0, li, sv, d->bmd.data() + h.off));
         }
-        const uint32_t li = q.first - h.off;
-        const double r2 = hv.r[li].x * hv.r[li].x + hv.r[li].y * hv.r[li].y + hv.r[li].z * hv.r[li].z;
-        h.pending.push_back(make_event((int)q.second, h.t, std::sqrt(r2), 0, li, hv, d->bmd.data() + h.off));
+        q0 = q1;
     }
     return RD_OK;
 }
@@ -1676,20 +1857,26 @@ int detect_collisions(rd_disk* d, const std::vector<char>& want)
     std::vector<std::pair<uint32_t, uint32_t>> pr;
     if (int rc = read_pairs(d, pr)) return rc;
     std::sort(pr.begin(), pr.end());
-    uint32_t cur = ~0u;
-    HostView hv;
-    for (auto& q : pr) {
-        const uint32_t k = disk_of_body(d, q.first);
+    for (size_t q0 = 0; q0 < pr.size();) {   // one disk's pairs at a time
+        const uint32_t k = disk_of_body(d, pr[q0].first);
         HostDisk& h = d->hd[k];
-        if (k != cur) {
-            if (int rc = fetch(d, k, hv)) return rc;
-            cur = k;
+        size_t q1 = q0;
+        std::vector<uint32_t> who{0};
+        for (; q1 < pr.size() && pr[q1].first < h.off + h.n; ++q1) {
+            who.push_back(pr[q1].first - h.off);
+            who.push_back(pr[q1].second - h.off);
         }
-        uint32_t s = q.first - h.off, m = q.second - h.off;
-        const double dx = hv.r[m].x - hv.r[s].x, dy = hv.r[m].y - hv.r[s].y, dz = hv.r[m].z - hv.r[s].z;
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        if (hv.p[m].mass > hv.p[s].mass) std::swap(s, m);
-        h.pending.push_back(make_event(RD_EVENT_COLLISION, h.t, std::sqrt(r2), s, m, hv, d->bmd.data() + h.off));
+        SparseView sv;
+        if (int rc = fetch_records(d, k, who, sv)) return rc;
+        for (size_t q = q0; q < q1; ++q) {
+            uint32_t s = pr[q].first - h.off, m = pr[q].second - h.off;
+            const rd_var4 &rs = sv.at(s).r, &rm = sv.at(m).r;
+            const double dx = rm.x - rs.x, dy = rm.y - rs.y, dz = rm.z - rs.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (sv.at(m).p.mass > sv.at(s).p.mass) std::swap(s, m);
+            h.pending.push_back(make_event(RD_EVENT_COLLISION, h.t, std::sqrt(r2), s, m, sv, d->bmd.data() + h.off));
+        }
+        q0 = q1;
     }
     return RD_OK;
 }
@@ -1754,8 +1941,6 @@ int rd_handle_collision(rd_disk* d, uint32_t k)
     if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
     HostDisk& h = d->hd[k];
     if (h.pending.empty()) return RD_OK;
-    HostView hv;
-    if (int rc = fetch(d, k, hv)) return rc;
     rd_body_md* bmd = d->bmd.data() + h.off;
     // populate_sp_events (pp_disk.cu:1250-1288)
     std::vector<rd_event> sp;
@@ -1771,16 +1956,24 @@ int rd_handle_collision(rd_disk* d, uint32_t k)
             }
         sp.push_back(e);
     }
+    if (int rc = refuse_superpl(sp, bmd, false)) return rc;
+    std::vector<uint32_t> who;
+    for (auto& e : sp) {
+        who.push_back(e.idx1);
+        who.push_back(e.idx2);
+    }
+    SparseView sv;
+    if (int rc = fetch_records(d, k, who, sv)) return rc;
     for (auto& e : sp) {
         const uint32_t ictv = e.idx2;
-        if (int rc = merge_pair(hv, bmd, e)) return rc;
+        merge_pair(sv, e);
         inc_counters(h, 0);
         bmd[ictv].id *= -1;
         h.inactive[bmd[ictv].body_type]++;
         h.log.push_back(e);
     }
     h.n_kind[0][2] = 0;   // pp_disk.cu:1247
-    return push_back_disk(d, k, hv);
+    return push_records(d, k, sv);
 }
 
 int rd_handle_ejection_hit_centrum(rd_disk* d, uint32_t k)
@@ -1789,20 +1982,25 @@ int rd_handle_ejection_hit_centrum(rd_disk* d, uint32_t k)
     if (!d || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
     HostDisk& h = d->hd[k];
     if (h.pending.empty()) return RD_OK;
-    HostView hv;
-    if (int rc = fetch(d, k, hv)) return rc;
     rd_body_md* bmd = d->bmd.data() + h.off;
+    if (int rc = refuse_superpl(h.pending, bmd, true)) return rc;
+    std::vector<uint32_t> who;
+    for (auto& e : h.pending) {
+        who.push_back(e.idx1);
+        who.push_back(e.idx2);
+    }
+    SparseView sv;
+    if (int rc = fetch_records(d, k, who, sv)) return rc;
     for (auto e : h.pending) {
         const uint32_t ictv = e.idx2;
-        if (RD_EVENT_EJECTION != e.event_name)
-            if (int rc = merge_pair(hv, bmd, e)) return rc;
+        if (RD_EVENT_EJECTION != e.event_name) merge_pair(sv, e);
         inc_counters(h, RD_EVENT_EJECTION == e.event_name ? 1 : 2);
         bmd[ictv].id *= -1;
         h.inactive[bmd[ictv].body_type]++;
         h.log.push_back(e);
     }
     h.n_kind[1][2] = h.n_kind[2][2] = 0;   // pp_disk.cu:1324-1325
-    return push_back_disk(d, k, hv);
+    return push_records(d, k, sv);
 }
 
 int rd_event_counters(rd_disk* d, uint32_t k, uint32_t n_event[3], uint32_t n_collision[3], uint32_t n_ejection[3],
@@ -1911,7 +2109,7 @@ int rd_calc_integral(rd_disk* d, uint32_t k, rd_integral* out)
 int rd_run(rd_integrator* it, double length, double output_interval, uint64_t max_steps)
 {
     DeviceScope dev_scope(it ? it->device : -1);
-    if (!it) return fail(RD_EINVAL, "NULL integrator");
+    if (int rc = check_live(it)) return rc;
     rd_disk* d = it->d;
     const uint32_t nd = d->n_disk;
     std::vector<double> elapsed(nd, 0.0), ps(nd, 0.0);

@@ -91,6 +91,7 @@ struct StageArgs {
     const DiskInfo* info;
     const DiskCtl* ctl;
     unsigned long long* err;     // per disk, bits of the raw max |E|
+    unsigned long long* errf;    // per disk, nonzero if an error term was NaN or infinite
     const double *mass, *radius, *density, *cd;
     const int32_t* id;
     GasDev gas;
@@ -100,6 +101,16 @@ struct StageArgs {
     DiskInfo info0;              // info[0] when one_disk (kernel arguments are scalar loads)
     double* trash;               // [3][kBodyBlock]: stores of a single-disk stage that does not apply
     unsigned long long* errw;    // single disk: error bits per wave of the last stage, [3][blocks]
+                                 // (kErrNonFinite set if a term of the wave was NaN or infinite)
+};
+
+constexpr unsigned long long kErrNonFinite = 1ull << 63;
+
+// The per-body arrays an event record is read from and written back to, in
+// record order: x, y, z, vx, vy, vz, mass, radius, density, cd.
+constexpr uint32_t kBodyRecord = 10;
+struct BodyArrays {
+    double* a[kBodyRecord];
 };
 
 // Launchers (kernels.hip).
@@ -145,6 +156,11 @@ hipError_t launch_features(hipStream_t s, const WorkItem* items, uint32_t n_item
                            const double* vy, const double* vz, const double* mass, const int32_t* id,
                            const uint32_t* disk_of, const DiskInfo* info, const DiskCtl* ctl, uint32_t* feat,
                            double* eterm, double* esum);
+// rec[kBodyRecord q + f] = b.a[f][idx[q]] for q < n
+hipError_t launch_gather_bodies(hipStream_t s, uint32_t n, const uint32_t* idx, const BodyArrays& b, double* rec);
+// the inverse, plus id[idx[q]] = ids[q]
+hipError_t launch_scatter_bodies(hipStream_t s, uint32_t n, const uint32_t* idx, const double* rec,
+                                 const int32_t* ids, const BodyArrays& b, int32_t* id);
 hipError_t launch_set_sources(hipStream_t s, uint32_t n_all, const double* mass, const int32_t* id,
                               const uint32_t* disk_of, const DiskInfo* info, double* m_ex, float* m32);
 

@@ -848,10 +848,14 @@ __device__ void rk8_comb_q(double y, double v, double h, double* kv, double* ka,
 }
 
 // Wave-level max into the per-disk error slot; waves whose lanes span two
-// disks fall back to one atomic per lane.
+// disks fall back to one atomic per lane.  The max takes a term only if it
+// compares greater, as the reference's max loop does (integrator.cu:248-263),
+// so a NaN term is never the maximum; NaN and infinite terms instead raise the
+// disk's non-finite flag a.errf[d] (read only by the opt-in step guard).
 __device__ void error_reduce(const StageArgs& a, bool valid, uint32_t d, double e)
 {
-    unsigned long long bits = valid ? (unsigned long long)__double_as_longlong(e) : 0ull;
+    unsigned long long bits = (valid && e == e) ? (unsigned long long)__double_as_longlong(e) : 0ull;
+    const bool bad = valid && !(e <= DBL_MAX);
     const uint32_t d0 = __builtin_amdgcn_readfirstlane(valid ? d : 0xffffffffu);
     const bool same = __all(!valid || d == d0);
     if (same) {
@@ -859,23 +863,31 @@ __device__ void error_reduce(const StageArgs& a, bool valid, uint32_t d, double 
             const unsigned long long o = __shfl_xor(bits, off, 64);
             bits = o > bits ? o : bits;
         }
-        if ((threadIdx.x & 63u) == 0 && d0 != 0xffffffffu && bits) atomicMax(&a.err[d0], bits);
-    } else if (valid && bits) {
-        atomicMax(&a.err[d], bits);
+        const bool any_bad = __any(bad);
+        if ((threadIdx.x & 63u) == 0 && d0 != 0xffffffffu) {
+            if (bits) atomicMax(&a.err[d0], bits);
+            if (any_bad) atomicOr(&a.errf[d0], 1ull);
+        }
+    } else if (valid) {
+        if (bits) atomicMax(&a.err[d], bits);
+        if (bad) atomicOr(&a.errf[d], 1ull);
     }
 }
 
 // Single-disk form of error_reduce: the wave's max goes to its own slot,
-// a.errw[blockIdx.y * gridDim.x + blockIdx.x] (one wave per block), and the
-// host takes the max of the slots.  One same-address atomic per wave would
-// serialise hundreds of waves on one word.
+// a.errw[blockIdx.y * gridDim.x + blockIdx.x] (one wave per block), with the
+// non-finite flag in bit 63 (free: a non-negative double's bits are below it),
+// and the host takes the max of the slots.  One same-address atomic per wave
+// would serialise hundreds of waves on one word.
 __device__ void error_store_wave(const StageArgs& a, bool valid, double e)
 {
-    unsigned long long bits = valid ? (unsigned long long)__double_as_longlong(e) : 0ull;
+    unsigned long long bits = (valid && e == e) ? (unsigned long long)__double_as_longlong(e) : 0ull;
+    const bool bad = valid && !(e <= DBL_MAX);
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long o = __shfl_xor(bits, off, 64);
         bits = o > bits ? o : bits;
     }
+    if (__any(bad)) bits |= kErrNonFinite;
     if ((threadIdx.x & 63u) == 0) a.errw[blockIdx.y * gridDim.x + blockIdx.x] = bits;
 }
 
@@ -1260,7 +1272,10 @@ __global__ __launch_bounds__(kBodyBlock) void k_comb1(StageArgs a)
     const double k0v = a.vel[q][i], k0a = ka_of(a, 0, q)[i];
     const DiskCtl c = a.ctl[d];
     if (!(c.flags & F_ACTIVE)) return;
-    if (q == 0 && i == a.info[d].off) a.err[d] = 0ull;
+    if (q == 0 && i == a.info[d].off) {
+        a.err[d] = 0ull;
+        a.errf[d] = 0ull;
+    }
     if constexpr (M == 4) {
         double kv[9], ka[9], p, w;
         rk8_comb_q<1, 0>(y, v, c.h, kv, ka, k0v, k0a, p, w);
@@ -1396,6 +1411,49 @@ hipError_t launch_set_sources(hipStream_t st, uint32_t n_all, const double* mass
     const uint32_t grid = (n_all + kBlock - 1) / kBlock;
     if (grid == 0) return hipSuccess;
     hipLaunchKernelGGL(k_set_sources, dim3(grid), dim3(kBlock), 0, st, n_all, mass, id, disk_of, info, m_ex, m32);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Body records of a few bodies (events): gather into / scatter from a packed
+// buffer of kBodyRecord doubles per body, so an event moves O(events) bytes
+// over PCIe instead of whole disks.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_gather_bodies(uint32_t n, const uint32_t* __restrict__ idx, BodyArrays b,
+                                                          double* __restrict__ rec)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n * kBodyRecord) return;
+    const uint32_t q = t / kBodyRecord, f = t % kBodyRecord;
+    rec[t] = b.a[f][idx[q]];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter_bodies(uint32_t n, const uint32_t* __restrict__ idx,
+                                                           const double* __restrict__ rec,
+                                                           const int32_t* __restrict__ ids, BodyArrays b,
+                                                           int32_t* __restrict__ id)
+{
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n * (kBodyRecord + 1)) return;
+    const uint32_t q = t / (kBodyRecord + 1), f = t % (kBodyRecord + 1);
+    if (f < kBodyRecord) b.a[f][idx[q]] = rec[q * kBodyRecord + f];
+    else id[idx[q]] = ids[q];
+}
+
+hipError_t launch_gather_bodies(hipStream_t st, uint32_t n, const uint32_t* idx, const BodyArrays& b, double* rec)
+{
+    const uint32_t grid = (n * kBodyRecord + kBlock - 1) / kBlock;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_bodies, dim3(grid), dim3(kBlock), 0, st, n, idx, b, rec);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_bodies(hipStream_t st, uint32_t n, const uint32_t* idx, const double* rec,
+                                 const int32_t* ids, const BodyArrays& b, int32_t* id)
+{
+    const uint32_t grid = (n * (kBodyRecord + 1) + kBlock - 1) / kBlock;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_bodies, dim3(grid), dim3(kBlock), 0, st, n, idx, rec, ids, b, id);
     return hipGetLastError();
 }
 

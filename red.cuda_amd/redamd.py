@@ -73,6 +73,7 @@ _SIGS = {
     "rd_set_dt_next": (C.c_int, [_P, _u32, _f64]),
     "rd_counters": (C.c_int, [_P, _u32, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)]),
     "rd_set_error_check_for_tp": (C.c_int, [_P, C.c_int]),
+    "rd_integrator_set_guard": (C.c_int, [_P, _u32, _f64, C.c_int]),
     "rd_work_counters": (C.c_int, [_P, C.POINTER(_u64), C.POINTER(_f64)]),
     "rd_run": (C.c_int, [_P, _f64, _f64, _u64]),
     "rd_check_ejection_hit_centrum": (C.c_int, [_P, _u32, C.POINTER(_u32)]),
@@ -128,6 +129,24 @@ def check(rc):
     if rc != RD_OK:
         raise RedError(rc, lib.rd_last_error().decode())
     return rc
+
+
+def _finalize(obj):
+    """Release obj's handle when it is garbage-collected, but not while the
+    interpreter shuts down: finalizers then run in no set order, possibly
+    after the library or the GPU runtime began its own teardown, and the
+    process exit releases every device resource anyway."""
+    if sys is None or sys.is_finalizing():
+        return
+    try:
+        obj.close()
+    except Exception:
+        pass
+
+
+def last_error():
+    """rd_last_error() of the calling thread."""
+    return lib.rd_last_error().decode()
 
 
 def device_count():
@@ -202,10 +221,7 @@ class Disk:
             self.h = C.c_void_p()
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
     def set_precision(self, prec):
         check(lib.rd_disk_set_precision(self.h, prec))
@@ -350,10 +366,7 @@ class Integrator:
             self.h = C.c_void_p()
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
     def step(self):
         dt = _f64()
@@ -381,6 +394,16 @@ class Integrator:
 
     def set_error_check_for_tp(self, on):
         check(lib.rd_set_error_check_for_tp(self.h, 1 if on else 0))
+
+    def set_guard(self, max_attempts=0, min_dt=0.0, nonfinite=False):
+        """Opt-in bound on the retry loop (rd_integrator_set_guard)."""
+        check(lib.rd_integrator_set_guard(self.h, max_attempts, min_dt, 1 if nonfinite else 0))
+
+    def step_rc(self):
+        """(status, dt_did) of rd_step without raising: the guard's RD_ESTATE."""
+        dt = _f64()
+        rc = lib.rd_step(self.h, C.byref(dt))
+        return rc, dt.value
 
     def work(self):
         e, p = _u64(), _f64()
@@ -423,10 +446,7 @@ class Mlp:
             self.h = C.c_void_p()
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
     def set_norm(self, mean, std, t_mean, t_std):
         m = np.ascontiguousarray(mean, np.float32)

@@ -80,6 +80,7 @@ _SIGS = {
     "or_counters": (None, [_P, _P]),
     "or_set_error_check_for_tp": (None, [_P, C.c_int]),
     "or_last_error": (_f64, [_P]),
+    "or_set_guard": (None, [_P, _u32, _f64, C.c_int]),
     "or_rk8_tableau": (None, [_P, _P, _P]),
     "or_last_error_terms": (_u32, [_P, _P, _u32]),
     "or_run": (C.c_int, [_P, _P, _f64, _f64, _u64, C.POINTER(_u64)]),
@@ -177,6 +178,15 @@ class OInteg:
         rc = lib.or_step(self.h, C.byref(dt))
         assert rc == 0, rc
         return dt.value
+
+    def step_rc(self):
+        """(status, dt_did) without asserting success."""
+        dt = _f64()
+        rc = lib.or_step(self.h, C.byref(dt))
+        return rc, dt.value
+
+    def set_guard(self, max_attempts=0, min_dt=0.0, nonfinite=False):
+        lib.or_set_guard(self.h, max_attempts, min_dt, 1 if nonfinite else 0)
 
     def counters(self):
         c = np.zeros(3, np.uint64)
