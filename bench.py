@@ -11,10 +11,23 @@ rejected attempt).  Inputs are resident in HBM before the timed region.
 Prints one JSON line:
   value         ordered pair interactions per second over the timed steps
                 (whole job: all stages, rejections, controller included)
-  rkf78_steps_per_s, energy error, force error vs the FP64 kernel, and the
-                final-state difference vs FP64 after 50 fixed-h RKF7(8) steps
+  rkf78_steps_per_s, force error vs the FP64 kernel, and the final-state
+                difference vs FP64 after 50 fixed-h RKF7(8) steps
+  rel_energy_error / energy_ok
+                over the workload's whole run: 1000 accepted steps from t = 0
+                (warmup, timed steps, then untimed steps up to 1000)
+  fp64_twin     the same disk and steps with the reference's FP64 pair
+                kernel: steps/s, pairs/s and its FP64 roofline
   roofline      pair kernel: 20 flop per pair over its HIP-event-timed
                 launch duration, against the 157.3 TFLOP/s FP32 peak
+  roofline_stage
+                the RK passes of one attempt (k_comb1 + 13 stage kernels):
+                4,512 algorithmic bytes per body over their HIP-event-timed
+                duration, against 8 TB/s
+  energy_horizon
+                relative energy drift over 10^3 orbits at 1 AU (P = 365.25 d):
+                bounded runs at the bench tolerance and at a tighter one,
+                the drift rate fitted and carried to 10^3 orbits
   cpu_baseline  the oracle's restatement of the reference CPU path
                 (symmetric FP64 loop, pp_disk.cu:822-865) on one core, timed
                 over the first RKF7(8) steps of the same disk (at least 10 s)
@@ -37,7 +50,13 @@ from proposer import batch_sizes  # noqa: E402
 
 PEAK_FP32 = 157.3e12
 PEAK_FP64 = 78.6e12
+PEAK_HBM = 8.0e12
 FLOP_PER_PAIR = 20.0
+# algorithmic HBM bytes of the RK passes per body and adaptive attempt: RKF7(8)
+# reads and writes 188 FP64 3-vectors (sum over stages of 1 + nonzero columns +
+# 1, y_{n+1}, error), RK4 (adaptive) 36
+STAGE_BYTES = {R.RD_RKF78: 188 * 24, R.RD_RK4: 36 * 24}
+YEAR_DAYS = 365.25     # orbital period at 1 AU: the energy horizon's orbit
 FS_STEPS = 50          # fixed-h steps of the final-state comparison against FP64 (config 2)
 GRAPH = 1   # replay each RK attempt from a captured hipGraph (--no-graph: direct launches)
 
@@ -194,6 +213,59 @@ def roofline(d, precision, traffic=None):
             "frac": achieved / (PEAK_FP32 if precision == "mixed" else PEAK_FP64), "traffic": traffic,
             "kernel": pair_kernel(precision, d.tuning()),
             "ms_per_launch": ms, "pairs_per_launch": pairs_launch}
+
+
+def roofline_stage(d, n_bodies, method=R.RD_RKF78, traffic=None):
+    """HBM roofline of one attempt's RK passes (rd_bench_stages, HIP events)."""
+    ms = d.bench_stages(method, 20)
+    alg = STAGE_BYTES[method] * n_bodies
+    achieved = alg / (ms * 1e-3)
+    return {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM, "traffic": traffic, "algorithmic_bytes": alg,
+            "kernels": "k_comb1 + k_stage_rk8 x13" if method == R.RD_RKF78 else "k_comb1 + k_stage_rk4 x5",
+            "ms_per_attempt": ms}
+
+
+def profiled_stage_traffic(workload_file):
+    """HBM bytes of one attempt's stage kernels from the committed PMC summary."""
+    try:
+        with open(os.path.join(ROOT, "profiles", workload_file)) as f:
+            m = re.search(r"Stage traffic per attempt: ([0-9.e+]+) B", f.read())
+    except OSError:
+        return None
+    return float(m[1]) if m else None
+
+
+def energy_horizon_leg(parts, tol, tuning, seconds, h0, orbits=1000.0, period_days=YEAR_DAYS):
+    """RKF7(8) on a fresh copy of the disk for about `seconds` of wall time,
+    |E/E0 - 1| every 500 steps; the drift rate is the least-squares slope of
+    the error against time through the origin, carried to `orbits` periods."""
+    r, v, p, b, nbt = parts
+    d = R.Disk(r, v, p, b, nbt)
+    d.set_precision(R.RD_PREC_MIXED)
+    d.set_tuning(sinks_per_lane=tuning.sinks_per_lane, chunk=tuning.chunk, blocks_per_cu=tuning.blocks_per_cu,
+                 pair_mode=tuning.pair_mode, sym_chunk_min=tuning.sym_chunk_min, use_graph=GRAPH)
+    it = R.Integrator(d, R.RD_RKF78, h0, adaptive=True, tol=tol)
+    E0 = d.integral()["E"]
+    P = period_days * R.K_GAUSS
+    ts, es = [], []
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds and (not ts or ts[-1] < orbits * P):
+        it.steps(500)
+        ts.append(d.state()[0])
+        es.append(abs(d.integral()["E"] / E0 - 1.0))
+    wall = time.perf_counter() - t0
+    ts, es = np.array(ts), np.array(es)
+    rate = float((ts * es).sum() / (ts * ts).sum())        # per time unit
+    tried, passed, _ = it.counters()
+    out = {"tol": tol, "period_days": period_days, "orbits_target": orbits, "orbits_run": float(ts[-1] / P),
+           "steps": passed, "attempts": tried, "wall_s": wall, "steps_per_s": passed / wall,
+           "rel_energy_error_at_end": float(es[-1]), "max_rel_energy_error": float(es.max()),
+           "drift_per_orbit": rate * P, "extrapolated_rel_error": rate * P * orbits,
+           "ok": bool(rate * P * orbits <= 1e-9),
+           "method": "linear fit |E/E0 - 1| = s t over the run, carried to the target"}
+    del it, d
+    return out
 
 
 def emit(out):
@@ -388,19 +460,23 @@ def bench_config5(args):
 
 def energy_drift(args):
     """Long-horizon energy check of the default disk: RKF7(8) at args.tol in
-    args.precision until args.energy_orbits periods of the disk's inner edge,
-    the relative energy error (rd_calc_integral) at every tenth of the run.
-    Not a throughput figure: the run prints its checkpoints and one JSON line."""
+    args.precision until args.energy_orbits orbits (P = --period-days, 365.25
+    d: 1 AU), the relative energy error (rd_calc_integral) at every tenth of
+    the run.  Not a throughput figure: the run prints its checkpoints and one
+    JSON line."""
     r, v, p, b, nbt = R.make_disk(args.n, args.seed)
     a_min = np.min(np.hypot(r["x"][1:], r["y"][1:]))
-    period = 2 * np.pi * np.sqrt(a_min ** 3 / 1.0)
+    h0 = 2 * np.pi * np.sqrt(a_min ** 3 / 1.0) / 1000.0
+    # the horizon's orbit: 1 AU (365.25 d) unless --period-days 0 asks for the
+    # disk's inner edge
+    period = args.period_days * R.K_GAUSS if args.period_days > 0 else 2 * np.pi * np.sqrt(a_min ** 3 / 1.0)
     d = R.Disk(r, v, p, b, nbt)
     d.set_precision(R.RD_PREC_MIXED if args.precision == "mixed" else R.RD_PREC_FP64)
     d.set_tuning(use_graph=GRAPH)
     if args.precision == "mixed" and not args.no_autotune:
         d.autotune()
         d.set_tuning(use_graph=GRAPH)
-    it = R.Integrator(d, R.RD_RKF78, period / 1000.0, adaptive=True, tol=args.tol)
+    it = R.Integrator(d, R.RD_RKF78, h0, adaptive=True, tol=args.tol)
     E0 = d.integral()["E"]
     t0 = time.perf_counter()
     checkpoints, mark = [], 0.1
@@ -418,7 +494,7 @@ def energy_drift(args):
         if t >= args.energy_orbits * period:
             break
     print(json.dumps({"energy_drift": {"n_bodies": int(len(r)), "seed": args.seed, "tolerance": args.tol,
-                                       "precision": args.precision, "inner_period_days": period / R.K_GAUSS,
+                                       "precision": args.precision, "period_days": period / R.K_GAUSS,
                                        "checkpoints": checkpoints}}))
 
 
@@ -449,7 +525,14 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run the byte-counter calibration copy of this size and exit")
     ap.add_argument("--energy-orbits", type=float, default=0.0,
-                    help="only run the default disk for this many inner-edge orbits and report the energy drift")
+                    help="only run the default disk for this many orbits and report the energy drift")
+    ap.add_argument("--period-days", type=float, default=YEAR_DAYS,
+                    help="orbital period of the energy horizon (365.25: 1 AU; 0: the disk's inner edge)")
+    ap.add_argument("--horizon-seconds", type=float, default=20.0,
+                    help="config 2: wall seconds of each energy-horizon leg (0: skip)")
+    ap.add_argument("--horizon-tol", type=float, default=3e-12,
+                    help="config 2: the tighter tolerance of the second energy-horizon leg")
+    ap.add_argument("--no-twin", action="store_true", help="config 2: skip the FP64 twin run")
     args = ap.parse_args()
     global GRAPH
     GRAPH = 0 if args.no_graph else 1
@@ -486,6 +569,7 @@ def main():
         tuning = d.autotune() if (args.precision == "mixed" and not args.no_autotune) else d.tuning()
     d.set_tuning(use_graph=GRAPH)
     it = R.Integrator(d, R.RD_RKF78, h0, adaptive=True, tol=args.tol)
+    E_init = d.integral()["E"]
     print(f"# warmup {args.warmup} steps", file=sys.stderr, flush=True)
     it.steps(args.warmup)
     d.sync()
@@ -517,6 +601,16 @@ def main():
     # pair-kernel launch time (HIP events on the engine stream) for the roofline
     ms, pairs_launch = d.bench_force(20)
     achieved = FLOP_PER_PAIR * pairs_launch / (ms * 1e-3)
+    # the RK passes of one attempt (HIP events on the engine stream)
+    stage_rl = roofline_stage(d, args.n + 1, traffic=profiled_stage_traffic(f"k_stage_rk8_disk{args.n}_rkf78.txt")
+                              if (args.precision == "mixed" and args.seed == 2) else None)
+
+    # the energy criterion over the workload's run: 1000 accepted steps from t = 0
+    run_steps = max(1000, passed1)
+    if passed1 < run_steps:
+        print(f"# untimed steps {passed1} -> {run_steps} (energy over the run)", file=sys.stderr, flush=True)
+        it.steps(run_steps - passed1)
+    rel_e_run = abs(d.integral()["E"] / E_init - 1.0)
 
     # force error of the mixed kernel against the FP64 reference kernel at the final state
     f_err = None
@@ -549,6 +643,41 @@ def main():
                    "max_rel_pos": float((np.linalg.norm(rm - re, axis=1) / np.linalg.norm(re, axis=1))[1:].max()),
                    "max_rel_vel": float((np.linalg.norm(vm - ve, axis=1) / np.linalg.norm(ve, axis=1))[1:].max())}
 
+    # the FP64 twin: the same disk and steps with the reference's FP64 pair kernel
+    twin = None
+    if args.precision == "mixed" and not args.no_twin:
+        print("# fp64 twin", file=sys.stderr, flush=True)
+        dt_ = R.Disk(r, v, p, b, nbt)
+        dt_.set_tuning(use_graph=GRAPH)
+        it_ = R.Integrator(dt_, R.RD_RKF78, h0, adaptive=True, tol=args.tol)
+        it_.steps(args.warmup)
+        _, q0 = it_.work()
+        c0 = it_.counters()
+        dt_.sync()
+        t0 = time.perf_counter()
+        it_.steps(args.steps)
+        dt_.sync()
+        el_ = time.perf_counter() - t0
+        _, q1 = it_.work()
+        c1 = it_.counters()
+        ms64, pl64 = dt_.bench_force(5)
+        ach64 = FLOP_PER_PAIR * pl64 / (ms64 * 1e-3)
+        twin = {"rkf78_steps_per_s": (c1[1] - c0[1]) / el_, "pair_interactions_per_s": (q1 - q0) / el_,
+                "attempts": c1[0] - c0[0], "rejected": c1[2] - c0[2], "ms_per_step": el_ * 1e3 / args.steps,
+                "mixed_over_fp64_steps_per_s": steps_per_s / ((c1[1] - c0[1]) / el_),
+                "roofline": {"bound": "valu-fp64", "achieved": ach64 / 1e12, "peak": PEAK_FP64 / 1e12,
+                             "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64, "kernel": "k_force_exact",
+                             "ms_per_launch": ms64, "pairs_per_launch": pl64}}
+        del it_, dt_
+
+    # energy over 10^3 orbits at 1 AU: bounded runs, drift carried to the horizon
+    horizon = None
+    if args.precision == "mixed" and args.horizon_seconds > 0:
+        horizon = []
+        for tol_h in (args.tol, args.horizon_tol):
+            print(f"# energy horizon leg, tol {tol_h:g}", file=sys.stderr, flush=True)
+            horizon.append(energy_horizon_leg((r, v, p, b, nbt), tol_h, tuning, args.horizon_seconds, h0))
+
     out = {
         "metric": "pair interactions/sec + RKF78 steps/sec at rel. energy error <=1e-9, 1 GPU",
         "value": value,
@@ -571,8 +700,10 @@ def main():
         "rejected": failed1 - failed0,
         "force_evals": ev1 - ev0,
         "sim_time_days": (t_end - t_start) / R.K_GAUSS,
-        "rel_energy_error": rel_e,
-        "energy_ok": bool(rel_e <= 1e-9),
+        "rel_energy_error": rel_e_run,
+        "energy_ok": bool(rel_e_run <= 1e-9),
+        "energy_basis": f"|E/E0 - 1| over the workload's run: {run_steps} accepted steps from t = 0 "
+                        f"(timed steps alone: {rel_e:.2e})",
         "max_rel_force_err_vs_fp64": f_err,
         "final_state_vs_fp64": fs_diff,
         "roofline": {"bound": "valu-fp32", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
@@ -582,6 +713,9 @@ def main():
                      if (args.precision == "mixed" and args.n == 16384 and args.seed == 2) else None,
                      "kernel": pair_kernel(args.precision, tuning),
                      "ms_per_launch": ms, "pairs_per_launch": pairs_launch},
+        "roofline_stage": stage_rl,
+        "fp64_twin": twin,
+        "energy_horizon": horizon,
     }
     if not args.no_cpu:
         print("# cpu baseline", file=sys.stderr, flush=True)

@@ -178,6 +178,12 @@ int rd_bench_copy(int device, size_t bytes, uint32_t n_iter, double* ms_per_laun
  * kernel launch plus the stage kernel that folds its partial sums (what
  * rd_autotune minimises). */
 int rd_bench_force_stage(rd_disk* d, uint32_t n_iter, double* ms);
+/* Measurement helper without a reference counterpart: mean time of the stage
+ * kernels of one adaptive attempt of `method` (RD_RKF78: k_comb1 and the 13
+ * stage kernels; RD_RK4: k_comb1 and 5), with dt = 0 so the state is kept,
+ * folding the partial sums of one pair-kernel launch made first.  The HIP
+ * events bracket the launches on the engine's stream. */
+int rd_bench_stages(rd_disk* d, int method, uint32_t n_iter, double* ms);
 
 /* ---- integrators --------------------------------------------------------- */
 /* rungekutta4 / rungekutta8 constructors (int_rungekutta4.h:15, int_rungekutta8.h:18) */

@@ -1065,6 +1065,36 @@ int rd_bench_copy(int device, size_t bytes, uint32_t n_iter, double* ms)
     return RD_OK;
 }
 
+int rd_bench_stages(rd_disk* d, int method, uint32_t n_iter, double* ms)
+{
+    DeviceScope dev_scope(d ? d->device : -1);
+    if (!d || n_iter == 0 || !ms || (method != RD_RKF78 && method != RD_RK4)) return fail(RD_EINVAL, "bad argument");
+    if (int rc = ensure_lists(d)) return rc;
+    const int n_stage = method == RD_RKF78 ? 13 : 5;
+    // h = 0: every stage rewrites y_n's values, the state is untouched
+    for (uint32_t q = 0; q < d->n_disk; ++q) d->h_ctl[q] = DiskCtl{d->hd[q].t, 0.0, F_NEED_K0 | F_ACTIVE, 0};
+    HIP_TRY(hipMemcpyAsync(d->ctl, d->h_ctl, d->n_disk * sizeof(DiskCtl), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(enqueue_force(d, d->x, d->y, d->z, F_NEED_K0));   // partial sums for the stages to fold
+    auto attempt = [&]() -> hipError_t {
+        const StageArgs a0 = make_args(d, 0, 1, 0);
+        if (hipError_t e = launch_stage(d->stream, method, 0, a0)) return e;
+        if (hipError_t e = launch_comb1(d->stream, method, a0)) return e;
+        for (int r = 1; r < n_stage; ++r)
+            if (hipError_t e = launch_stage(d->stream, method, r, make_args(d, r, 1, 0))) return e;
+        return hipSuccess;
+    };
+    HIP_TRY(attempt());
+    EventPair ev;
+    HIP_TRY(ev.create());
+    HIP_TRY(hipEventRecord(ev.e0, d->stream));
+    for (uint32_t q = 0; q < n_iter; ++q) HIP_TRY(attempt());
+    HIP_TRY(hipEventRecord(ev.e1, d->stream));
+    float el = 0.0f;
+    HIP_TRY(ev.elapsed(&el));
+    *ms = (double)el / n_iter;
+    return RD_OK;
+}
+
 int rd_bench_force_stage(rd_disk* d, uint32_t n_iter, double* ms)
 {
     DeviceScope dev_scope(d ? d->device : -1);

@@ -74,6 +74,7 @@ _SIGS = {
     "rd_counters": (C.c_int, [_P, _u32, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)]),
     "rd_set_error_check_for_tp": (C.c_int, [_P, C.c_int]),
     "rd_integrator_set_guard": (C.c_int, [_P, _u32, _f64, C.c_int]),
+    "rd_bench_stages": (C.c_int, [_P, C.c_int, _u32, C.POINTER(_f64)]),
     "rd_work_counters": (C.c_int, [_P, C.POINTER(_u64), C.POINTER(_f64)]),
     "rd_run": (C.c_int, [_P, _f64, _f64, _u64]),
     "rd_check_ejection_hit_centrum": (C.c_int, [_P, _u32, C.POINTER(_u32)]),
@@ -296,6 +297,12 @@ class Disk:
     def bench_force_stage(self, n_iter=10):
         ms = _f64()
         check(lib.rd_bench_force_stage(self.h, n_iter, C.byref(ms)))
+        return ms.value
+
+    def bench_stages(self, method=None, n_iter=10):
+        """Mean ms of one adaptive attempt's stage kernels (rd_bench_stages)."""
+        ms = _f64()
+        check(lib.rd_bench_stages(self.h, RD_RKF78 if method is None else method, n_iter, C.byref(ms)))
         return ms.value
 
     def check_ejection_hit_centrum(self, k=0):
