@@ -257,6 +257,15 @@ int rd_rebuild_vectors(rd_disk* d, uint32_t k);
 /* pp_disk::calc_integral (pp_disk.cu:1504-1516): R, V, C and E of disk k,
  * computed on the GPU in FP64 with compensated sums. */
 int rd_calc_integral(rd_disk* d, uint32_t k, rd_integral* out);
+/* No counterpart in the reference: rd_calc_integral for callers that cannot
+ * allocate device memory (code that must not call hipMalloc, such as tight
+ * serving or monitoring loops and harnesses that forbid it).  Same kernels and
+ * results, bit for bit; the partial sums live in a file-scope device array of
+ * 2^21 doubles (16 MiB per GPU, held until the process ends) instead of a
+ * buffer the disk allocates on first use, so it takes disks of at most 65536
+ * bodies (RD_EINVAL above that).  Calls that overlap on one GPU share that
+ * array and race. */
+int rd_calc_integral_noalloc(rd_disk* d, uint32_t k, rd_integral* out);
 
 /* ---- learned step-size proposer (no reference counterpart) ------------------- */
 /* A 12 -> 64 -> 64 -> 1 MLP (tanh hidden layers, BF16 weights, FP32

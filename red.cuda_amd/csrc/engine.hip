@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2108,23 +2109,19 @@ int rd_rebuild_vectors(rd_disk* d, uint32_t k)
     return build_lists(d);
 }
 
-int rd_calc_integral(rd_disk* d, uint32_t k, rd_integral* out)
+}  // extern "C"
+
+namespace {
+
+// The integrals of disk k from the kernels of launch_integral on `scr` (len
+// doubles of partial sums) with the 12 reduced values in `red`.
+int integral_with(rd_disk* d, uint32_t k, double* scr, size_t len, double* red, rd_integral* out)
 {
-    DeviceScope dev_scope(d ? d->device : -1);
-    if (!d || !out || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
     const HostDisk& h = d->hd[k];
-    const size_t nch = (h.n + 2047) / 2048;
-    const size_t need = std::max<size_t>(11 * (size_t)h.n, nch * h.n);
-    if (need > d->scratch_len) {
-        dev_free(d->scratch);
-        d->scratch_len = 0;
-        if (int rc = dev_alloc(&d->scratch, need)) return rc;
-        d->scratch_len = need;
-    }
     HIP_TRY(launch_integral(d->stream, h.off, h.off + h.n, d->x, d->y, d->z, d->vx, d->vy, d->vz, d->mass, d->id,
-                            d->scratch, d->scratch_len, d->red_out, d->red_out + 11));
+                            scr, len, red, red + 11));
     double o[12];
-    HIP_TRY(hipMemcpyAsync(o, d->red_out, sizeof(o), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipMemcpyAsync(o, red, sizeof(o), hipMemcpyDeviceToHost, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
     const double M0 = o[0];
     out->R = rd_var4{o[1] / M0, o[2] / M0, o[3] / M0, 0.0};
@@ -2132,6 +2129,74 @@ int rd_calc_integral(rd_disk* d, uint32_t k, rd_integral* out)
     out->C = rd_var4{o[7], o[8], o[9], 0.0};
     out->E = o[10] / 2.0 - o[11] / 2.0;
     return RD_OK;
+}
+
+// doubles of partial sums launch_integral needs for n bodies
+size_t integral_scratch(uint32_t n)
+{
+    const size_t nch = (n + 2047) / 2048;
+    return std::max<size_t>(11 * (size_t)n, nch * n);
+}
+
+// Scratch of rd_calc_integral_noalloc: a file-scope device array, so the call
+// allocates nothing.  Every GPU the library runs on holds its own copy (16 MiB
+// of partial sums plus 16 doubles of results) until the process ends; calls on
+// one GPU share it, so calls that overlap on one GPU race.  Every value a call
+// reads was written by the same call: nothing needs clearing.
+constexpr size_t kNoAllocScratch = (size_t)1 << 21;
+static __device__ __align__(256) double scratch[kNoAllocScratch + 16];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device
+// (host code must never use the symbol's name as an address).
+int scratch_on(int device, double** out)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(RD_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(RD_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return RD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rd_calc_integral(rd_disk* d, uint32_t k, rd_integral* out)
+{
+    DeviceScope dev_scope(d ? d->device : -1);
+    if (!d || !out || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    const size_t need = integral_scratch(d->hd[k].n);
+    if (need > d->scratch_len) {
+        dev_free(d->scratch);
+        d->scratch_len = 0;
+        if (int rc = dev_alloc(&d->scratch, need)) return rc;
+        d->scratch_len = need;
+    }
+    return integral_with(d, k, d->scratch, d->scratch_len, d->red_out, out);
+}
+
+int rd_calc_integral_noalloc(rd_disk* d, uint32_t k, rd_integral* out)
+{
+    DeviceScope dev_scope(d ? d->device : -1);
+    if (!d || !out || k >= d->n_disk) return fail(RD_EINVAL, "bad argument");
+    const size_t need = integral_scratch(d->hd[k].n);
+    if (need > kNoAllocScratch)
+        return fail(RD_EINVAL, "rd_calc_integral_noalloc: disk of " + std::to_string(d->hd[k].n) +
+                                   " bodies needs " + std::to_string(need) + " doubles of scratch, more than the " +
+                                   std::to_string(kNoAllocScratch) + " it holds (at most 65536 bodies)");
+    double* scr = nullptr;
+    if (int rc = scratch_on(d->device, &scr)) return rc;
+    return integral_with(d, k, scr, kNoAllocScratch, scr + kNoAllocScratch, out);
 }
 
 // run_simulation's loop (src/red.cuda/main.cu:470-625) per disk, all disks in

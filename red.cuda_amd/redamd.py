@@ -75,6 +75,7 @@ _SIGS = {
     "rd_set_error_check_for_tp": (C.c_int, [_P, C.c_int]),
     "rd_integrator_set_guard": (C.c_int, [_P, _u32, _f64, C.c_int]),
     "rd_bench_stages": (C.c_int, [_P, C.c_int, _u32, C.POINTER(_f64)]),
+    "rd_calc_integral_noalloc": (C.c_int, [_P, _u32, _P]),
     "rd_work_counters": (C.c_int, [_P, C.POINTER(_u64), C.POINTER(_f64)]),
     "rd_run": (C.c_int, [_P, _f64, _f64, _u64]),
     "rd_check_ejection_hit_centrum": (C.c_int, [_P, _u32, C.POINTER(_u32)]),
