@@ -533,7 +533,13 @@ def main():
     ap.add_argument("--horizon-tol", type=float, default=3e-12,
                     help="config 2: the tighter tolerance of the second energy-horizon leg")
     ap.add_argument("--no-twin", action="store_true", help="config 2: skip the FP64 twin run")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="the timed steps only (no CPU leg, FP64 comparisons, twin, energy run or horizon): "
+                         "for profilers")
     args = ap.parse_args()
+    if args.profile_run:
+        args.no_cpu = args.no_final_state = args.no_twin = True
+        args.horizon_seconds = 0.0
     global GRAPH
     GRAPH = 0 if args.no_graph else 1
     if args.copy_bytes:
@@ -606,7 +612,7 @@ def main():
                               if (args.precision == "mixed" and args.seed == 2) else None)
 
     # the energy criterion over the workload's run: 1000 accepted steps from t = 0
-    run_steps = max(1000, passed1)
+    run_steps = passed1 if args.profile_run else max(1000, passed1)
     if passed1 < run_steps:
         print(f"# untimed steps {passed1} -> {run_steps} (energy over the run)", file=sys.stderr, flush=True)
         it.steps(run_steps - passed1)
@@ -614,7 +620,7 @@ def main():
 
     # force error of the mixed kernel against the FP64 reference kernel at the final state
     f_err = None
-    if args.precision == "mixed":
+    if args.precision == "mixed" and not args.profile_run:
         am = d.calc_dydx(1, 0, t_end, rr, vv)
         d.set_precision(R.RD_PREC_FP64)
         ae = d.calc_dydx(1, 0, t_end, rr, vv)

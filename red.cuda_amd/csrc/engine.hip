@@ -660,11 +660,13 @@ hipError_t enqueue_force(rd_disk* d, const double* fx, const double* fy, const d
     if (d->precision == RD_PREC_FP64)
         return launch_force_exact(d->stream, d->exact_lane ? 1 : 0, d->items_force, d->n_items_force, d->grid_force,
                                   fx, fy, fz, d->m_ex, d->ctl, need, d->part, d->n_all);
+    // mixed precision: FP32 slot totals, in float units of the same buffer
+    float* part32 = reinterpret_cast<float*>(d->part);
     if (hipError_t e = launch_force_sym(d->stream, d->sym_R, d->items_sym, d->n_items_sym, d->grid_sym, fx, fy, fz,
-                                        d->m32, d->ctl, need, d->part, d->n_all))
+                                        d->m32, d->ctl, need, part32, d->n_all))
         return e;
     return launch_force_mixed(d->stream, (int)d->tuning.sinks_per_lane, d->items_force, d->n_items_force,
-                              d->grid_force, fx, fy, fz, d->m32, d->ctl, need, d->part, d->n_all);
+                              d->grid_force, fx, fy, fz, d->m32, d->ctl, need, part32, d->n_all);
 }
 
 int last_stage(const rd_integrator* it) { return it->n_stage - 1; }

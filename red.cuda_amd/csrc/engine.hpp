@@ -6,7 +6,8 @@
 //   stage positions: P[2][3]                      FP64 SoA, double-buffered by stage parity
 //   stage k        : kv[r][3] (= velocity at stage r, the position derivative)
 //                    ka[r][3] (= acceleration at stage r)            r < 13
-//   pair partials  : part[(c*3 + comp) * n_all + i]                  FP64
+//   pair partials  : part[(c*3 + comp) * n_all + i]: FP64 from the FP64 pair
+//                    kernel (one slot), FP32 chunk totals in mixed precision
 //   sources        : m_ex (FP64 mass, -1 if inactive), m32 (FP32 mass, 0 if
 //                    inactive or summed in FP64)
 #pragma once
@@ -119,14 +120,14 @@ hipError_t launch_force_exact(hipStream_t s, int lane, const WorkItem* items, ui
                               const DiskCtl* ctl, uint32_t need, double* part, uint32_t n_all);
 hipError_t launch_force_mixed(hipStream_t s, int sinks_per_lane, const WorkItem* items, uint32_t n_items,
                               uint32_t grid, const double* x, const double* y, const double* z,
-                              const float* m32, const DiskCtl* ctl, uint32_t need_mask, double* part,
+                              const float* m32, const DiskCtl* ctl, uint32_t need_mask, float* part,
                               uint32_t n_all);
 // Symmetric pair kernel over (chunk a, chunk c) items: sink_lo/hi = chunk a,
 // src_lo/hi = chunk c (256-body blocks, R per chunk), slot = c (sink side),
 // src_slot = a (source side).
 hipError_t launch_force_sym(hipStream_t s, uint32_t R, const WorkItem* items, uint32_t n_items, uint32_t grid,
                             const double* x, const double* y, const double* z, const float* m32,
-                            const DiskCtl* ctl, uint32_t need_mask, double* part, uint32_t n_all);
+                            const DiskCtl* ctl, uint32_t need_mask, float* part, uint32_t n_all);
 size_t force_sym_lds_bytes(uint32_t R);
 hipError_t launch_stage(hipStream_t s, int method, int r, const StageArgs& a);
 hipError_t launch_comb1(hipStream_t s, int method, const StageArgs& a);

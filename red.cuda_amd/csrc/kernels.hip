@@ -223,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_force_mixed(const WorkItem* __restri
                                                         const double* __restrict__ z,
                                                         const float* __restrict__ m32,
                                                         const DiskCtl* __restrict__ ctl, uint32_t need,
-                                                        double* __restrict__ part, uint32_t n_all)
+                                                        float* __restrict__ part, uint32_t n_all)
 {
     __shared__ SrcF tile[kTile];
     __shared__ double red[3][S][3][64];
@@ -307,9 +307,9 @@ __global__ __launch_bounds__(kBlock) void k_force_mixed(const WorkItem* __restri
                     const double sx = ((ax[s] + red[0][s][0][lane]) + red[1][s][0][lane]) + red[2][s][0][lane];
                     const double sy = ((ay[s] + red[0][s][1][lane]) + red[1][s][1][lane]) + red[2][s][1][lane];
                     const double sz = ((az[s] + red[0][s][2][lane]) + red[1][s][2][lane]) + red[2][s][2][lane];
-                    part[(size_t)(w.slot * 3 + 0) * n_all + i] = sx;
-                    part[(size_t)(w.slot * 3 + 1) * n_all + i] = sy;
-                    part[(size_t)(w.slot * 3 + 2) * n_all + i] = sz;
+                    part[(size_t)(w.slot * 3 + 0) * n_all + i] = (float)sx;
+                    part[(size_t)(w.slot * 3 + 1) * n_all + i] = (float)sy;
+                    part[(size_t)(w.slot * 3 + 2) * n_all + i] = (float)sz;
                 }
             }
         }
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict
                                                       const double* __restrict__ x, const double* __restrict__ y,
                                                       const double* __restrict__ z, const float* __restrict__ m32,
                                                       const DiskCtl* __restrict__ ctl, uint32_t need,
-                                                      double* __restrict__ part, uint32_t n_all)
+                                                      float* __restrict__ part, uint32_t n_all)
 {
     extern __shared__ __align__(16) unsigned char sym_lds[];
     double2* txy = reinterpret_cast<double2*>(sym_lds);
@@ -565,9 +565,9 @@ __global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict
                         ab[kSymBlock + k] += sy;
                         ab[2 * kSymBlock + k] += sz;
                     } else if (i0 + k < w.sink_hi) {
-                        part[(size_t)(w.slot * 3 + 0) * n_all + i0 + k] = sx;
-                        part[(size_t)(w.slot * 3 + 1) * n_all + i0 + k] = sy;
-                        part[(size_t)(w.slot * 3 + 2) * n_all + i0 + k] = sz;
+                        part[(size_t)(w.slot * 3 + 0) * n_all + i0 + k] = (float)sx;
+                        part[(size_t)(w.slot * 3 + 1) * n_all + i0 + k] = (float)sy;
+                        part[(size_t)(w.slot * 3 + 2) * n_all + i0 + k] = (float)sz;
                     }
                 }
             }
@@ -578,9 +578,9 @@ __global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict
             const uint32_t j = w.src_lo + q;
             if (j < w.src_hi) {
                 const double* ab = acc + (size_t)(q / kSymBlock) * 3 * kSymBlock + q % kSymBlock;
-                part[(size_t)(w.src_slot * 3 + 0) * n_all + j] = ab[0];
-                part[(size_t)(w.src_slot * 3 + 1) * n_all + j] = ab[kSymBlock];
-                part[(size_t)(w.src_slot * 3 + 2) * n_all + j] = ab[2 * kSymBlock];
+                part[(size_t)(w.src_slot * 3 + 0) * n_all + j] = (float)ab[0];
+                part[(size_t)(w.src_slot * 3 + 1) * n_all + j] = (float)ab[kSymBlock];
+                part[(size_t)(w.src_slot * 3 + 2) * n_all + j] = (float)ab[2 * kSymBlock];
             }
         }
         __syncthreads();
@@ -589,7 +589,7 @@ __global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict
 
 hipError_t launch_force_sym(hipStream_t st, uint32_t R, const WorkItem* items, uint32_t n_items, uint32_t grid,
                             const double* x, const double* y, const double* z, const float* m32,
-                            const DiskCtl* ctl, uint32_t need, double* part, uint32_t n_all)
+                            const DiskCtl* ctl, uint32_t need, float* part, uint32_t n_all)
 {
     if (n_items == 0) return hipSuccess;
     if (R < 1 || R > kSymMaxR) return hipErrorInvalidValue;
@@ -616,7 +616,7 @@ hipError_t launch_force_exact(hipStream_t st, int lane, const WorkItem* items, u
 
 hipError_t launch_force_mixed(hipStream_t st, int S, const WorkItem* items, uint32_t n_items, uint32_t grid,
                               const double* x, const double* y, const double* z, const float* m32,
-                              const DiskCtl* ctl, uint32_t need, double* part, uint32_t n_all)
+                              const DiskCtl* ctl, uint32_t need, float* part, uint32_t n_all)
 {
     if (n_items == 0) return hipSuccess;
     switch (S) {
@@ -702,49 +702,64 @@ __device__ inline const double* ka_of(const StageArgs& a, int r, int q)
 __device__ inline double* kv_w(const StageArgs& a, int r, int q) { return a.K + ((size_t)r * 6 + q) * a.n_all; }
 __device__ inline double* ka_w(const StageArgs& a, int r, int q) { return a.K + ((size_t)r * 6 + 3 + q) * a.n_all; }
 
-// Sum of the pair partials of body i, component q, in slot order.
-__device__ double partial_sum(const StageArgs& a, uint32_t i, int q, uint32_t nc)
+// Sum of the pair partials of body i, component q, in slot order, in FP64.
+// The FP64 pair kernel leaves one FP64 slot; the mixed-precision kernels leave
+// one FP32 slot per source chunk (the chunk's total, formed in FP64 and
+// rounded once), at the same index in float units.  Loads go out in batches
+// of 128 bytes per lane (16 doubles or 32 floats); a slot index past the last
+// is clamped to it, so every load is unconditional and in bounds and only the
+// sum skips it (a guarded load would become a branch and a wait per slot).
+template <typename T>
+__device__ double partial_sum_t(const T* __restrict__ p, size_t st, uint32_t nc)
 {
-    const double* __restrict__ p = a.part + (size_t)q * a.n_all + i;
-    const size_t st = (size_t)3 * a.n_all;
+    constexpr uint32_t kB = 128 / sizeof(T);
     double s = 0.0;
-    uint32_t c = 0;
-    for (; c + 16 <= nc; c += 16) {
-        double v[16];
+    for (uint32_t c = 0; c < nc; c += kB) {
+        T v[kB];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = p[(size_t)(c + u) * st];
+        for (uint32_t u = 0; u < kB; ++u) v[u] = p[(size_t)min(c + u, nc - 1) * st];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) s = (c + u == 0) ? v[u] : s + v[u];
-    }
-    if (c < nc) {
-        double v[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = (c + u < nc) ? p[(size_t)(c + u) * st] : 0.0;
-#pragma unroll
-        for (int u = 0; u < 16; ++u)
-            if (c + u < nc) s = (c + u == 0) ? v[u] : s + v[u];
+        for (uint32_t u = 0; u < kB; ++u)
+            s = (c + u >= nc) ? s : ((c + u == 0) ? (double)v[u] : s + (double)v[u]);
     }
     return s;
+}
+
+__device__ double partial_sum(const StageArgs& a, uint32_t i, int q, uint32_t nc)
+{
+    const size_t off = (size_t)q * a.n_all + i, st = (size_t)3 * a.n_all;
+    if (a.mixed) return partial_sum_t(reinterpret_cast<const float*>(a.part) + off, st, nc);
+    return partial_sum_t(a.part + off, st, nc);
 }
 
 // partial_sum for a launch over one disk: the loop runs to ncmax, the
 // disk's largest slot count (uniform, from the kernel arguments), and each
 // lane adds its own nc slots in slot order.  The first batch of loads then
 // waits for nothing.  Slots in [nc, ncmax) lie inside the partial buffer,
-// which holds the largest slot count of any body.
-__device__ double partial_sum_upto(const StageArgs& a, uint32_t i, int q, uint32_t nc, uint32_t ncmax)
+// which holds the largest slot count of any body.  A disk's stage kernels
+// are few waves (one per 64 bodies and component), so a batch as wide as the
+// register budget allows costs no occupancy and saves round trips.
+template <typename T, uint32_t kB>
+__device__ double partial_sum_upto_t(const T* __restrict__ p, size_t st, uint32_t nc, uint32_t ncmax)
 {
-    const double* __restrict__ p = a.part + (size_t)q * a.n_all + i;
-    const size_t st = (size_t)3 * a.n_all;
     double s = 0.0;
-    for (uint32_t c = 0; c < ncmax; c += 16) {
-        double v[16];
+    for (uint32_t c = 0; c < ncmax; c += kB) {
+        T v[kB];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = (c + u < ncmax) ? p[(size_t)(c + u) * st] : 0.0;
+        for (uint32_t u = 0; u < kB; ++u) v[u] = p[(size_t)min(c + u, ncmax - 1) * st];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) s = (c + u >= nc) ? s : ((c + u == 0) ? v[u] : s + v[u]);
+        for (uint32_t u = 0; u < kB; ++u)
+            s = (c + u >= nc) ? s : ((c + u == 0) ? (double)v[u] : s + (double)v[u]);
     }
     return s;
+}
+
+__device__ double partial_sum_upto(const StageArgs& a, uint32_t i, int q, uint32_t nc, uint32_t ncmax)
+{
+    const size_t off = (size_t)q * a.n_all + i, st = (size_t)3 * a.n_all;
+    // one batch of 72 FP32 slots covers a 65-chunk disk (16385 bodies) at once
+    if (a.mixed) return partial_sum_upto_t<float, 72>(reinterpret_cast<const float*>(a.part) + off, st, nc, ncmax);
+    return partial_sum_upto_t<double, 16>(a.part + off, st, nc, ncmax);
 }
 
 // Mixed mode: the FP64 prefix sources' (the star block's) acceleration on
