@@ -9,7 +9,13 @@ Restated from the reference, step by step:
   red_random             distribution.cpp:11-31 (MT19937, one word per call)
   distributions          distribution.cpp:33-204
   presets                main.cu:846-1168 (Two_body, n_gp, n_pp, n_spl, n_pl,
-                         n_tp), main.cu:571-625 + 1804-1917 (Dvorak), nebula.cpp
+                         n_tp), main.cu:571-625 + 1804-1917 (Dvorak), nebula.cpp,
+                         main.cu:1467-1688 (Chambers2001), 505-569 + 1690-1802
+                         (coll_stat_run), 626-700 + 1917-2003 (Hansen_2009),
+                         456-503 + 2863-2911 (n_pl_to_test_anal_gd), 41-249 +
+                         438-454 + 2913-2961 (solar_system), 1412-1461 +
+                         3055-3153 (Colliding_bodies); util_init.cpp:327-366
+                         (the HORIZONS columns)
   populate_disk          main.cu:251-314, util_init.cpp:56-145
   calculate_phase        main.cu:317-342, tools.cpp:715-828, 928-933
   barycentre and units   tools.cpp:351-453
@@ -32,6 +38,12 @@ KM_TO_AU = 1.0 / (1.495978707e11 / 1.0e3)
 R_SUN = 1.0 / 215.094
 EARTH = 1.0 / 3.3294605e5
 SATURN = 1.0 / 3.497898e3
+MERCURY = 1.0 / 6.023600e6
+VENUS = 1.0 / 4.0852371e5
+MARS = 1.0 / 3.098708e6
+JUPITER = 1.0 / 1.0473486e3
+URANUS = 1.0 / 2.290298e4
+NEPTUNE = 1.0 / 1.941224e4
 MOON = 1.0 / 2.7071155e7
 CERES = 1.0 / 2.109342e9
 RHO = GRAM_TO_SOLAR / ((1.0e-2 * METER_TO_AU) * (1.0e-2 * METER_TO_AU) * (1.0e-2 * METER_TO_AU))
@@ -167,6 +179,91 @@ def solid_mass(r_1, r_2, r_SL, f_neb, Sigma_1, f_ice, p):
 
 
 DVORAK_NEB = (0.9, 2.7, 2.7, 3.0, 7.0 * SIGMA, 4.2, -3.0 / 2.0)
+CHAMBERS_NEB = (0.9, 2.7, 2.7, 1.0, 8.0 * SIGMA, 4.2, -3.0 / 2.0)
+COLL_STAT_NEB = (2.5, 3.0, 2.7, 1.0, 7.0 * SIGMA, 4.2, -3.0 / 2.0)
+
+# Elements of the planets on 2015-May-11 (JPL HORIZONS), in the order of the
+# columns util_init.cpp:327-366 reads: e, i, Omega, omega, M (degrees), a (AU)
+HORIZONS = {
+    "Mercury": (2.056283170656704E-01, 7.004033081870772E+00, 4.831135275416084E+01, 2.917018579009503E+01,
+                8.665226794919347E+01, 3.870990540148295E-01),
+    "Venus": (6.761247965610893E-03, 3.394467689974794E+00, 7.663950855912078E+01, 5.463888630600258E+01,
+              3.625130289912041E+01, 7.233287920706470E-01),
+    "Earth": (1.706151645376220E-02, 2.832604859699792E-03, 1.991276585462631E+02, 2.644902808333621E+02,
+              1.245850770307868E+02, 9.999423845001243E-01),
+    "Mars": (9.345598920376896E-02, 1.848403968432629E+00, 4.951276588949865E+01, 2.865318030175783E+02,
+             7.844645806929105E+01, 1.523589291796774E+00),
+    "Jupiter": (4.895926045495331E-02, 1.303927595561599E+00, 1.005220570364609E+02, 2.736235124666692E+02,
+                1.262463884135522E+02, 5.201472759810757E+00),
+    "Saturn": (5.370451604891303E-02, 2.486273622454862E+00, 1.136214918787423E+02, 3.402344971420399E+02,
+               1.437153605146956E+02, 9.553843040430950E+00),
+    "Uranus": (4.928987553618186E-02, 7.717997843738144E-01, 7.400078543435393E+01, 9.677390970582711E+01,
+               2.083879289018873E+02, 1.915532588905842E+01),
+    "Neptune": (8.258758489207289E-03, 1.765657591232885E+00, 1.317164628477979E+02, 2.932421741299731E+02,
+                2.734992366503029E+02, 2.997852567031729E+01),
+}
+
+
+def horizons_oe(name):
+    """[sma, ecc, inc, peri, node, mean] with the degrees in radians."""
+    e, i, node, peri, mean, a = HORIZONS[name]
+    return [a, e, i * DEG, peri * DEG, node * DEG, mean * DEG]
+
+
+def _dvorak_types(neb):
+    return [(STAR, 1, 0, _star()),
+            (PROTO, 300, 1, [("sma", "power_law", neb[0], neb[1], neb[6]), ("ecc", "rayleigh", 0.01),
+                             ("inc", "uniform", 0.0, 0.0), ("peri", "uniform", 0.0, 2.0 * PI),
+                             ("node", "uniform", 0.0, 0.0), ("mean", "uniform", 0.0, 2.0 * PI),
+                             ("mass", "lognormal", 0.1, 3.0, 0.0, 0.25),
+                             ("density", "uniform", 2.7 * RHO, 2.7 * RHO), ("cd", "uniform", 0.0, 0.0)])]
+
+
+def _hansen_post(bodies):
+    """main.cu:1955-1965: the first giant becomes Jupiter."""
+    for b in bodies[1:]:
+        if b["type"] == GIANT:
+            b["name"] = "Jupiter"
+            b["mass"] = 1.0 * JUPITER
+            b["radius"] = 71492.0 * KM_TO_AU
+            b["density"] = calc_density(b["mass"], b["radius"])
+            b["oe"] = horizons_oe("Jupiter")
+            return
+
+
+def _colliding_post(bodies):
+    """main.cu:3090-3115: target and projectile on one orbit."""
+    k = 0
+    for b in bodies[1:]:
+        if b["type"] != PROTO or k >= 2:
+            continue
+        b["name"] = "target" if k == 0 else "projectile"
+        b["oe"] = [2.7, 0.05, 0.0, 0.0, 0.0, 0.0 if k == 0 else 0.01 * DEG]
+        k += 1
+
+
+def _solar_system_bodies():
+    """populate_solar_system (main.cu:85-249): Sun, then Mercury..Mars (rocky),
+    then Jupiter..Neptune (giant), ids from 1, cd 0."""
+    rows = [("Mercury", ROCKY, MERCURY, 2439.7), ("Venus", ROCKY, VENUS, 6051.8), ("Earth", ROCKY, EARTH, 6371.0),
+            ("Mars", ROCKY, MARS, 3389.5), ("Jupiter", GIANT, JUPITER, 71492.0), ("Saturn", GIANT, SATURN, 60268.0),
+            ("Uranus", GIANT, URANUS, 25559.0), ("Neptune", GIANT, NEPTUNE, 24766.0)]
+    sun = {"id": 1, "type": STAR, "name": "Sun", "mass": 1.0, "radius": 1.0 * R_SUN, "cd": 0.0, "oe": [0.0] * 6}
+    sun["density"] = calc_density(sun["mass"], sun["radius"])
+    bodies = [sun]
+    for k, (name, btype, m, r_km) in enumerate(rows):
+        b = {"id": k + 2, "type": btype, "name": name, "mass": 1.0 * m, "radius": r_km * KM_TO_AU, "cd": 0.0,
+             "oe": horizons_oe(name)}
+        b["density"] = calc_density(b["mass"], b["radius"])
+        bodies.append(b)
+    return bodies
+
+
+def _chambers_names():
+    """set_parameters::Chambers2001 pushes star + proto1..proto153 before
+    Dvorak's star + proto1..proto300 (main.cu:402-433, 590-621)."""
+    return (["star"] + ["proto" + str(i) for i in range(1, 154)] + ["star"]
+            + ["proto" + str(i) for i in range(1, 301)])
 
 # name -> (types [(body type, count, name offset, [(slot, kind, args...)] in construction order)],
 #          recompute pass, scale to the MMSN solid mass)
@@ -208,6 +305,64 @@ PRESETS = {
                                  ("density", "uniform", 2.7 * RHO, 2.7 * RHO), ("cd", "uniform", 0.0, 0.0)])],
                True, True),
 }
+
+# The scenarios whose drivers do more than populate + calculate_phase: MMSN
+# mass ("scale" or "share"), the nebula, the reference's name list, the
+# driver's changes before calculate_phase, or a table of bodies.
+EXTRA = {
+    "Chambers2001": dict(types=_dvorak_types(CHAMBERS_NEB), recompute=True, mmsn="scale", neb=CHAMBERS_NEB,
+                         names="chambers"),
+    "coll_stat_run": dict(types=[(STAR, 1, 0, _star()),
+                                 (PROTO, 1000, 1, [("sma", "power_law", 2.5, 3.0, -3.0 / 2.0),
+                                                   ("ecc", "rayleigh", 0.02), ("inc", "rayleigh", 0.04),
+                                                   ("peri", "uniform", 0.0, 2.0 * PI),
+                                                   ("node", "uniform", 0.0, 2.0 * PI),
+                                                   ("mean", "uniform", 0.0, 2.0 * PI),
+                                                   ("mass", "uniform", CERES, CERES),
+                                                   ("density", "uniform", 2.5 * RHO, 2.5 * RHO),
+                                                   ("cd", "uniform", 0.0, 0.0)])],
+                          recompute=True, mmsn="share", neb=COLL_STAT_NEB),
+    "Hansen_2009": dict(types=[(STAR, 1, 0, _star()),
+                               (GIANT, 1, 1, [("sma", "uniform", 5.2, 5.2), ("ecc", "uniform", 0.05, 0.05),
+                                              ("inc", "uniform", 0.0, 0.0), ("peri", "uniform", 0.0, 2.0 * PI),
+                                              ("node", "uniform", 0.0, 0.0), ("mean", "uniform", 0.0, 2.0 * PI),
+                                              ("mass", "uniform", JUPITER, JUPITER),
+                                              ("density", "uniform", 1.326 * RHO, 1.326 * RHO),
+                                              ("cd", "uniform", 0.0, 0.0)]),
+                               (PROTO, 400, 1, [("sma", "uniform", 0.7, 1.0), ("ecc", "uniform", 0.0, 0.0),
+                                                ("inc", "uniform", 0.0, 0.5 * DEG), ("peri", "uniform", 0.0, 0.0),
+                                                ("node", "uniform", 0.0, 2.0 * PI),
+                                                ("mean", "uniform", 0.0, 2.0 * PI),
+                                                ("mass", "uniform", 5.0e-3 * EARTH, 5.0e-3 * EARTH),
+                                                ("density", "uniform", 2.7 * RHO, 2.7 * RHO),
+                                                ("cd", "uniform", 0.0, 0.0)])],
+                        post=_hansen_post),
+    "n_pl_to_test_anal_gd": dict(types=[(STAR, 1, 0, _star()),
+                                        (PL, 10, 0, [("sma", "uniform", 1.0, 10.0), ("ecc", "uniform", 0.0, 0.0),
+                                                     ("inc", "uniform", 0.0, 0.0), ("peri", "uniform", 0.0, 0.0),
+                                                     ("node", "uniform", 0.0, 0.0), ("mean", "uniform", 0.0, 0.0),
+                                                     ("radius", "uniform", METER_TO_AU, METER_TO_AU),
+                                                     ("density", "uniform", 2.7 * RHO, 2.7 * RHO),
+                                                     ("cd", "uniform", 1.0, 1.0)])]),
+    "solar_system": dict(table=_solar_system_bodies),
+    "Colliding_bodies": dict(types=[(STAR, 1, 0, _star()),
+                                    (PROTO, 2, 0, [("sma", "uniform", 1.0, 2.0), ("ecc", "uniform", 0.0, 0.0),
+                                                   ("inc", "uniform", 0.0, 0.0), ("peri", "uniform", 0.0, 0.0),
+                                                   ("node", "uniform", 0.0, 0.0), ("mean", "uniform", 0.0, 0.0),
+                                                   ("mass", "uniform", CERES, CERES),
+                                                   ("density", "uniform", 2.7 * RHO, 2.7 * RHO),
+                                                   ("cd", "uniform", 0.0, 0.0)])],
+                             post=_colliding_post),
+}
+
+
+def scenario(name):
+    """The generation recipe of a preset, as a dict."""
+    if name in EXTRA:
+        return EXTRA[name]
+    types, recompute, scale = PRESETS[name]
+    return dict(types=types, recompute=recompute, mmsn="scale" if scale else None, neb=DVORAK_NEB)
+
 
 FOUR_PI_3 = 4.188790204786391
 
@@ -270,7 +425,14 @@ def calc_phase(mu, oe):
 
 def generate(name, seed):
     """Bodies (list of dicts), t0, dt and body counts per type, in red's units."""
-    types, recompute, scale = PRESETS[name]
+    sc = scenario(name)
+    if "table" in sc:
+        bodies = sc["table"]()
+        n_type = [0] * 7
+        for b in bodies:
+            n_type[b["type"]] += 1
+        return _phase_bc_units(bodies, n_type)
+    types, recompute, mmsn = sc["types"], sc.get("recompute", False), sc.get("mmsn")
     rnd = crand(seed)
     dists = []
     for (_, _, _, entries) in types:
@@ -305,8 +467,8 @@ def generate(name, seed):
             b["k"] = k
             bodies.append(b)
             bid += 1
-    if scale:
-        m_solid = solid_mass(*DVORAK_NEB)
+    if mmsn:
+        m_solid = solid_mass(*sc["neb"])
 
         def total():
             m = 0.0
@@ -314,10 +476,16 @@ def generate(name, seed):
                 if b["type"] == PROTO and b["id"] > 0:
                     m += b["mass"]
             return m
-        f = m_solid / total()
-        for b in bodies:
-            if b["type"] == PROTO:
-                b["mass"] *= f
+        if mmsn == "scale":
+            f = m_solid / total()
+            for b in bodies:
+                if b["type"] == PROTO:
+                    b["mass"] *= f
+        else:
+            m_pp = m_solid / n_type[PROTO]
+            for b in bodies:
+                if b["type"] == PROTO:
+                    b["mass"] = m_pp
         if abs(total() - m_solid) > 1.0e-15:
             return None
     if recompute:
@@ -328,6 +496,16 @@ def generate(name, seed):
                     b["density"] = calc_density(b["mass"], b["radius"])
                 if "radius" not in D and "density" in D:
                     b["radius"] = calc_radius(b["mass"], b["density"])
+    if sc.get("names") == "chambers":
+        for b, nm in zip(bodies, _chambers_names()):
+            b["name"] = nm
+    if sc.get("post"):
+        sc["post"](bodies)
+    return _phase_bc_units(bodies, n_type)
+
+
+def _phase_bc_units(bodies, n_type):
+    """calculate_phase (main.cu:317-342), the barycentre and k-day units."""
     m0 = bodies[0]["mass"]
     bodies[0]["r"], bodies[0]["v"] = [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]
     min_P = 1.7976931348623157e308

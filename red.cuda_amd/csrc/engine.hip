@@ -440,7 +440,13 @@ void fill_info(rd_disk* d)
         di.n = h.n;
         di.n_si = h.n_si();
         di.n_massive = h.n_massive();
-        di.n_fp64 = d->precision == RD_PREC_MIXED ? h.playing[0] + h.playing[1] : 0;
+        // mixed precision sums the leading stars and giants in FP64 (the
+        // types' counts when the SI class is sorted by type)
+        uint32_t lead = 0;
+        while (lead < h.n && (d->bmd[h.off + lead].body_type == RD_BODY_STAR ||
+                              d->bmd[h.off + lead].body_type == RD_BODY_GIANTPLANET))
+            ++lead;
+        di.n_fp64 = d->precision == RD_PREC_MIXED ? lead : 0;
         if (d->drag_mode == RD_DRAG_NSI) {
             di.drag_lo = h.n_si();
             di.drag_hi = h.n_si() + h.n_nsi();
@@ -749,18 +755,34 @@ int rd_device_count(int* n)
     return RD_OK;
 }
 
+// Each interaction class -- SI (star, giant, rocky, protoplanet), NSI
+// (super-planetesimal, planetesimal), NI (test particle) -- must be one
+// contiguous block, in that class order, and the bodies' types must count up
+// to n_by_type.  Within a class the order is free: the reference's bounds are
+// class ranges (red_type.h:670-700), and its own solar-system generator
+// writes the rocky planets before the giants (src/red.cuda.initial/main.cu:
+// 85-249).
 static int validate_disk(const uint32_t* nbt, const rd_body_md* bmd, uint32_t n)
 {
     uint32_t tot = 0;
     for (int t = 0; t < kNType; ++t) tot += nbt[t];
     if (tot != n) return fail(RD_EINVAL, "n_by_type does not sum to the body count");
     if (n == 0) return fail(RD_EINVAL, "a disk needs at least one body");
-    uint32_t i = 0;
+    uint32_t cnt[kNType] = {0};
+    int prev = 0;
+    for (uint32_t q = 0; q < n; ++q) {
+        const int t = bmd[q].body_type;
+        if (t < 0 || t >= kNType) return fail(RD_EINVAL, "invalid body_type (body " + std::to_string(q) + ")");
+        const int cls = t <= RD_BODY_PROTOPLANET ? 0 : (t <= RD_BODY_PLANETESIMAL ? 1 : 2);
+        if (cls < prev)
+            return fail(RD_EINVAL, "bodies must be grouped by interaction class (SI, NSI, NI), in that order (body " +
+                                       std::to_string(q) + ")");
+        prev = cls;
+        cnt[t]++;
+    }
     for (int t = 0; t < kNType; ++t)
-        for (uint32_t q = 0; q < nbt[t]; ++q, ++i)
-            if (bmd[i].body_type != t)
-                return fail(RD_EINVAL, "bodies must be sorted by body_type with the counts of n_by_type (body " +
-                                           std::to_string(i) + ")");
+        if (cnt[t] != nbt[t])
+            return fail(RD_EINVAL, "n_by_type does not match the body types (type " + std::to_string(t) + ")");
     for (uint32_t q = 0; q < n; ++q)
         if (bmd[q].id == 0) return fail(RD_EINVAL, "body id 0 is not allowed (ids are > 0, negative = inactive)");
     return RD_OK;

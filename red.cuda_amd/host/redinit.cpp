@@ -58,6 +58,12 @@ constexpr double GramToSolar = 1.0 / SolarToGram;
 constexpr double MoonToSolar = 1.0 / 2.7071155e7;
 constexpr double EarthToSolar = 1.0 / 3.3294605e5;
 constexpr double SaturnToSolar = 1.0 / 3.497898e3;
+constexpr double MercuryToSolar = 1.0 / 6.023600e6;
+constexpr double VenusToSolar = 1.0 / 4.0852371e5;
+constexpr double MarsToSolar = 1.0 / 3.098708e6;
+constexpr double JupiterToSolar = 1.0 / 1.0473486e3;
+constexpr double UranusToSolar = 1.0 / 2.290298e4;
+constexpr double NeptuneToSolar = 1.0 / 1.941224e4;
 constexpr double CeresToSolar = 1.0 / 2.109342e9;
 constexpr double AuToMeter = 1.495978707e11;
 constexpr double AuToKilometer = AuToMeter / 1.0e3;
@@ -197,13 +203,29 @@ struct Nebula {                  // nebula.cpp: solid component of a MMSN
     }
 };
 
+struct Disk;
+
+enum MmsnMass { MMSN_NONE = 0, MMSN_SCALE = 1, MMSN_SHARE = 2 };
+
 struct Preset {
+    Preset(std::string n, std::vector<TypeSpec> t = {}, bool rc = false, int mm = MMSN_NONE, Nebula nb = Nebula{})
+        : name(std::move(n)), types(std::move(t)), recompute(rc), mmsn(mm), neb(nb)
+    {
+    }
     std::string name;
     std::vector<TypeSpec> types;
     bool recompute = false;      // the driver's "physical quantities with the new mass" pass
-    bool scale_to_mmsn = false;  // Dvorak: protoplanet masses scaled to the solid mass
+    int mmsn = MMSN_NONE;        // protoplanet masses: scaled to the solid mass (Dvorak), or its equal shares
     Nebula neb{};
+    std::vector<std::string> names;   // the reference's name list when it is not the types' own
+    void (*post)(Disk&) = nullptr;    // the driver's own changes before calculate_phase
+    bool table = false;          // bodies from a table (no distributions): the solar system
 };
+
+// The drivers' own changes between populate_disk and calculate_phase.
+void hansen_post(Disk& D);
+void colliding_post(Disk& D);
+std::string create_name(int i, int type);
 
 // The presets (set_parameters:: in main.cu): body counts and distributions.
 std::vector<Preset> presets()
@@ -279,14 +301,14 @@ std::vector<Preset> presets()
                    generic_oe({{MASS, K::UNIFORM, 0.0, 0.0, 0, 0}, {RADIUS, K::UNIFORM, 0.0, 0.0, 0, 0},
                                {DENSITY, K::UNIFORM, 0.0, 0.0, 0, 0}, {CD, K::UNIFORM, 0.0, 0.0, 0, 0}})}},
                  false});
-    // Dvorak (main.cu:571-625, driver 1804-1917): 300 protoplanets in a MMSN
-    // solid disk of 0.9-2.7 AU, Sigma_1 = 7 g/cm^2, f_neb = 3, p = -3/2
-    {
-        Preset p{"Dvorak", {}, true, true,
-                 Nebula{0.9, 2.7, 2.7, 3.0, 7.0 * cst::GramPerCm2ToSolarPerAu2, 4.2, -3.0 / 2.0}};
+    // set_parameters::Dvorak (main.cu:571-625): 300 protoplanets on the MMSN
+    // solid disk `neb` (the power law of a takes its r_1, r_2 and p)
+    const double sigma = cst::GramPerCm2ToSolarPerAu2;
+    auto dvorak = [&](const std::string& name, const Nebula& neb) {
+        Preset p{name, {}, true, MMSN_SCALE, neb};
         p.types = {star,
                    {PROTO, 300, 1,
-                    {{SMA, K::POWER_LAW, p.neb.r_1, p.neb.r_2, p.neb.p, 0},
+                    {{SMA, K::POWER_LAW, neb.r_1, neb.r_2, neb.p, 0},
                      {ECC, K::RAYLEIGH, 0.01, 0, 0, 0},
                      {INC, K::UNIFORM, 0.0, 0.0, 0, 0},
                      {PERI, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
@@ -295,6 +317,98 @@ std::vector<Preset> presets()
                      {MASS, K::LOGNORMAL, 0.1, 3.0, 0.0, 0.25},
                      {DENSITY, K::UNIFORM, 2.7 * rho, 2.7 * rho, 0, 0},
                      {CD, K::UNIFORM, 0.0, 0.0, 0, 0}}}};
+        return p;
+    };
+    // Dvorak (driver main.cu:1804-1917): 0.9-2.7 AU, Sigma_1 = 7 g/cm^2, f_neb = 3, p = -3/2
+    v.push_back(dvorak("Dvorak", Nebula{0.9, 2.7, 2.7, 3.0, 7.0 * sigma, 4.2, -3.0 / 2.0}));
+    // Chambers2001 (main.cu:1467-1688).  The disk it prints is the last one it
+    // builds: Dvorak's parameters on a 0.9-2.7 AU solid disk with Sigma_1 =
+    // 8 g/cm^2 and f_neb = 1.  Its earlier set_parameters::Chambers2001 call
+    // (main.cu:382-436) seeds rand() with the same clock second, so Dvorak's
+    // setter draws the same rand() values; but it pushed its own 154 names
+    // first, so the printed names are those 154 followed by Dvorak's.
+    {
+        Preset p = dvorak("Chambers2001", Nebula{0.9, 2.7, 2.7, 1.0, 8.0 * sigma, 4.2, -3.0 / 2.0});
+        for (int round = 0; round < 2; ++round) {
+            p.names.push_back("star");
+            for (int i = 1; i <= (round == 0 ? 153 : 300); ++i) p.names.push_back(create_name(i, PROTO));
+        }
+        v.push_back(p);
+    }
+    // coll_stat_run (set_parameters main.cu:505-569, driver 1690-1802): 1000
+    // Ceres-mass protoplanets on 2.5-3 AU, the MMSN solid mass (Sigma_1 =
+    // 7 g/cm^2, f_neb = 1) shared equally among them
+    {
+        Preset p{"coll_stat_run", {}, true, MMSN_SHARE, Nebula{2.5, 3.0, 2.7, 1.0, 7.0 * sigma, 4.2, -3.0 / 2.0}};
+        p.types = {star,
+                   {PROTO, 1000, 1,
+                    {{SMA, K::POWER_LAW, 2.5, 3.0, -3.0 / 2.0, 0},
+                     {ECC, K::RAYLEIGH, 0.02, 0, 0, 0},
+                     {INC, K::RAYLEIGH, 0.04, 0, 0, 0},
+                     {PERI, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                     {NODE, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                     {MEAN, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                     {MASS, K::UNIFORM, cst::CeresToSolar, cst::CeresToSolar, 0, 0},
+                     {DENSITY, K::UNIFORM, 2.5 * rho, 2.5 * rho, 0, 0},
+                     {CD, K::UNIFORM, 0.0, 0.0, 0, 0}}}};
+        v.push_back(p);
+    }
+    // Hansen_2009 (set_parameters main.cu:626-700, driver 1917-2003): a
+    // Jupiter at 5.2 AU (replaced by the real Jupiter) and 400 protoplanets on
+    // 0.7-1 AU
+    {
+        Preset p{"Hansen_2009",
+                 {star,
+                  {GIANT, 1, 1,
+                   {{SMA, K::UNIFORM, 5.2, 5.2, 0, 0}, {ECC, K::UNIFORM, 0.05, 0.05, 0, 0},
+                    {INC, K::UNIFORM, 0.0, 0.0, 0, 0}, {PERI, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {NODE, K::UNIFORM, 0.0, 0.0, 0, 0}, {MEAN, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {MASS, K::UNIFORM, cst::JupiterToSolar, cst::JupiterToSolar, 0, 0},
+                    {DENSITY, K::UNIFORM, 1.326 * rho, 1.326 * rho, 0, 0},
+                    {CD, K::UNIFORM, 0.0, 0.0, 0, 0}}},
+                  {PROTO, 400, 1,
+                   {{SMA, K::UNIFORM, 0.7, 1.0, 0, 0}, {ECC, K::UNIFORM, 0.0, 0.0, 0, 0},
+                    {INC, K::UNIFORM, 0.0, 0.5 * cst::DegreeToRadian, 0, 0}, {PERI, K::UNIFORM, 0.0, 0.0, 0, 0},
+                    {NODE, K::UNIFORM, 0.0, 2.0 * PI, 0, 0}, {MEAN, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {MASS, K::UNIFORM, 5.0e-3 * cst::EarthToSolar, 5.0e-3 * cst::EarthToSolar, 0, 0},
+                    {DENSITY, K::UNIFORM, 2.7 * rho, 2.7 * rho, 0, 0},
+                    {CD, K::UNIFORM, 0.0, 0.0, 0, 0}}}}};
+        p.post = hansen_post;
+        v.push_back(p);
+    }
+    // n_pl_to_test_anal_gd (set_parameters main.cu:456-503, driver
+    // 2863-2911): 10 planetesimals of 1 m radius on 1-10 AU circles, named
+    // from pl0
+    v.push_back({"n_pl_to_test_anal_gd",
+                 {star,
+                  {PL, 10, 0,
+                   {{SMA, K::UNIFORM, 1.0, 10.0, 0, 0}, {ECC, K::UNIFORM, 0.0, 0.0, 0, 0},
+                    {INC, K::UNIFORM, 0.0, 0.0, 0, 0}, {PERI, K::UNIFORM, 0.0, 0.0, 0, 0},
+                    {NODE, K::UNIFORM, 0.0, 0.0, 0, 0}, {MEAN, K::UNIFORM, 0.0, 0.0, 0, 0},
+                    {RADIUS, K::UNIFORM, cst::MeterToAu, cst::MeterToAu, 0, 0},
+                    {DENSITY, K::UNIFORM, 2.7 * rho, 2.7 * rho, 0, 0},
+                    {CD, K::UNIFORM, 1.0, 1.0, 0, 0}}}}});
+    // solar_system (set_parameters main.cu:438-454, driver 2913-2961,
+    // populate_solar_system 85-249): the Sun and the eight planets from a table
+    {
+        Preset p{"solar_system", {}};
+        p.table = true;
+        v.push_back(p);
+    }
+    // Colliding_bodies (set_parameters main.cu:1412-1461, driver 3055-3153):
+    // two Ceres-mass protoplanets, a target and a projectile 0.01 deg behind
+    // it on the same orbit
+    {
+        Preset p{"Colliding_bodies",
+                 {star,
+                  {PROTO, 2, 0,
+                   {{SMA, K::UNIFORM, 1.0, 2.0, 0, 0}, {ECC, K::UNIFORM, 0.0, 0.0, 0, 0},
+                    {INC, K::UNIFORM, 0.0, 0.0, 0, 0}, {PERI, K::UNIFORM, 0.0, 0.0, 0, 0},
+                    {NODE, K::UNIFORM, 0.0, 0.0, 0, 0}, {MEAN, K::UNIFORM, 0.0, 0.0, 0, 0},
+                    {MASS, K::UNIFORM, cst::CeresToSolar, cst::CeresToSolar, 0, 0},
+                    {DENSITY, K::UNIFORM, 2.7 * rho, 2.7 * rho, 0, 0},
+                    {CD, K::UNIFORM, 0.0, 0.0, 0, 0}}}}};
+        p.post = colliding_post;
         v.push_back(p);
     }
     return v;
@@ -371,8 +485,12 @@ struct Disk {
     double t0 = 0.0, dt = 0.0;
 };
 
+Disk solar_system_table();
+Disk finish(Disk D);
+
 Disk generate(const Preset& P, uint32_t seed)
 {
+    if (P.table) return finish(solar_system_table());
     CRand crand;
     crand.seed(seed);
     // distributions, one rand() each, in construction order
@@ -423,7 +541,7 @@ Disk generate(const Preset& P, uint32_t seed)
         }
     }
     const uint32_t n = (uint32_t)D.bodies.size();
-    if (P.scale_to_mmsn) {
+    if (P.mmsn != MMSN_NONE) {
         // tools::get_total_mass over the protoplanets, descending (tools.cpp:351-365)
         const double m_solid = P.neb.solid_mass();
         auto total = [&]() {
@@ -432,9 +550,15 @@ Disk generate(const Preset& P, uint32_t seed)
                 if (D.bodies[j].type == PROTO && D.bodies[j].id > 0) m += D.bodies[j].mass;
             return m;
         };
-        const double f = m_solid / total();
-        for (Body& b : D.bodies)
-            if (b.type == PROTO) b.mass *= f;
+        if (P.mmsn == MMSN_SCALE) {   // Dvorak, Chambers2001 (main.cu:1841-1858)
+            const double f = m_solid / total();
+            for (Body& b : D.bodies)
+                if (b.type == PROTO) b.mass *= f;
+        } else {                      // coll_stat_run (main.cu:1734-1748): equal shares
+            const double m_pp = m_solid / D.n_type[PROTO];
+            for (Body& b : D.bodies)
+                if (b.type == PROTO) b.mass = m_pp;
+        }
         if (std::fabs(total() - m_solid) > 1.0e-15) {
             std::cerr << "The required mass was not reached." << std::endl;
             std::exit(0);   // as the reference does, writing nothing
@@ -452,7 +576,15 @@ Disk generate(const Preset& P, uint32_t seed)
             }
         }
     }
-    // calculate_phase (main.cu:317-342)
+    for (size_t i = 0; i < P.names.size() && i < D.bodies.size(); ++i) D.bodies[i].name = P.names[i];
+    if (P.post) P.post(D);
+    return finish(std::move(D));
+}
+
+// calculate_phase (main.cu:317-342), the barycentre and red's units
+Disk finish(Disk D)
+{
+    const uint32_t n = (uint32_t)D.bodies.size();
     double min_P = 1.7976931348623157e308;
     const double m0 = D.bodies[0].mass;
     D.bodies[0].r = Vec{0, 0, 0};
@@ -489,6 +621,112 @@ Disk generate(const Preset& P, uint32_t seed)
     }
     D.t0 *= cst::Gauss;
     D.dt *= cst::Gauss;
+    return D;
+}
+
+// Osculating elements of the planets on 2015-May-11 (JPL HORIZONS; the values
+// of main.cu:41-83), as extract_from_horizon_output reads them from their
+// fixed columns (util_init.cpp:327-366): e, i, Omega, omega, M in degrees, a
+// in AU.  The degrees become radians as there.
+struct Horizons {
+    double ecc, inc, node, peri, mean, sma;
+};
+constexpr Horizons kMercury{2.056283170656704E-01, 7.004033081870772E+00, 4.831135275416084E+01,
+                            2.917018579009503E+01, 8.665226794919347E+01, 3.870990540148295E-01};
+constexpr Horizons kVenus{6.761247965610893E-03, 3.394467689974794E+00, 7.663950855912078E+01,
+                          5.463888630600258E+01, 3.625130289912041E+01, 7.233287920706470E-01};
+constexpr Horizons kEarth{1.706151645376220E-02, 2.832604859699792E-03, 1.991276585462631E+02,
+                          2.644902808333621E+02, 1.245850770307868E+02, 9.999423845001243E-01};
+constexpr Horizons kMars{9.345598920376896E-02, 1.848403968432629E+00, 4.951276588949865E+01,
+                         2.865318030175783E+02, 7.844645806929105E+01, 1.523589291796774E+00};
+constexpr Horizons kJupiter{4.895926045495331E-02, 1.303927595561599E+00, 1.005220570364609E+02,
+                            2.736235124666692E+02, 1.262463884135522E+02, 5.201472759810757E+00};
+constexpr Horizons kSaturn{5.370451604891303E-02, 2.486273622454862E+00, 1.136214918787423E+02,
+                           3.402344971420399E+02, 1.437153605146956E+02, 9.553843040430950E+00};
+constexpr Horizons kUranus{4.928987553618186E-02, 7.717997843738144E-01, 7.400078543435393E+01,
+                           9.677390970582711E+01, 2.083879289018873E+02, 1.915532588905842E+01};
+constexpr Horizons kNeptune{8.258758489207289E-03, 1.765657591232885E+00, 1.317164628477979E+02,
+                            2.932421741299731E+02, 2.734992366503029E+02, 2.997852567031729E+01};
+
+Oe oe_of(const Horizons& h)
+{
+    return Oe{h.sma, h.ecc, h.inc * cst::DegreeToRadian, h.peri * cst::DegreeToRadian,
+              h.node * cst::DegreeToRadian, h.mean * cst::DegreeToRadian};
+}
+
+// Hansen_2009 (main.cu:1955-1965): the first giant becomes Jupiter
+void hansen_post(Disk& D)
+{
+    for (size_t i = 1; i < D.bodies.size(); ++i) {
+        Body& b = D.bodies[i];
+        if (b.type != GIANT) continue;
+        b.name = "Jupiter";
+        b.mass = 1.0 * cst::JupiterToSolar;
+        b.radius = 71492.0 * cst::KilometerToAu;
+        b.density = calc_density(b.mass, b.radius);
+        b.oe = oe_of(kJupiter);
+        return;
+    }
+}
+
+// Colliding_bodies (main.cu:3090-3115): the first two protoplanets on one
+// orbit, the second 0.01 degree behind in mean anomaly
+void colliding_post(Disk& D)
+{
+    int k = 0;
+    for (size_t i = 1; i < D.bodies.size() && k < 2; ++i) {
+        Body& b = D.bodies[i];
+        if (b.type != PROTO) continue;
+        b.name = k == 0 ? "target" : "projectile";
+        b.oe = Oe{2.7, 0.05, 0.0, 0.0, 0.0, k == 0 ? 0.0 : 0.01 * cst::DegreeToRadian};
+        ++k;
+    }
+}
+
+// populate_solar_system (main.cu:85-249): the Sun, the rocky planets, then the
+// giants, in that order (rocky before giant, not sorted by body type);
+// parameters from red's constants and radii, the remaining one of mass,
+// radius and density computed, cd = 0
+Disk solar_system_table()
+{
+    struct Row {
+        const char* name;
+        int type;
+        double mass, radius_km;
+        const Horizons* oe;
+    };
+    const Row rows[] = {{"Mercury", ROCKY, cst::MercuryToSolar, 2439.7, &kMercury},
+                        {"Venus", ROCKY, cst::VenusToSolar, 6051.8, &kVenus},
+                        {"Earth", ROCKY, cst::EarthToSolar, 6371.0, &kEarth},
+                        {"Mars", ROCKY, cst::MarsToSolar, 3389.5, &kMars},
+                        {"Jupiter", GIANT, cst::JupiterToSolar, 71492.0, &kJupiter},
+                        {"Saturn", GIANT, cst::SaturnToSolar, 60268.0, &kSaturn},
+                        {"Uranus", GIANT, cst::UranusToSolar, 25559.0, &kUranus},
+                        {"Neptune", GIANT, cst::NeptuneToSolar, 24766.0, &kNeptune}};
+    Disk D;
+    Body sun{};
+    sun.name = "Sun";
+    sun.id = 1;
+    sun.type = STAR;
+    sun.mass = 1.0;
+    sun.radius = 1.0 * cst::SolarRadiusToAu;
+    sun.density = calc_density(sun.mass, sun.radius);
+    D.bodies.push_back(sun);
+    int32_t id = 2;
+    for (const Row& r : rows) {
+        Body b{};
+        b.name = r.name;
+        b.id = id++;
+        b.type = r.type;
+        b.mass = 1.0 * r.mass;
+        b.radius = r.radius_km * cst::KilometerToAu;
+        b.density = calc_density(b.mass, b.radius);
+        b.oe = oe_of(*r.oe);
+        D.bodies.push_back(b);
+    }
+    D.n_type[STAR] = 1;
+    D.n_type[ROCKY] = 4;
+    D.n_type[GIANT] = 4;
     return D;
 }
 

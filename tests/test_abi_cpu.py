@@ -46,7 +46,16 @@ def test_validation_errors():
     bad[5] += 1          # counts do not match the body types
     assert R.lib.rd_disk_create(17, R._ptr(bad), R._ptr(r), R._ptr(v), R._ptr(p), R._ptr(b), 0.0, None, None, 0,
                                 C.byref(h)) == R.RD_EINVAL
-    assert "sorted by body_type" in R.lib.rd_last_error().decode()
+    assert "does not match the body types" in R.lib.rd_last_error().decode()
+    # a planetesimal (NSI) among the protoplanets (SI): the classes must be contiguous
+    b3 = b.copy()
+    b3["body_type"][5] = 5
+    nbt3 = nbt.copy()
+    nbt3[3] -= 1
+    nbt3[5] += 1
+    assert R.lib.rd_disk_create(17, R._ptr(nbt3), R._ptr(r), R._ptr(v), R._ptr(p), R._ptr(b3), 0.0, None, None, 0,
+                                C.byref(h)) == R.RD_EINVAL
+    assert "grouped by interaction class" in R.lib.rd_last_error().decode()
     b2 = b.copy()
     b2["id"][4] = 0
     assert R.lib.rd_disk_create(17, R._ptr(nbt), R._ptr(r), R._ptr(v), R._ptr(p), R._ptr(b2), 0.0, None, None, 0,
@@ -55,6 +64,19 @@ def test_validation_errors():
                                 C.byref(h)) == R.RD_EINVAL
     with pytest.raises(R.RedError):
         R.check(R.lib.rd_integrator_create(None, 4, 1.0, 1, 1e-10, C.byref(h)))
+    # within the SI class any order passes validation (the reference's own
+    # solar-system generator writes rocky planets before giants)
+    b4 = b.copy()
+    b4["body_type"][1:3] = [2, 1]
+    nbt4 = nbt.copy()
+    nbt4[3] -= 2
+    nbt4[1] += 1
+    nbt4[2] += 1
+    rc = R.lib.rd_disk_create(17, R._ptr(nbt4), R._ptr(r), R._ptr(v), R._ptr(p), R._ptr(b4), 0.0, None, None, 0,
+                              C.byref(h))
+    assert rc != R.RD_EINVAL, R.lib.rd_last_error().decode()
+    if rc == R.RD_OK:
+        R.lib.rd_disk_destroy(h)
     # a batch holding an empty disk is refused before any device call
     nbt2 = np.stack([nbt, np.zeros_like(nbt)])
     assert R.lib.rd_batch_create(2, R._ptr(nbt2), R._ptr(r), R._ptr(v), R._ptr(p), R._ptr(b), None, None, None, 0,

@@ -67,7 +67,11 @@ def read_outputs(d, stem):
 @needs_bin
 @pytest.mark.parametrize("preset,seed", [("two_body", 1), ("n_gp", 12345), ("n_gp", 1733170000), ("n_pp", 7),
                                          ("n_spl", 99), ("n_pl", 2 ** 31 + 5), ("n_tp", 3), ("Dvorak", 7),
-                                         ("Dvorak", 1717171717)])
+                                         ("Dvorak", 1717171717), ("Chambers2001", 7), ("Chambers2001", 1733170000),
+                                         ("coll_stat_run", 12345), ("coll_stat_run", 2 ** 32 - 1),
+                                         ("Hansen_2009", 5), ("Hansen_2009", 1733170001),
+                                         ("n_pl_to_test_anal_gd", 17), ("solar_system", 1733170002),
+                                         ("Colliding_bodies", 3), ("Colliding_bodies", 99)])
 def test_preset_matches_oracle(tmp_path, preset, seed):
     run(tmp_path, preset, seed)
     info, rows, oe, seed_txt, start = read_outputs(str(tmp_path), preset)
@@ -146,5 +150,107 @@ def test_redinit_errors(args, msg):
 def test_redinit_lists_presets():
     res = subprocess.run([REDINIT, "-l"], capture_output=True, text=True, timeout=30)
     assert res.returncode == 0
-    for p in RO.PRESETS:
+    for p in list(RO.PRESETS) + list(RO.EXTRA):
         assert p in res.stdout
+
+
+def elements(r, v, mu):
+    """a, e, i, Omega, omega and the eccentric anomaly of a relative state
+    (AU, AU/day), from the angular momentum and eccentricity vectors."""
+    r, v = np.asarray(r, float), np.asarray(v, float)
+    h = np.cross(r, v)
+    rn = np.linalg.norm(r)
+    evec = np.cross(v, h) / mu - r / rn
+    e = np.linalg.norm(evec)
+    a = 1.0 / (2.0 / rn - v.dot(v) / mu)
+    inc = math.acos(h[2] / np.linalg.norm(h))
+    node = math.atan2(h[0], -h[1]) % (2 * math.pi)
+    nvec = np.array([-h[1], h[0], 0.0])
+    peri = math.atan2(np.cross(nvec, evec).dot(h) / np.linalg.norm(h), nvec.dot(evec)) % (2 * math.pi)
+    E = math.atan2(r.dot(v) / math.sqrt(mu * a), 1.0 - rn / a) % (2 * math.pi)
+    return a, e, inc, node, peri, E
+
+
+@needs_bin
+def test_solar_system_recovers_the_horizons_elements(tmp_path):
+    """The planets' written states give back the 2015-May-11 elements the
+    generator read (a, e, i, Omega, omega; M through Kepler's equation where
+    the reference solves it, E = M where it takes its shortcut, M > pi), in
+    the reference's order Sun, rocky planets, giants."""
+    run(tmp_path, "solar_system", 20150511)
+    info, rows, _, _, _ = read_outputs(str(tmp_path), "solar_system")
+    assert [int(x) for x in info[3:]] == [1, 4, 4, 0, 0, 0, 0]
+    names = [r[0] for r in rows]
+    assert names == ["Sun", "Mercury", "Venus", "Earth", "Mars", "Jupiter", "Saturn", "Uranus", "Neptune"]
+    assert [int(r[2]) for r in rows] == [0, 2, 2, 2, 2, 1, 1, 1, 1]
+    sun = [float(x) for x in rows[0][9:15]]
+    for row in rows[1:]:
+        st = [float(x) for x in row[9:15]]
+        m = float(row[5])
+        r = [st[c] - sun[c] for c in range(3)]
+        v = [(st[3 + c] - sun[3 + c]) * RO.GAUSS for c in range(3)]      # k-day units back to AU/day
+        a, e, inc, node, peri, E = elements(r, v, RO.K2 * (1.0 + m))
+        ecc, inc_d, node_d, peri_d, mean_d, sma = RO.HORIZONS[row[0]]
+        assert abs(a / sma - 1) < 1e-12 and abs(e - ecc) < 1e-12, row[0]
+        assert abs(inc - inc_d * math.pi / 180) < 1e-11, row[0]
+        assert abs(node - node_d * math.pi / 180) < 1e-9 and abs(peri - peri_d * math.pi / 180) < 1e-9, row[0]
+        M = mean_d * math.pi / 180
+        if M < math.pi:
+            assert abs((E - e * math.sin(E)) - M) < 1e-8, row[0]
+        else:
+            assert abs(E - M) < 1e-9, row[0]
+        assert abs(float(row[7]) - m / (4.0 / 3.0 * math.pi * float(row[6]) ** 3)) < 1e-12 * float(row[7])
+
+
+@needs_bin
+def test_hansen_jupiter_and_protoplanets(tmp_path):
+    run(tmp_path, "Hansen_2009", 9)
+    _, rows, _, _, _ = read_outputs(str(tmp_path), "Hansen_2009")
+    assert rows[1][0] == "Jupiter" and int(rows[1][2]) == 1
+    assert float(rows[1][5]) == 1.0 / 1.0473486e3
+    assert abs(float(rows[1][6]) - 71492.0 / 1.495978707e8) < 1e-18
+    st = [[float(x) for x in r[9:15]] for r in rows]
+    rel = [st[1][c] - st[0][c] for c in range(3)]
+    vel = [(st[1][3 + c] - st[0][3 + c]) * RO.GAUSS for c in range(3)]
+    a, e, *_ = elements(rel, vel, RO.K2 * (1.0 + float(rows[1][5])))
+    assert abs(a - 5.201472759810757) < 1e-11 and abs(e - 4.895926045495331E-02) < 1e-12
+    # 400 protoplanets of 0.005 Earth masses on circles of 0.7-1 AU
+    m = np.array([float(r[5]) for r in rows[2:]])
+    assert len(m) == 400 and np.all(m == 5.0e-3 * (1.0 / 3.3294605e5))
+    d = np.array([np.linalg.norm(np.array(s[:3]) - np.array(st[0][:3])) for s in st[2:]])
+    assert d.min() > 0.7 - 1e-9 and d.max() < 1.0 + 1e-9
+
+
+@needs_bin
+def test_coll_stat_run_shares_the_integrated_solid_mass(tmp_path):
+    """1000 equal protoplanet masses summing to the solid disk's mass, the
+    surface density 7 g/cm^2 r^-3/2 (times 4.2 beyond the 2.7 AU snowline)
+    integrated over 2.5-3 AU numerically."""
+    from scipy.integrate import quad
+    run(tmp_path, "coll_stat_run", 4)
+    _, rows, _, _, _ = read_outputs(str(tmp_path), "coll_stat_run")
+    m = np.array([float(r[5]) for r in rows[1:]])
+    assert len(m) == 1000 and np.all(m == m[0])
+    sig = lambda r: 7.0 * RO.SIGMA * r ** -1.5 * (4.2 if r > 2.7 else 1.0) * 2 * math.pi * r
+    total = quad(sig, 2.5, 2.7, epsrel=1e-13)[0] + quad(sig, 2.7, 3.0, epsrel=1e-13)[0]
+    assert abs(m.sum() / total - 1.0) < 1e-12
+
+
+@needs_bin
+def test_chambers2001_names_and_colliding_pair(tmp_path):
+    run(tmp_path, "Chambers2001", 8)
+    _, rows, _, _, _ = read_outputs(str(tmp_path), "Chambers2001")
+    assert len(rows) == 301
+    # the reference pushes Chambers2001's 154 names before Dvorak's 301
+    assert rows[153][0] == "proto153" and rows[154][0] == "star" and rows[155][0] == "proto1"
+    assert rows[300][0] == "proto146"
+    run(tmp_path, "Colliding_bodies", 8)
+    _, rows, _, _, _ = read_outputs(str(tmp_path), "Colliding_bodies")
+    assert [r[0] for r in rows] == ["star", "target", "projectile"]
+    t, p = (np.array([float(x) for x in r[9:12]]) for r in rows[1:])
+    # 0.01 degree of mean anomaly at periapsis of a = 2.7, e = 0.05: a chord of
+    # a (1 + e)^2 / sqrt(1 - e^2)^3 ... to first order r_p * dnu, dnu = dM (1+e)^2/(1-e^2)^1.5
+    dM = 0.01 * math.pi / 180
+    r_p = 2.7 * (1 - 0.05)
+    chord = r_p * dM * (1 + 0.05) ** 2 / (1 - 0.05 ** 2) ** 1.5
+    assert abs(np.linalg.norm(p - t) / chord - 1.0) < 1e-4
