@@ -161,7 +161,7 @@ def test_redinit_then_red(tmp_path, preset, length):
     o = O.ODisk(r, v, p, b, nbt, t=t0)
     oi = O.OInteg(o, 4, dt, adaptive=True, tol=1e-10)
     rc, steps = oi.run(CF.internal_time(length), CF.internal_time(str(float(length) / 4)), 2 ** 62)
-    assert rc == 0 and steps > 10
+    assert rc == 0 and steps >= 4     # at least one step per output interval
     last = sorted(glob.glob(os.path.join(str(tmp_path), "data_*[0-9].txt")))[-1]
     names, ids, rr, vv, _ = CF.read_data(last, R.BODY_MD)
     ot, orr, ovv, _, obb = o.state()
