@@ -414,7 +414,7 @@ def bench_config5(args):
     d.close()
     print(f"# {len(T)} samples in {t_log:.1f} s; training", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    m, loss = P.fit(X, T, epochs=args.epochs)
+    m, loss = P.fit(X, T, epochs=args.epochs, base=args.c5_base, quantile=args.c5_quantile)
     t_fit = time.perf_counter() - t0
     rms = float(np.sqrt(np.mean((m.forward(X) - T) ** 2)))
     n_samples = int(len(T))
@@ -436,6 +436,23 @@ def bench_config5(args):
                             energy_disks=range(0, len(sizes), 32))
         batch[name]["rk_evals"] = rk_evals(batch[name])
         d.close()
+    # what any proposer could save against the classical controller (mixed precision), from one
+    # more classical run of each workload with sample logging on
+    print("# ceilings", file=sys.stderr, flush=True)
+    d, h0 = P.config3_disk()
+    ceil = {"config3_disk": P.ceilings(d, args.c5_length, h0, tol)}
+    d.close()
+    d = P.make_batch(sizes, 4000, gas=True)
+    ceil["heldout_batch"] = P.ceilings(d, args.c5_batch_length, dt0, tol)
+    d.close()
+    f64 = disk3["classical_fp64"]
+    ceil["config3_disk_fp64_rejection_free"] = f64["rk_evals"] / (13.0 * f64["passed"])
+    achieved = {"config3_disk": disk3["classical_mixed"]["rk_evals"] / disk3["learned_mixed"]["rk_evals"],
+                "heldout_batch": batch["classical_mixed"]["rk_evals"] / batch["learned_mixed"]["rk_evals"]}
+    for w in ("config3_disk", "heldout_batch"):
+        ceil[w]["achieved"] = achieved[w]
+        ceil[w]["achieved_over_rejection_free"] = achieved[w] / ceil[w]["rejection_free"]
+        ceil[w]["achieved_over_exact_step"] = achieved[w] / ceil[w]["exact_step"]
     value = disk3["classical_fp64"]["rk_evals"] / disk3["learned_mixed"]["rk_evals"]
     out = {"metric": "RK evaluations, plain FP64 RKF78 / learned proposer (config 5)", "value": value,
            "unit": "x fewer RK evaluations", "n_gpus": 1, "steps": disk3["learned_mixed"]["passed"], "warmup": 0,
@@ -446,10 +463,11 @@ def bench_config5(args):
                                   "config-3 physics (seeds 7000+k); ablation to equal simulated time on the config-3 "
                                   f"disk (N=131072, seed 3, {args.c5_length:g} time units) and on the held-out batch "
                                   f"(seeds 4000+k, {args.c5_batch_length:g} time units), RKF7(8) tol {tol:g}",
-                      "safety": args.safety, "epochs": args.epochs},
+                      "safety": args.safety, "epochs": args.epochs,
+                      "fit": f"quantile {args.c5_quantile:g} of log h*" + (f" - feature {args.c5_base}" if args.c5_base >= 0 else "")},
            "training": {"samples": n_samples, "log_seconds": t_log, "fit_seconds": t_fit, "final_loss": loss,
                         "rms_log_h": rms},
-           "config3_disk": disk3, "heldout_batch": batch,
+           "config3_disk": disk3, "heldout_batch": batch, "ceilings": ceil,
            "ratios": {"config3_evals_fp64_over_learned": value,
                       "config3_evals_mixed_over_learned": disk3["classical_mixed"]["rk_evals"] / disk3["learned_mixed"]["rk_evals"],
                       "config3_time_mixed_over_learned": disk3["classical_mixed"]["seconds"] / disk3["learned_mixed"]["seconds"],
@@ -519,7 +537,12 @@ def main():
                          "4 batch of 512 disks, 5 learned step proposer")
     ap.add_argument("--log-steps", type=int, default=2000, help="config 5: logged steps per disk")
     ap.add_argument("--epochs", type=int, default=10, help="config 5: training epochs")
-    ap.add_argument("--safety", type=float, default=0.85, help="config 5: proposer safety factor")
+    ap.add_argument("--safety", type=float, default=None, help="config 5: proposer safety factor (default: "
+                    "proposer.SAFETY)")
+    ap.add_argument("--c5-quantile", type=float, default=None, help="config 5: fitted quantile, 0 = mean "
+                    "(default: proposer.QUANTILE)")
+    ap.add_argument("--c5-base", type=int, default=None, help="config 5: targets as offsets from this feature, "
+                    "-1 = none (default: proposer.BASE)")
     ap.add_argument("--c5-length", type=float, default=1.5, help="config 5: config-3 disk run length (time units)")
     ap.add_argument("--c5-batch-length", type=float, default=2.9, help="config 5: held-out batch run length")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -560,6 +583,10 @@ def main():
     if args.config == 4:
         return bench_config4(args)
     if args.config == 5:
+        import proposer as P
+        args.safety = P.SAFETY if args.safety is None else args.safety
+        args.c5_quantile = P.QUANTILE if args.c5_quantile is None else args.c5_quantile
+        args.c5_base = P.BASE if args.c5_base is None else args.c5_base
         return bench_config5(args)
 
     r, v, p, b, nbt = R.make_disk(args.n, args.seed)

@@ -288,12 +288,21 @@ int rd_mlp_create(int device, uint32_t seed, rd_mlp** out);
 int rd_mlp_destroy(rd_mlp* m);
 /* feature / target standardisation applied inside the network */
 int rd_mlp_set_norm(rd_mlp* m, const float mean[RD_MLP_IN], const float stdev[RD_MLP_IN], float t_mean, float t_std);
+/* What the network is trained on (default: base -1, quantile 0).  The output
+ * y' becomes y' t_std + t_mean + x[base_feature] (no offset for -1), so a
+ * network whose training targets are offsets from a feature -- feature 9, the
+ * reference controller's proposal -- still returns log h.  quantile 0 fits
+ * the mean (loss 0.5 mean e^2 in standardised units); q in (0, 1) fits the
+ * q-quantile (pinball loss max(q u, (q - 1) u), u = t' - y'), so about a
+ * fraction q of the targets fall below the output. */
+int rd_mlp_set_target(rd_mlp* m, int base_feature, float quantile);
 /* FP32 master parameters (layout W1[12][64] b1[64] W2[64][64] b2[64] w3[64] b3) */
 int rd_mlp_set_params(rd_mlp* m, const float* params);
 int rd_mlp_get_params(rd_mlp* m, float* params, uint16_t* params_bf16);
 /* y = log h for n raw feature rows */
 int rd_mlp_forward(rd_mlp* m, const float* x, uint32_t n, float* y);
-/* loss 0.5 mean((y - t)/t_std)^2 and its gradient w.r.t. the FP32 parameters */
+/* the loss set by rd_mlp_set_target (default 0.5 mean((y - t)/t_std)^2) and
+ * its gradient w.r.t. the FP32 parameters */
 int rd_mlp_loss_grad(rd_mlp* m, const float* x, const float* t, uint32_t n, float* loss, float* grad);
 /* Adam on the GPU over shuffled minibatches */
 int rd_mlp_train(rd_mlp* m, const float* x, const float* t, uint32_t n, uint32_t epochs, uint32_t batch, float lr,

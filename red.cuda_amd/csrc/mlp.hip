@@ -41,9 +41,16 @@ __host__ __device__ inline uint16_t f32_to_bf16(float f)
     return (uint16_t)((u + r) >> 16);
 }
 
+// Standardisation and target.  The network's output y' is de-normalised to
+// y' t_std + t_mean + x[base] (base < 0: no offset), so a network trained on
+// targets t - x[base] still returns log h.  quantile 0 trains the mean
+// (loss 0.5 e^2); quantile q in (0, 1) trains the q-quantile (pinball loss
+// max(q u, (q - 1) u), u = t' - y').
 struct Norm {
     float mean[kIn], inv_std[kIn];
     float t_mean, t_std;
+    int base;
+    float quantile;
 };
 
 // Weights of one block, expanded to FP32 in LDS.
@@ -101,10 +108,14 @@ __global__ __launch_bounds__(kH) void k_mlp_forward(const uint16_t* __restrict__
     for (int k = 0; k < kH; ++k) b = fmaf(h1[k], bf16_to_f32(wb[kOffW2 + k * kH + j]), b);
     float o = tanhf(b) * bf16_to_f32(wb[kOffW3 + j]);
     for (int off = 32; off > 0; off >>= 1) o += __shfl_xor(o, off, 64);
-    if (j == 0) y[q] = (o + bf16_to_f32(wb[kOffB3])) * nm.t_std + nm.t_mean;
+    if (j == 0) {
+        const float off = nm.base >= 0 ? xraw[(size_t)q * kIn + nm.base] : 0.0f;
+        y[q] = (o + bf16_to_f32(wb[kOffB3])) * nm.t_std + nm.t_mean + off;
+    }
 }
 
-// Loss 0.5 * mean (y - t)^2 in normalised target units and its gradient.
+// Loss in normalised target units -- 0.5 mean e^2, or the mean pinball loss
+// when nm.quantile > 0 (e = y' - t') -- and its gradient.
 // Each block (64 samples) writes its partial gradient (and loss) to
 // part[block][*]; k_mlp_reduce sums the partials in block order.  Outer
 // products over the block's samples go through two LDS buffers in phases
@@ -131,9 +142,16 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_grad(const uint16_t* __restri
         const uint32_t src = idx ? idx[q] : q;
         for (int i = 0; i < kIn; ++i) x[i] = (xraw[(size_t)src * kIn + i] - nm.mean[i]) * nm.inv_std[i];
         const float y = forward_one(s, x, h1, h2);
-        const float e = y - (tr[src] - nm.t_mean) / nm.t_std;
-        dy = e * scale;
-        l = 0.5f * e * e * scale;
+        const float tb = tr[src] - (nm.base >= 0 ? xraw[(size_t)src * kIn + nm.base] : 0.0f);
+        const float e = y - (tb - nm.t_mean) / nm.t_std;
+        if (nm.quantile > 0.0f) {
+            const float w = e > 0.0f ? 1.0f - nm.quantile : -nm.quantile;
+            dy = w * scale;
+            l = w * e * scale;
+        } else {
+            dy = e * scale;
+            l = 0.5f * e * e * scale;
+        }
     } else {
         for (int i = 0; i < kIn; ++i) x[i] = 0.0f;
         for (int j = 0; j < kH; ++j) h1[j] = h2[j] = 0.0f;
@@ -338,6 +356,8 @@ int rd_mlp_create(int device, uint32_t seed, rd_mlp** out)
     for (int i = 0; i < kIn; ++i) { p->nm.mean[i] = 0.0f; p->nm.inv_std[i] = 1.0f; }
     p->nm.t_mean = 0.0f;
     p->nm.t_std = 1.0f;
+    p->nm.base = -1;
+    p->nm.quantile = 0.0f;
     if (int rc = rd_mlp_set_params(p, w.data())) {
         rd_mlp_destroy(p);
         return rc;
@@ -370,6 +390,17 @@ int rd_mlp_set_norm(rd_mlp* p, const float mean[RD_MLP_IN], const float stdev[RD
     }
     p->nm.t_mean = t_mean;
     p->nm.t_std = t_std;
+    return RD_OK;
+}
+
+int rd_mlp_set_target(rd_mlp* p, int base_feature, float quantile)
+{
+    if (!p) return mfail(RD_EINVAL, "NULL model");
+    if (base_feature < -1 || base_feature >= kIn) return mfail(RD_EINVAL, "base feature must be -1 or 0..11");
+    if (!(quantile == 0.0f || (quantile > 0.0f && quantile < 1.0f)))
+        return mfail(RD_EINVAL, "quantile must be 0 (mean) or in (0, 1)");
+    p->nm.base = base_feature;
+    p->nm.quantile = quantile;
     return RD_OK;
 }
 

@@ -157,6 +157,11 @@ class mlp {
     {
         rd_check(rd_mlp_set_norm(h_, mean, stdev, t_mean, t_std), "mlp::set_norm");
     }
+    // targets as offsets from feature base_feature (-1: none); quantile 0 fits the mean
+    void set_target(int base_feature, float quantile)
+    {
+        rd_check(rd_mlp_set_target(h_, base_feature, quantile), "mlp::set_target");
+    }
     // fits the samples (x: n rows of RD_MLP_IN features, t: n targets); returns the last epoch's loss
     float train(const std::vector<float>& x, const std::vector<float>& t, uint32_t epochs, uint32_t batch, float lr)
     {
@@ -209,6 +214,11 @@ class integrator {
     uint64_t get_n_passed_step() const { return counter(1); }
     uint64_t get_n_failed_step() const { return counter(2); }
     void set_error_check_for_tp(bool on) { rd_check(rd_set_error_check_for_tp(h_, on ? 1 : 0), "error_check_for_tp"); }
+    // opt-in bound on the retry loop (red_v2 int_rungekutta7.cu:176-185); 0 turns a bound off
+    void set_guard(uint32_t max_attempts, double min_dt, bool nonfinite)
+    {
+        rd_check(rd_integrator_set_guard(h_, max_attempts, min_dt, nonfinite ? 1 : 0), "set_guard");
+    }
     rd_integrator* handle() { return h_; }
 
     // learned step proposer: first try of each step from m (nullptr: reference controller only)

@@ -93,6 +93,7 @@ _SIGS = {
     "rd_mlp_create": (C.c_int, [C.c_int, _u32, C.POINTER(_P)]),
     "rd_mlp_destroy": (C.c_int, [_P]),
     "rd_mlp_set_norm": (C.c_int, [_P, _P, _P, C.c_float, C.c_float]),
+    "rd_mlp_set_target": (C.c_int, [_P, C.c_int, C.c_float]),
     "rd_mlp_set_params": (C.c_int, [_P, _P]),
     "rd_mlp_get_params": (C.c_int, [_P, _P, _P]),
     "rd_mlp_forward": (C.c_int, [_P, _P, _u32, _P]),
@@ -460,6 +461,11 @@ class Mlp:
         m = np.ascontiguousarray(mean, np.float32)
         s = np.ascontiguousarray(std, np.float32)
         check(lib.rd_mlp_set_norm(self.h, _ptr(m), _ptr(s), float(t_mean), float(t_std)))
+
+    def set_target(self, base=-1, quantile=0.0):
+        """Train on offsets from feature `base` (-1: none) and fit the mean
+        (quantile 0) or the given quantile; the output stays log h."""
+        check(lib.rd_mlp_set_target(self.h, int(base), float(quantile)))
 
     def params(self):
         w = np.zeros(MLP_NPARAM, np.float32)
