@@ -166,7 +166,12 @@ def profiled_traffic(kernel, workload_file, tuning):
         return None
     m = re.search(r"--tuning (\d+),(\d+),(\d+)(?:,(\d+))?(?:,(\d+))?", text)
     t = re.search(r"Traffic per launch: ([0-9.e+]+) B", text)
-    if not m or not t or kernel not in text:
+    if not t or kernel not in text:
+        return None
+    if kernel == "k_force_exact":
+        # the FP64 kernel has one launch shape per disk
+        return float(t[1])
+    if not m:
         return None
     shape = (int(m[1]), int(m[2]), int(m[3]), int(m[4]) if m[4] else R.RD_PAIRS_ONE_SIDED)
     chunk_min = int(m[5]) if m[5] else 1
@@ -294,7 +299,9 @@ def bench_config1(args):
            "ms_per_step": el * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": "synthetic seeded disk",
            "config": {"workload": "star + 256 planetesimals, RK4 fixed h = 0.1 d, FP64, seed 1"},
-           "rk4_steps_per_s": steps / el, "rel_energy_error": rel_e, "roofline": roofline(d, "fp64")}
+           "rk4_steps_per_s": steps / el, "rel_energy_error": rel_e,
+           "roofline": roofline(d, "fp64", traffic=profiled_traffic("k_force_exact", "k_force_exact_disk256_rk4.txt",
+                                                                    None))}
     if not args.no_cpu:
         print("# cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_rk4_baseline(r, v, p, b, nbt, 0.1 * R.K_GAUSS, 10000)
@@ -339,7 +346,9 @@ def bench_config3(args):
            "rkf78_steps_per_s": steps / el, "attempts": c1[0] - c0[0], "rejected": c1[2] - c0[2],
            "events": int(len(d.event_log())), "rel_energy_error": rel_e,
            "note": "energy is not conserved with gas drag; the error column is the drag dissipation",
-           "roofline": roofline(d, "mixed")}
+           "roofline": roofline(d, "mixed", traffic=profiled_traffic(
+               pair_kernel("mixed", tuning), f"{pair_kernel('mixed', tuning)}_disk131072_gas_rkf78.txt", tuning)
+               if n == 131072 else None)}
     if not args.no_cpu:
         print("# cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_force_baseline(r, v, p, b, nbt, gas=g, drag_mode=1,
@@ -363,6 +372,8 @@ def bench_config4(args):
     d.set_tuning(use_graph=GRAPH)
     steps = args.steps if args.steps != 1000 else 100
     it = R.Integrator(d, R.RD_RKF78, 0.1 * R.K_GAUSS, adaptive=True, tol=args.tol)
+    probe = list(range(0, len(sizes), 32))          # energy checked on every 32nd disk, from t = 0
+    E0 = [d.integral(k)["E"] for k in probe]
     it.steps(args.warmup)
     ev0, p0 = it.work()
     d.sync()
@@ -372,6 +383,7 @@ def bench_config4(args):
     el = time.perf_counter() - t0
     ev1, p1 = it.work()
     tried = sum(it.counters(k)[0] for k in range(len(sizes)))
+    rel_e = max(abs(d.integral(k)["E"] / e - 1.0) for k, e in zip(probe, E0))
     out = {"metric": "pair interactions/sec + RKF78 steps/sec (config 4, batched)", "value": (p1 - p0) / el,
            "unit": "pair interactions/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
            "ms_per_step": el * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -381,7 +393,12 @@ def bench_config4(args):
                                   f"RKF7(8) tol {args.tol:g}, {args.precision}",
                       "tuning": tuning_json(tuning)},
            "disk_steps_per_s": steps * len(sizes) / el, "attempts_all_disks": tried,
-           "roofline": roofline(d, args.precision)}
+           "rel_energy_error": rel_e, "energy_ok": bool(rel_e <= 1e-9),
+           "energy_basis": f"max |E/E0 - 1| over disks {probe[0]}, {probe[1]}, ..., {probe[-1]} after "
+                           f"{args.warmup + steps} accepted steps from t = 0",
+           "roofline": roofline(d, args.precision, traffic=profiled_traffic(
+               pair_kernel(args.precision, tuning), f"{pair_kernel(args.precision, tuning)}_batch512_rkf78.txt", tuning)
+               if args.precision == "mixed" else None)}
     if not args.no_cpu:
         print("# cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_batch_baseline(parts, 0.1 * R.K_GAUSS, args.tol, 64)
@@ -407,7 +424,7 @@ def bench_config5(args):
     tol = args.tol
     dt0 = 0.1 * R.K_GAUSS
     print(f"# logging {args.log_steps} steps x {len(sizes)} disks", file=sys.stderr, flush=True)
-    d = P.make_batch(sizes, 7000, gas=True)
+    d = P.make_batch(sizes, 7000, gas=True, use_graph=GRAPH)
     t0 = time.perf_counter()
     X, T = P.collect(d, args.log_steps, dt0, tol)
     t_log = time.perf_counter() - t0
@@ -424,14 +441,14 @@ def bench_config5(args):
     disk3 = {}
     for name, prec, prop in runs:
         print(f"# config-3 disk {name}", file=sys.stderr, flush=True)
-        d, h0 = P.config3_disk(precision=prec)
+        d, h0 = P.config3_disk(precision=prec, use_graph=GRAPH)
         disk3[name] = P.run(d, length=args.c5_length, proposer=prop, safety=args.safety, dt0=h0, tol=tol)
         disk3[name]["rk_evals"] = rk_evals(disk3[name])
         d.close()
     batch = {}
     for name, prec, prop in runs[1:]:
         print(f"# held-out batch {name}", file=sys.stderr, flush=True)
-        d = P.make_batch(sizes, 4000, precision=prec, gas=True)
+        d = P.make_batch(sizes, 4000, precision=prec, gas=True, use_graph=GRAPH)
         batch[name] = P.run(d, length=args.c5_batch_length, proposer=prop, safety=args.safety, dt0=dt0, tol=tol,
                             energy_disks=range(0, len(sizes), 32))
         batch[name]["rk_evals"] = rk_evals(batch[name])
@@ -439,10 +456,10 @@ def bench_config5(args):
     # what any proposer could save against the classical controller (mixed precision), from one
     # more classical run of each workload with sample logging on
     print("# ceilings", file=sys.stderr, flush=True)
-    d, h0 = P.config3_disk()
+    d, h0 = P.config3_disk(use_graph=GRAPH)
     ceil = {"config3_disk": P.ceilings(d, args.c5_length, h0, tol)}
     d.close()
-    d = P.make_batch(sizes, 4000, gas=True)
+    d = P.make_batch(sizes, 4000, gas=True, use_graph=GRAPH)
     ceil["heldout_batch"] = P.ceilings(d, args.c5_batch_length, dt0, tol)
     d.close()
     f64 = disk3["classical_fp64"]

@@ -37,7 +37,7 @@ def batch_sizes(n_disk=512, lo=64, hi=4096, seed=4):
     return out
 
 
-def make_batch(sizes, base_seed, precision=R.RD_PREC_MIXED, gas=False, device=0):
+def make_batch(sizes, base_seed, precision=R.RD_PREC_MIXED, gas=False, device=0, use_graph=1):
     """One engine holding len(sizes) disks, disk k seeded base_seed + k. gas:
     the config-3 physics (radii 1-100 km, gas drag on every massive body,
     merging with radius factor 1, hit-centrum 0.1 AU)."""
@@ -53,18 +53,18 @@ def make_batch(sizes, base_seed, precision=R.RD_PREC_MIXED, gas=False, device=0)
     else:
         d = R.Disk(*cat, nbt, batch=True, device=device)
     d.set_precision(precision)
-    d.set_tuning(use_graph=1)
+    d.set_tuning(use_graph=use_graph)
     return d
 
 
-def config3_disk(n=131072, seed=3, precision=R.RD_PREC_MIXED, device=0):
+def config3_disk(n=131072, seed=3, precision=R.RD_PREC_MIXED, device=0, use_graph=1):
     """The config-3 disk (gas, merging, hit-centrum) and its initial step
     (min orbital period / 1000, src/red.cuda.initial/main.cu:317-342)."""
     r, v, p, b, nbt = R.make_disk(n, seed, size_mode=1, lo=1.0, hi=100.0, cd=1.0)
     d = R.Disk(r, v, p, b, nbt, threshold=(0.1, 0.0, 1.0), gas=R.gas_params(), device=device)
     d.set_drag_mode(R.RD_DRAG_ALL_MASSIVE)
     d.set_precision(precision)
-    d.set_tuning(use_graph=1)
+    d.set_tuning(use_graph=use_graph)
     a_min = np.min(np.hypot(r["x"][1:], r["y"][1:]))
     return d, 2 * np.pi * np.sqrt(a_min ** 3) / 1000.0
 
