@@ -7,15 +7,19 @@ Restated from the reference, step by step:
                          the Park-Miller LCG (Schrage's method on the signed
                          seed), 310 outputs discarded, each output r >> 1
   red_random             distribution.cpp:11-31 (MT19937, one word per call)
-  distributions          distribution.cpp:33-204
+  distributions          distribution.cpp:33-204 (uniform, exponential,
+                         Rayleigh, normal, power law, lognormal)
   presets                main.cu:846-1168 (Two_body, n_gp, n_pp, n_spl, n_pl,
                          n_tp), main.cu:571-625 + 1804-1917 (Dvorak), nebula.cpp,
                          main.cu:1467-1688 (Chambers2001), 505-569 + 1690-1802
                          (coll_stat_run), 626-700 + 1917-2003 (Hansen_2009),
                          456-503 + 2863-2911 (n_pl_to_test_anal_gd), 41-249 +
                          438-454 + 2913-2961 (solar_system), 1412-1461 +
-                         3055-3153 (Colliding_bodies); util_init.cpp:327-366
-                         (the HORIZONS columns)
+                         3055-3153 (Colliding_bodies), 702-844 + 2005-2202
+                         (Birgit_2016, Birgit_scenario), 1170-1322 + 2204-2495
+                         (GT_scenario, GT_scenario_mod), 1324-1410 + 2963-3053
+                         (Elke_binary_test); util_init.cpp:327-366 (the
+                         HORIZONS columns)
   populate_disk          main.cu:251-314, util_init.cpp:56-145
   calculate_phase        main.cu:317-342, tools.cpp:715-828, 928-933
   barycentre and units   tools.cpp:351-453
@@ -46,6 +50,7 @@ URANUS = 1.0 / 2.290298e4
 NEPTUNE = 1.0 / 1.941224e4
 MOON = 1.0 / 2.7071155e7
 CERES = 1.0 / 2.109342e9
+MOON_TO_EARTH = 1.0 / 8.130059e1
 RHO = GRAM_TO_SOLAR / ((1.0e-2 * METER_TO_AU) * (1.0e-2 * METER_TO_AU) * (1.0e-2 * METER_TO_AU))
 SIGMA = GRAM_TO_SOLAR / ((1.0e-2 * METER_TO_AU) * (1.0e-2 * METER_TO_AU))
 
@@ -123,6 +128,11 @@ class Dist:
             return rr.uniform(a[0], a[1])
         if self.kind == "rayleigh":
             return a[0] * math.sqrt(-2.0 * math.log(rr.uniform()))
+        if self.kind == "exponential":                      # distribution.cpp:90-94
+            return -1.0 / a[0] * math.log(rr.uniform())
+        if self.kind == "normal":                           # distribution.cpp:120-124: u first, then rand()
+            u = rr.uniform()
+            return a[0] + a[1] * math.sqrt(-2.0 * math.log(u)) * math.cos(1.4629180792671596E-9 * (rr.rand() + 1.0))
         if self.kind == "power_law":
             x_min, x_max, p = a
             y_min = 0.0 if p == 0.0 else math.pow(x_min, p)
@@ -219,16 +229,29 @@ def _dvorak_types(neb):
                              ("density", "uniform", 2.7 * RHO, 2.7 * RHO), ("cd", "uniform", 0.0, 0.0)])]
 
 
-def _hansen_post(bodies):
-    """main.cu:1955-1965: the first giant becomes Jupiter."""
+# (name, mass, radius in km) of the planets that drivers put in place of their
+# first giants (e.g. main.cu:1955-1965, 2265-2292, 2428-2452)
+JUPITER_SUB = ("Jupiter", JUPITER, 71492.0)
+SATURN_SUB = ("Saturn", SATURN, 60268.0)
+URANUS_SUB = ("Uranus", URANUS, 25362.0)
+
+
+def _giants_and_floor(bodies, giants, sma_floor):
+    """The drivers' loop over bodies 1..n-1 before calculate_phase: the k-th
+    giant becomes giants[k] (its HORIZONS elements, mass, radius and the
+    density from them), then a < sma_floor is raised to sma_floor."""
+    g = 0
     for b in bodies[1:]:
-        if b["type"] == GIANT:
-            b["name"] = "Jupiter"
-            b["mass"] = 1.0 * JUPITER
-            b["radius"] = 71492.0 * KM_TO_AU
+        if b["type"] == GIANT and g < len(giants):
+            name, m, r_km = giants[g]
+            g += 1
+            b["name"] = name
+            b["mass"] = 1.0 * m
+            b["radius"] = r_km * KM_TO_AU
             b["density"] = calc_density(b["mass"], b["radius"])
-            b["oe"] = horizons_oe("Jupiter")
-            return
+            b["oe"] = horizons_oe(name)
+        if sma_floor > b["oe"][0]:
+            b["oe"][0] = sma_floor
 
 
 def _colliding_post(bodies):
@@ -336,7 +359,7 @@ EXTRA = {
                                                 ("mass", "uniform", 5.0e-3 * EARTH, 5.0e-3 * EARTH),
                                                 ("density", "uniform", 2.7 * RHO, 2.7 * RHO),
                                                 ("cd", "uniform", 0.0, 0.0)])],
-                        post=_hansen_post),
+                        giants=[JUPITER_SUB]),
     "n_pl_to_test_anal_gd": dict(types=[(STAR, 1, 0, _star()),
                                         (PL, 10, 0, [("sma", "uniform", 1.0, 10.0), ("ecc", "uniform", 0.0, 0.0),
                                                      ("inc", "uniform", 0.0, 0.0), ("peri", "uniform", 0.0, 0.0),
@@ -353,13 +376,89 @@ EXTRA = {
                                                    ("density", "uniform", 2.7 * RHO, 2.7 * RHO),
                                                    ("cd", "uniform", 0.0, 0.0)])],
                              post=_colliding_post),
+    # set_parameters main.cu:702-775, driver 2005-2099
+    "Birgit_2016": dict(types=[(STAR, 1, 0, _star()),
+                               (GIANT, 2, 1, [("sma", "uniform", 5.2, 5.2), ("ecc", "uniform", 0.05, 0.05),
+                                              ("inc", "uniform", 0.0, 0.0), ("peri", "uniform", 0.0, 2.0 * PI),
+                                              ("node", "uniform", 0.0, 0.0), ("mean", "uniform", 0.0, 2.0 * PI),
+                                              ("mass", "uniform", JUPITER, JUPITER),
+                                              ("density", "uniform", 1.326 * RHO, 1.326 * RHO),
+                                              ("cd", "uniform", 0.0, 0.0)]),
+                               (PROTO, 1000, 1, [("sma", "uniform", 0.3, 2.4), ("ecc", "uniform", 0.02, 0.02),
+                                                 ("inc", "uniform", 1.0 * DEG, 1.0 * DEG),
+                                                 ("peri", "uniform", 0.0, 2.0 * PI),
+                                                 ("node", "uniform", 0.0, 2.0 * PI),
+                                                 ("mean", "uniform", 0.0, 2.0 * PI),
+                                                 ("mass", "uniform", 0.1 * MOON, 0.1 * MOON),
+                                                 ("density", "uniform", 2.0 * RHO, 2.0 * RHO),
+                                                 ("cd", "uniform", 0.0, 0.0)])],
+                        giants=[JUPITER_SUB, SATURN_SUB]),
+    # set_parameters main.cu:777-844, driver 2101-2202 (its Saturn has R = 58232 km)
+    "Birgit_scenario": dict(types=[(STAR, 1, 0, _star()),
+                                   (GIANT, 2, 1, [("sma", "uniform", 1.0, 2.0), ("ecc", "uniform", 0.0, 0.05),
+                                                  ("inc", "uniform", 0.0, 1.0 * DEG),
+                                                  ("peri", "uniform", 0.0, 2.0 * PI),
+                                                  ("node", "uniform", 0.0, 2.0 * PI),
+                                                  ("mean", "uniform", 0.0, 2.0 * PI),
+                                                  ("mass", "uniform", JUPITER, JUPITER),
+                                                  ("density", "uniform", 2.7 * RHO, 2.7 * RHO),
+                                                  ("cd", "uniform", 0.0, 0.0)]),
+                                   (TESTP, 100, 1, [("sma", "uniform", 50.0, 80.0), ("ecc", "uniform", 0.99, 0.9999),
+                                                    ("inc", "uniform", 0.0, 0.5), ("peri", "uniform", 0.0, 2.0 * PI),
+                                                    ("node", "uniform", 0.0, 2.0 * PI),
+                                                    ("mean", "uniform", 0.0, 1.0 * PI)])],
+                            giants=[JUPITER_SUB, ("Saturn", SATURN, 58232.0)]),
+    "GT_scenario": dict(types=None, gt=(2, ("sma", "normal", 1.45, 0.35)), recompute=True,
+                        giants=[JUPITER_SUB, SATURN_SUB], sma_floor=0.4),
+    "GT_scenario_mod": dict(types=None, gt=(3, ("sma", "exponential", 0.5)), recompute=True,
+                            giants=[JUPITER_SUB, SATURN_SUB, URANUS_SUB], sma_floor=0.4, names=["Sun"]),
+    # set_parameters main.cu:1324-1410, driver 2963-3053
+    "Elke_binary_test": dict(types=[(STAR, 2, 0, [("sma", "uniform", 100.0, 100.0), ("ecc", "uniform", 0.1, 0.1),
+                                                  ("inc", "uniform", 0.0, 0.0), ("peri", "uniform", 0.0, 0.0 * PI),
+                                                  ("node", "uniform", 0.0, 0.0), ("mean", "uniform", 0.0, 0.0 * PI)]
+                                    + _star()),
+                                    (GIANT, 1, 1, [("sma", "uniform", 5.2, 5.2), ("ecc", "uniform", 0.0, 0.0),
+                                                   ("inc", "uniform", 0.0, 0.0), ("peri", "uniform", 0.0, 2.0 * PI),
+                                                   ("node", "uniform", 0.0, 0.0), ("mean", "uniform", 0.0, 2.0 * PI),
+                                                   ("mass", "uniform", 1.0 * JUPITER, 1.0 * JUPITER),
+                                                   ("density", "uniform", 1.326 * RHO, 1.326 * RHO),
+                                                   ("cd", "uniform", 0.0, 0.0)]),
+                                    (PROTO, 10000, 1, [("sma", "uniform", 0.5, 4.0), ("ecc", "rayleigh", 0.02),
+                                                       ("inc", "uniform", 0.0, 0.0),
+                                                       ("peri", "uniform", 0.0, 2.0 * PI),
+                                                       ("node", "uniform", 0.0, 0.0),
+                                                       ("mean", "uniform", 0.0, 2.0 * PI),
+                                                       ("mass", "uniform", 1.0 * CERES, 1.0 * CERES),
+                                                       ("density", "uniform", 2.0 * RHO, 2.0 * RHO),
+                                                       ("cd", "uniform", 0.0, 0.0)])],
+                             names=["primary_star", "secondary_star", "Jupiter"], giants=[JUPITER_SUB], sma_floor=0.4),
 }
+
+
+def _gt_types(n_giant, sma):
+    """GT_scenario / GT_scenario_mod (main.cu:1170-1322): the giants' mass and
+    every density are uniform(x, 0) -- drawn as x + (0 - x) u, as a build with
+    NDEBUG does (distribution.cpp's assert(x_min <= x_max) stops one without)."""
+    m_pp = 1.0 / 2.0 * MOON_TO_EARTH * EARTH
+    return [(STAR, 1, 0, _star()),
+            (GIANT, n_giant, 1, [("sma", "uniform", 1.0, 2.00), ("ecc", "uniform", 0.0, 0.05),
+                                 ("inc", "uniform", 0.0, 1.0 * DEG), ("peri", "uniform", 0.0, 2.0 * PI),
+                                 ("node", "uniform", 0.0, 2.0 * PI), ("mean", "uniform", 0.0, 2.0 * PI),
+                                 ("mass", "uniform", 1.0 * JUPITER, 0.0), ("density", "uniform", 2.7 * RHO, 0.0),
+                                 ("cd", "uniform", 0.0, 0.0)]),
+            (PROTO, 2000, 1, [sma, ("ecc", "uniform", 0.0, 0.05), ("inc", "uniform", 0.0, 1.0 * DEG),
+                              ("peri", "uniform", 0.0, 2.0 * PI), ("node", "uniform", 0.0, 2.0 * PI),
+                              ("mean", "uniform", 0.0, 2.0 * PI), ("mass", "uniform", m_pp, m_pp),
+                              ("density", "uniform", 2.7 * RHO, 0.0), ("cd", "uniform", 0.0, 0.0)])]
 
 
 def scenario(name):
     """The generation recipe of a preset, as a dict."""
     if name in EXTRA:
-        return EXTRA[name]
+        sc = dict(EXTRA[name])
+        if sc.get("gt"):
+            sc["types"] = _gt_types(*sc["gt"])
+        return sc
     types, recompute, scale = PRESETS[name]
     return dict(types=types, recompute=recompute, mmsn="scale" if scale else None, neb=DVORAK_NEB)
 
@@ -496,11 +595,12 @@ def generate(name, seed):
                     b["density"] = calc_density(b["mass"], b["radius"])
                 if "radius" not in D and "density" in D:
                     b["radius"] = calc_radius(b["mass"], b["density"])
-    if sc.get("names") == "chambers":
-        for b, nm in zip(bodies, _chambers_names()):
-            b["name"] = nm
+    names = sc.get("names")
+    for b, nm in zip(bodies, _chambers_names() if names == "chambers" else (names or [])):
+        b["name"] = nm
     if sc.get("post"):
         sc["post"](bodies)
+    _giants_and_floor(bodies, sc.get("giants", []), sc.get("sma_floor", 0.0))
     return _phase_bc_units(bodies, n_type)
 
 

@@ -18,7 +18,9 @@
 //     range check of generate_oe, util_init.cpp:56-66, never rejects: its
 //     condition x_min > x && x_max < x cannot hold, and this keeps that);
 //   - for Dvorak, the protoplanet masses scaled to the MMSN solid mass
-//     (nebula.cpp) and radii recomputed (main.cu:1841-1880);
+//     (nebula.cpp) and radii recomputed (main.cu:1841-1880); the other
+//     drivers' own steps (equal MMSN shares, the real Jupiter, Saturn and
+//     Uranus in place of the first giants, the 0.4 AU floor on a);
 //   - calculate_phase (main.cu:317-342) with tools::calc_phase and its Kepler
 //     solver as written (tools.cpp:715-828, including the shortcut that
 //     returns E = M for M > pi), dt = shortest period / 1000;
@@ -65,6 +67,7 @@ constexpr double JupiterToSolar = 1.0 / 1.0473486e3;
 constexpr double UranusToSolar = 1.0 / 2.290298e4;
 constexpr double NeptuneToSolar = 1.0 / 1.941224e4;
 constexpr double CeresToSolar = 1.0 / 2.109342e9;
+constexpr double MoonToEarth = 1.0 / 8.130059e1;
 constexpr double AuToMeter = 1.495978707e11;
 constexpr double AuToKilometer = AuToMeter / 1.0e3;
 constexpr double MeterToAu = 1.0 / AuToMeter;
@@ -203,9 +206,51 @@ struct Nebula {                  // nebula.cpp: solid component of a MMSN
     }
 };
 
+struct Vec { double x, y, z; };
+struct Oe { double sma, ecc, inc, peri, node, mean; };
+
+// Osculating elements of the planets on 2015-May-11 (JPL HORIZONS; the values
+// of main.cu:41-83), as extract_from_horizon_output reads them from their
+// fixed columns (util_init.cpp:327-366): e, i, Omega, omega, M in degrees, a
+// in AU.  The degrees become radians as there.
+struct Horizons {
+    double ecc, inc, node, peri, mean, sma;
+};
+constexpr Horizons kMercury{2.056283170656704E-01, 7.004033081870772E+00, 4.831135275416084E+01,
+                            2.917018579009503E+01, 8.665226794919347E+01, 3.870990540148295E-01};
+constexpr Horizons kVenus{6.761247965610893E-03, 3.394467689974794E+00, 7.663950855912078E+01,
+                          5.463888630600258E+01, 3.625130289912041E+01, 7.233287920706470E-01};
+constexpr Horizons kEarth{1.706151645376220E-02, 2.832604859699792E-03, 1.991276585462631E+02,
+                          2.644902808333621E+02, 1.245850770307868E+02, 9.999423845001243E-01};
+constexpr Horizons kMars{9.345598920376896E-02, 1.848403968432629E+00, 4.951276588949865E+01,
+                         2.865318030175783E+02, 7.844645806929105E+01, 1.523589291796774E+00};
+constexpr Horizons kJupiter{4.895926045495331E-02, 1.303927595561599E+00, 1.005220570364609E+02,
+                            2.736235124666692E+02, 1.262463884135522E+02, 5.201472759810757E+00};
+constexpr Horizons kSaturn{5.370451604891303E-02, 2.486273622454862E+00, 1.136214918787423E+02,
+                           3.402344971420399E+02, 1.437153605146956E+02, 9.553843040430950E+00};
+constexpr Horizons kUranus{4.928987553618186E-02, 7.717997843738144E-01, 7.400078543435393E+01,
+                           9.677390970582711E+01, 2.083879289018873E+02, 1.915532588905842E+01};
+constexpr Horizons kNeptune{8.258758489207289E-03, 1.765657591232885E+00, 1.317164628477979E+02,
+                            2.932421741299731E+02, 2.734992366503029E+02, 2.997852567031729E+01};
+
+Oe oe_of(const Horizons& h)
+{
+    return Oe{h.sma, h.ecc, h.inc * cst::DegreeToRadian, h.peri * cst::DegreeToRadian,
+              h.node * cst::DegreeToRadian, h.mean * cst::DegreeToRadian};
+}
+
 struct Disk;
 
 enum MmsnMass { MMSN_NONE = 0, MMSN_SCALE = 1, MMSN_SHARE = 2 };
+
+// A giant the driver replaces with a real planet before calculate_phase:
+// name, mass, radius and the HORIZONS elements of 2015-May-11 (the density
+// follows from mass and radius).
+struct GiantSub {
+    const char* name;
+    double mass, radius_km;
+    const Horizons* oe;
+};
 
 struct Preset {
     Preset(std::string n, std::vector<TypeSpec> t = {}, bool rc = false, int mm = MMSN_NONE, Nebula nb = Nebula{})
@@ -220,10 +265,11 @@ struct Preset {
     std::vector<std::string> names;   // the reference's name list when it is not the types' own
     void (*post)(Disk&) = nullptr;    // the driver's own changes before calculate_phase
     bool table = false;          // bodies from a table (no distributions): the solar system
+    std::vector<GiantSub> giants;     // the first giants, in order, become these planets
+    double sma_floor = 0.0;      // a < sma_floor is raised to it (bodies 1..n-1), after the giants
 };
 
 // The drivers' own changes between populate_disk and calculate_phase.
-void hansen_post(Disk& D);
 void colliding_post(Disk& D);
 std::string create_name(int i, int type);
 
@@ -304,6 +350,9 @@ std::vector<Preset> presets()
     // set_parameters::Dvorak (main.cu:571-625): 300 protoplanets on the MMSN
     // solid disk `neb` (the power law of a takes its r_1, r_2 and p)
     const double sigma = cst::GramPerCm2ToSolarPerAu2;
+    const GiantSub jupiter{"Jupiter", cst::JupiterToSolar, 71492.0, &kJupiter},
+        saturn{"Saturn", cst::SaturnToSolar, 60268.0, &kSaturn},
+        uranus{"Uranus", cst::UranusToSolar, 25362.0, &kUranus};
     auto dvorak = [&](const std::string& name, const Nebula& neb) {
         Preset p{name, {}, true, MMSN_SCALE, neb};
         p.types = {star,
@@ -373,7 +422,7 @@ std::vector<Preset> presets()
                     {MASS, K::UNIFORM, 5.0e-3 * cst::EarthToSolar, 5.0e-3 * cst::EarthToSolar, 0, 0},
                     {DENSITY, K::UNIFORM, 2.7 * rho, 2.7 * rho, 0, 0},
                     {CD, K::UNIFORM, 0.0, 0.0, 0, 0}}}}};
-        p.post = hansen_post;
+        p.giants = {jupiter};
         v.push_back(p);
     }
     // n_pl_to_test_anal_gd (set_parameters main.cu:456-503, driver
@@ -411,6 +460,119 @@ std::vector<Preset> presets()
         p.post = colliding_post;
         v.push_back(p);
     }
+    const double deg = cst::DegreeToRadian;
+    // Birgit_2016 (set_parameters main.cu:702-775, driver 2005-2099): Jupiter and
+    // Saturn, 1000 tenth-Moon protoplanets on 0.3-2.4 AU
+    {
+        Preset p{"Birgit_2016",
+                 {star,
+                  {GIANT, 2, 1,
+                   {{SMA, K::UNIFORM, 5.2, 5.2, 0, 0}, {ECC, K::UNIFORM, 0.05, 0.05, 0, 0},
+                    {INC, K::UNIFORM, 0.0, 0.0, 0, 0}, {PERI, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {NODE, K::UNIFORM, 0.0, 0.0, 0, 0}, {MEAN, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {MASS, K::UNIFORM, cst::JupiterToSolar, cst::JupiterToSolar, 0, 0},
+                    {DENSITY, K::UNIFORM, 1.326 * rho, 1.326 * rho, 0, 0},
+                    {CD, K::UNIFORM, 0.0, 0.0, 0, 0}}},
+                  {PROTO, 1000, 1,
+                   {{SMA, K::UNIFORM, 0.3, 2.4, 0, 0}, {ECC, K::UNIFORM, 0.02, 0.02, 0, 0},
+                    {INC, K::UNIFORM, 1.0 * deg, 1.0 * deg, 0, 0}, {PERI, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {NODE, K::UNIFORM, 0.0, 2.0 * PI, 0, 0}, {MEAN, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {MASS, K::UNIFORM, 0.1 * cst::MoonToSolar, 0.1 * cst::MoonToSolar, 0, 0},
+                    {DENSITY, K::UNIFORM, 2.0 * rho, 2.0 * rho, 0, 0},
+                    {CD, K::UNIFORM, 0.0, 0.0, 0, 0}}}}};
+        p.giants = {jupiter, saturn};
+        v.push_back(p);
+    }
+    // Birgit_scenario (set_parameters main.cu:777-844, driver 2101-2202): Jupiter,
+    // a Saturn of radius 58232 km, and 100 comets (test particles, e 0.99-0.9999)
+    {
+        Preset p{"Birgit_scenario",
+                 {star,
+                  {GIANT, 2, 1,
+                   {{SMA, K::UNIFORM, 1.0, 2.0, 0, 0}, {ECC, K::UNIFORM, 0.0, 0.05, 0, 0},
+                    {INC, K::UNIFORM, 0.0, 1.0 * deg, 0, 0}, {PERI, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {NODE, K::UNIFORM, 0.0, 2.0 * PI, 0, 0}, {MEAN, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {MASS, K::UNIFORM, cst::JupiterToSolar, cst::JupiterToSolar, 0, 0},
+                    {DENSITY, K::UNIFORM, 2.7 * rho, 2.7 * rho, 0, 0},
+                    {CD, K::UNIFORM, 0.0, 0.0, 0, 0}}},
+                  {TESTP, 100, 1,
+                   {{SMA, K::UNIFORM, 50.0, 80.0, 0, 0}, {ECC, K::UNIFORM, 0.99, 0.9999, 0, 0},
+                    {INC, K::UNIFORM, 0.0, 0.5, 0, 0}, {PERI, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {NODE, K::UNIFORM, 0.0, 2.0 * PI, 0, 0}, {MEAN, K::UNIFORM, 0.0, 1.0 * PI, 0, 0}}}}};
+        p.giants = {jupiter, GiantSub{"Saturn", cst::SaturnToSolar, 58232.0, &kSaturn}};
+        v.push_back(p);
+    }
+    // GT_scenario (set_parameters main.cu:1170-1245, driver 2204-2345) and
+    // GT_scenario_mod (1247-1322, 2347-2495): giants on 1-2 AU replaced by the
+    // real ones, 2000 half-Moon protoplanets, a < 0.4 AU raised to 0.4.  The
+    // giants' mass and every density are set up as uniform(x, 0), which
+    // distribution.cpp's assert(x_min <= x_max) rejects in a build without
+    // NDEBUG (the Makefile's); with it (the Visual Studio Release builds) the
+    // draw is x + (0 - x) u, as here.
+    auto gt = [&](const std::string& name, uint32_t n_giant, Entry sma) {
+        const double m_pp = 1.0 / 2.0 * cst::MoonToEarth * cst::EarthToSolar;
+        Preset p{name,
+                 {star,
+                  {GIANT, n_giant, 1,
+                   {{SMA, K::UNIFORM, 1.0, 2.00, 0, 0}, {ECC, K::UNIFORM, 0.0, 0.05, 0, 0},
+                    {INC, K::UNIFORM, 0.0, 1.0 * deg, 0, 0}, {PERI, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {NODE, K::UNIFORM, 0.0, 2.0 * PI, 0, 0}, {MEAN, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {MASS, K::UNIFORM, 1.0 * cst::JupiterToSolar, 0.0, 0, 0},
+                    {DENSITY, K::UNIFORM, 2.7 * rho, 0.0, 0, 0},
+                    {CD, K::UNIFORM, 0.0, 0.0, 0, 0}}},
+                  {PROTO, 2000, 1,
+                   {sma, {ECC, K::UNIFORM, 0.0, 0.05, 0, 0},
+                    {INC, K::UNIFORM, 0.0, 1.0 * deg, 0, 0}, {PERI, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {NODE, K::UNIFORM, 0.0, 2.0 * PI, 0, 0}, {MEAN, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {MASS, K::UNIFORM, m_pp, m_pp, 0, 0},
+                    {DENSITY, K::UNIFORM, 2.7 * rho, 0.0, 0, 0},
+                    {CD, K::UNIFORM, 0.0, 0.0, 0, 0}}}},
+                 true};
+        p.sma_floor = 0.4;
+        return p;
+    };
+    {
+        Preset p = gt("GT_scenario", 2, Entry{SMA, K::NORMAL, 1.45, 0.35, 0, 0});
+        p.giants = {jupiter, saturn};
+        v.push_back(p);
+    }
+    {
+        Preset p = gt("GT_scenario_mod", 3, Entry{SMA, K::EXPONENTIAL, 0.5, 0, 0, 0});
+        p.giants = {jupiter, saturn, uranus};
+        p.names.push_back("Sun");
+        v.push_back(p);
+    }
+    // Elke_binary_test (set_parameters main.cu:1324-1410, driver 2963-3053): a
+    // second solar-mass star on a 100 AU orbit (e = 0.1), Jupiter, and 10000
+    // Ceres-mass protoplanets on 0.5-4 AU.  The eccentricity's range [0, 0.2]
+    // redraws nothing: util_init.cpp:63 repeats while x_min > e && x_max < e,
+    // which no e satisfies.
+    {
+        Preset p{"Elke_binary_test",
+                 {{STAR, 2, 0,
+                   {{SMA, K::UNIFORM, 100.0, 100.0, 0, 0}, {ECC, K::UNIFORM, 0.1, 0.1, 0, 0},
+                    {INC, K::UNIFORM, 0.0, 0.0, 0, 0}, {PERI, K::UNIFORM, 0.0, 0.0 * PI, 0, 0},
+                    {NODE, K::UNIFORM, 0.0, 0.0, 0, 0}, {MEAN, K::UNIFORM, 0.0, 0.0 * PI, 0, 0},
+                    star_mass, star_radius, star_cd}},
+                  {GIANT, 1, 1,
+                   {{SMA, K::UNIFORM, 5.2, 5.2, 0, 0}, {ECC, K::UNIFORM, 0.0, 0.0, 0, 0},
+                    {INC, K::UNIFORM, 0.0, 0.0, 0, 0}, {PERI, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {NODE, K::UNIFORM, 0.0, 0.0, 0, 0}, {MEAN, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {MASS, K::UNIFORM, 1.0 * cst::JupiterToSolar, 1.0 * cst::JupiterToSolar, 0, 0},
+                    {DENSITY, K::UNIFORM, 1.326 * rho, 1.326 * rho, 0, 0},
+                    {CD, K::UNIFORM, 0.0, 0.0, 0, 0}}},
+                  {PROTO, 10000, 1,
+                   {{SMA, K::UNIFORM, 0.5, 4.0, 0, 0}, {ECC, K::RAYLEIGH, 0.02, 0, 0, 0},
+                    {INC, K::UNIFORM, 0.0, 0.0, 0, 0}, {PERI, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {NODE, K::UNIFORM, 0.0, 0.0, 0, 0}, {MEAN, K::UNIFORM, 0.0, 2.0 * PI, 0, 0},
+                    {MASS, K::UNIFORM, 1.0 * cst::CeresToSolar, 1.0 * cst::CeresToSolar, 0, 0},
+                    {DENSITY, K::UNIFORM, 2.0 * rho, 2.0 * rho, 0, 0},
+                    {CD, K::UNIFORM, 0.0, 0.0, 0, 0}}}}};
+        p.names = {"primary_star", "secondary_star", "Jupiter"};
+        p.giants = {jupiter};
+        p.sma_floor = 0.4;
+        v.push_back(p);
+    }
     return v;
 }
 
@@ -443,8 +605,6 @@ double kepler(double ecc, double mean, double eps)
     return E;
 }
 
-struct Vec { double x, y, z; };
-struct Oe { double sma, ecc, inc, peri, node, mean; };
 
 // calc_phase (tools.cpp:789-828)
 void calc_phase(double mu, const Oe& oe, Vec& r, Vec& v)
@@ -578,6 +738,21 @@ Disk generate(const Preset& P, uint32_t seed)
     }
     for (size_t i = 0; i < P.names.size() && i < D.bodies.size(); ++i) D.bodies[i].name = P.names[i];
     if (P.post) P.post(D);
+    // the drivers' loop before calculate_phase (e.g. main.cu:1955-1965, 2265-2292):
+    // the first giants become the listed planets, then the floor on a
+    size_t g = 0;
+    for (size_t i = 1; i < D.bodies.size(); ++i) {
+        Body& b = D.bodies[i];
+        if (b.type == GIANT && g < P.giants.size()) {
+            const GiantSub& G = P.giants[g++];
+            b.name = G.name;
+            b.mass = 1.0 * G.mass;
+            b.radius = G.radius_km * cst::KilometerToAu;
+            b.density = calc_density(b.mass, b.radius);
+            b.oe = oe_of(*G.oe);
+        }
+        if (P.sma_floor > b.oe.sma) b.oe.sma = P.sma_floor;
+    }
     return finish(std::move(D));
 }
 
@@ -622,51 +797,6 @@ Disk finish(Disk D)
     D.t0 *= cst::Gauss;
     D.dt *= cst::Gauss;
     return D;
-}
-
-// Osculating elements of the planets on 2015-May-11 (JPL HORIZONS; the values
-// of main.cu:41-83), as extract_from_horizon_output reads them from their
-// fixed columns (util_init.cpp:327-366): e, i, Omega, omega, M in degrees, a
-// in AU.  The degrees become radians as there.
-struct Horizons {
-    double ecc, inc, node, peri, mean, sma;
-};
-constexpr Horizons kMercury{2.056283170656704E-01, 7.004033081870772E+00, 4.831135275416084E+01,
-                            2.917018579009503E+01, 8.665226794919347E+01, 3.870990540148295E-01};
-constexpr Horizons kVenus{6.761247965610893E-03, 3.394467689974794E+00, 7.663950855912078E+01,
-                          5.463888630600258E+01, 3.625130289912041E+01, 7.233287920706470E-01};
-constexpr Horizons kEarth{1.706151645376220E-02, 2.832604859699792E-03, 1.991276585462631E+02,
-                          2.644902808333621E+02, 1.245850770307868E+02, 9.999423845001243E-01};
-constexpr Horizons kMars{9.345598920376896E-02, 1.848403968432629E+00, 4.951276588949865E+01,
-                         2.865318030175783E+02, 7.844645806929105E+01, 1.523589291796774E+00};
-constexpr Horizons kJupiter{4.895926045495331E-02, 1.303927595561599E+00, 1.005220570364609E+02,
-                            2.736235124666692E+02, 1.262463884135522E+02, 5.201472759810757E+00};
-constexpr Horizons kSaturn{5.370451604891303E-02, 2.486273622454862E+00, 1.136214918787423E+02,
-                           3.402344971420399E+02, 1.437153605146956E+02, 9.553843040430950E+00};
-constexpr Horizons kUranus{4.928987553618186E-02, 7.717997843738144E-01, 7.400078543435393E+01,
-                           9.677390970582711E+01, 2.083879289018873E+02, 1.915532588905842E+01};
-constexpr Horizons kNeptune{8.258758489207289E-03, 1.765657591232885E+00, 1.317164628477979E+02,
-                            2.932421741299731E+02, 2.734992366503029E+02, 2.997852567031729E+01};
-
-Oe oe_of(const Horizons& h)
-{
-    return Oe{h.sma, h.ecc, h.inc * cst::DegreeToRadian, h.peri * cst::DegreeToRadian,
-              h.node * cst::DegreeToRadian, h.mean * cst::DegreeToRadian};
-}
-
-// Hansen_2009 (main.cu:1955-1965): the first giant becomes Jupiter
-void hansen_post(Disk& D)
-{
-    for (size_t i = 1; i < D.bodies.size(); ++i) {
-        Body& b = D.bodies[i];
-        if (b.type != GIANT) continue;
-        b.name = "Jupiter";
-        b.mass = 1.0 * cst::JupiterToSolar;
-        b.radius = 71492.0 * cst::KilometerToAu;
-        b.density = calc_density(b.mass, b.radius);
-        b.oe = oe_of(kJupiter);
-        return;
-    }
 }
 
 // Colliding_bodies (main.cu:3090-3115): the first two protoplanets on one

@@ -140,7 +140,8 @@ def test_cli_backward_run_matches_oracle(tmp_path):
 
 @pytest.mark.parametrize("preset,length", [("n_gp", "2.0"), ("solar_system", "1.0"), ("Hansen_2009", "0.5"),
                                            ("coll_stat_run", "0.25"), ("Chambers2001", "0.5"),
-                                           ("n_pl_to_test_anal_gd", "1.0"), ("Colliding_bodies", "1.0")])
+                                           ("n_pl_to_test_anal_gd", "1.0"), ("Colliding_bodies", "1.0"),
+                                           ("Birgit_scenario", "0.05"), ("GT_scenario_mod", "0.01")])
 def test_redinit_then_red(tmp_path, preset, length):
     # red's initial-condition generator writes a run directory that red reads;
     # RKF7(8) over `length` years against the oracle's driver loop, bit for bit

@@ -71,7 +71,10 @@ def read_outputs(d, stem):
                                          ("coll_stat_run", 12345), ("coll_stat_run", 2 ** 32 - 1),
                                          ("Hansen_2009", 5), ("Hansen_2009", 1733170001),
                                          ("n_pl_to_test_anal_gd", 17), ("solar_system", 1733170002),
-                                         ("Colliding_bodies", 3), ("Colliding_bodies", 99)])
+                                         ("Colliding_bodies", 3), ("Colliding_bodies", 99),
+                                         ("Birgit_2016", 11), ("Birgit_scenario", 12), ("Birgit_scenario", 1733170003),
+                                         ("GT_scenario", 13), ("GT_scenario_mod", 14), ("GT_scenario_mod", 2 ** 31 - 1),
+                                         ("Elke_binary_test", 15)])
 def test_preset_matches_oracle(tmp_path, preset, seed):
     run(tmp_path, preset, seed)
     info, rows, oe, seed_txt, start = read_outputs(str(tmp_path), preset)
@@ -254,3 +257,63 @@ def test_chambers2001_names_and_colliding_pair(tmp_path):
     r_p = 2.7 * (1 - 0.05)
     chord = r_p * dM * (1 + 0.05) ** 2 / (1 - 0.05 ** 2) ** 1.5
     assert abs(np.linalg.norm(p - t) / chord - 1.0) < 1e-4
+
+
+def test_normal_and_exponential_samplers_have_their_moments():
+    """distribution.cpp's normal (mean, 'variance' used as the standard
+    deviation, Box-Muller's cosine branch) and exponential (rate lambda)
+    samplers, against their closed-form moments over 200000 draws."""
+    n = 200000
+    d = RO.Dist(4242, "normal", 1.45, 0.35)
+    x = np.array([d.next() for _ in range(n)])
+    assert abs(x.mean() - 1.45) < 5 * 0.35 / math.sqrt(n)
+    assert abs(x.std() / 0.35 - 1.0) < 0.01
+    assert abs(np.mean(x < 1.45 - 0.35) - 0.158655) < 0.005        # Phi(-1)
+    d = RO.Dist(4243, "exponential", 0.5)
+    y = np.array([d.next() for _ in range(n)])
+    assert abs(y.mean() - 2.0) < 5 * 2.0 / math.sqrt(n)
+    assert abs(np.mean(y > 2.0) - math.exp(-1.0)) < 0.005          # P(Y > 1/lambda) = 1/e
+
+
+@needs_bin
+def test_gt_scenarios_floor_and_giants(tmp_path):
+    """Both GT presets: giants replaced by Jupiter, Saturn (and Uranus) in
+    order, every other body's a at least 0.4 AU (the driver's floor), and the
+    protoplanets' half-Moon masses."""
+    for preset, giants in (("GT_scenario", ["Jupiter", "Saturn"]), ("GT_scenario_mod", ["Jupiter", "Saturn", "Uranus"])):
+        run(tmp_path, preset, 21)
+        _, rows, oe, _, _ = read_outputs(str(tmp_path), preset)
+        assert [r[0] for r in rows[1:1 + len(giants)]] == giants
+        assert rows[0][0] == ("Sun" if preset == "GT_scenario_mod" else "star")
+        sma = np.array([float(line.split()[7]) for line in oe[1:]])
+        assert sma.min() >= 0.4
+        m = np.array([float(r[5]) for r in rows[1 + len(giants):]])
+        assert len(m) == 2000 and np.all(m == 1.0 / 2.0 * (1.0 / 8.130059e1) * (1.0 / 3.3294605e5))
+        # GT_scenario's normal(1.45, 0.35) puts ~0.4% below the floor; the exponential's ~18%
+        frac = np.mean(sma[len(giants):] == 0.4)
+        assert (frac < 0.02) if preset == "GT_scenario" else (0.1 < frac < 0.3)
+
+
+@needs_bin
+def test_elke_binary_and_birgit_comets(tmp_path):
+    """Elke_binary_test: the secondary star on its a = 100 AU, e = 0.1 orbit
+    about the primary; Birgit_scenario: 100 comets on a = 50-80 AU with
+    e = 0.99-0.9999, massless, next to Jupiter and a Saturn of 58232 km."""
+    run(tmp_path, "Elke_binary_test", 23)
+    _, rows, _, _, _ = read_outputs(str(tmp_path), "Elke_binary_test")
+    assert [r[0] for r in rows[:3]] == ["primary_star", "secondary_star", "Jupiter"]
+    assert [int(r[2]) for r in rows[:3]] == [0, 0, 1] and len(rows) == 10003
+    st = [[float(x) for x in r[9:15]] for r in rows[:2]]
+    rel = [st[1][c] - st[0][c] for c in range(3)]
+    vel = [(st[1][3 + c] - st[0][3 + c]) * RO.GAUSS for c in range(3)]
+    a, e, inc, *_ = elements(rel, vel, RO.K2 * 2.0)
+    assert abs(a / 100.0 - 1) < 1e-12 and abs(e - 0.1) < 1e-12 and inc == 0.0
+    run(tmp_path, "Birgit_scenario", 23)
+    _, rows, oe, _, _ = read_outputs(str(tmp_path), "Birgit_scenario")
+    assert [r[0] for r in rows[:3]] == ["star", "Jupiter", "Saturn"]
+    assert abs(float(rows[2][6]) - 58232.0 / 1.495978707e8) < 1e-18
+    tp = [line.split() for line in oe[3:]]
+    assert len(tp) == 100 and all(float(t[3]) == 0.0 for t in tp)
+    a = np.array([float(t[7]) for t in tp])
+    e = np.array([float(t[8]) for t in tp])
+    assert a.min() >= 50.0 and a.max() <= 80.0 and e.min() >= 0.99 and e.max() <= 0.9999
