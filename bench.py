@@ -263,8 +263,10 @@ def energy_horizon_leg(parts, tol, tuning, seconds, h0, orbits=1000.0, period_da
     ts, es = np.array(ts), np.array(es)
     rate = float((ts * es).sum() / (ts * ts).sum())        # per time unit
     tried, passed, _ = it.counters()
+    _, pairs = it.work()
     out = {"tol": tol, "period_days": period_days, "orbits_target": orbits, "orbits_run": float(ts[-1] / P),
            "steps": passed, "attempts": tried, "wall_s": wall, "steps_per_s": passed / wall,
+           "pair_interactions_per_s": pairs / wall,
            "rel_energy_error_at_end": float(es[-1]), "max_rel_energy_error": float(es.max()),
            "drift_per_orbit": rate * P, "extrapolated_rel_error": rate * P * orbits,
            "ok": bool(rate * P * orbits <= 1e-9),
@@ -372,6 +374,7 @@ def bench_config4(args):
     d.set_tuning(use_graph=GRAPH)
     steps = args.steps if args.steps != 1000 else 100
     it = R.Integrator(d, R.RD_RKF78, 0.1 * R.K_GAUSS, adaptive=True, tol=args.tol)
+    it.set_overlap(not args.no_overlap)
     probe = list(range(0, len(sizes), 32))          # energy checked on every 32nd disk, from t = 0
     E0 = [d.integral(k)["E"] for k in probe]
     it.steps(args.warmup)
@@ -575,6 +578,9 @@ def main():
                          "that keeps the N = 16384 disk within 1e-9 over 10^3 orbits at 1 AU, by a 40 s scan "
                          "over 1e-11, 7e-12, 5e-12 and 4e-12)")
     ap.add_argument("--no-twin", action="store_true", help="config 2: skip the FP64 twin run")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="configs 2 and 4: the host-only step sequence (no device-side accept, commit and k0 "
+                         "pass while the host computes the step size)")
     ap.add_argument("--profile-run", action="store_true",
                     help="the timed steps only (no CPU leg, FP64 comparisons, twin, energy run or horizon): "
                          "for profilers")
@@ -621,6 +627,7 @@ def main():
         tuning = d.autotune() if (args.precision == "mixed" and not args.no_autotune) else d.tuning()
     d.set_tuning(use_graph=GRAPH)
     it = R.Integrator(d, R.RD_RKF78, h0, adaptive=True, tol=args.tol)
+    it.set_overlap(not args.no_overlap)
     E_init = d.integral()["E"]
     print(f"# warmup {args.warmup} steps", file=sys.stderr, flush=True)
     it.steps(args.warmup)

@@ -215,6 +215,15 @@ int rd_set_error_check_for_tp(rd_integrator* it, int on);
  * their step, then rd_step / rd_steps / rd_run return RD_ESTATE naming the
  * first failing disk. */
 int rd_integrator_set_guard(rd_integrator* it, uint32_t max_attempts, double min_dt, int nonfinite);
+/* No reference counterpart (an execution option, results unchanged): in
+ * rd_steps, after each adaptive attempt the device makes the accept decision
+ * with the host's arithmetic (|lambda h max| > tol rejects,
+ * int_rungekutta8.cu:693-708), commits accepted disks and starts the next
+ * step's stage-0 pair pass while the host reads the error and computes the
+ * next step size, which it still does with its own pow.  On by default; 0
+ * restores the host-only sequence.  rd_step and rd_run, which may handle
+ * events between steps, never overlap. */
+int rd_integrator_set_overlap(rd_integrator* it, int on);
 /* Number of pair-kernel launches and the ordered pairs they summed so far. */
 int rd_work_counters(rd_integrator* it, uint64_t* force_evals, double* pairs);
 

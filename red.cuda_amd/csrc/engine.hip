@@ -118,6 +118,7 @@ struct EventPair {
 struct rd_disk {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipEvent_t ev_err = nullptr;   // the error words of an attempt are on the host
     int n_cu = 256;
     uint32_t n_disk = 0, n_all = 0;
     std::vector<HostDisk> hd;
@@ -216,6 +217,9 @@ struct rd_integrator {
     uint32_t max_attempts = 0;
     double min_dt = 0.0;
     bool reject_nonfinite = false;
+    // rd_steps: accept, commit and the next k0 pass on the device while the
+    // host reads the error (same arithmetic, so the same results)
+    bool overlap = true;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     uint64_t graph_gen = ~0ull;
@@ -695,7 +699,7 @@ hipError_t enqueue_attempt(rd_integrator* it)
     hipError_t e;
     {
         StageArgs a = make_args(d, 0, it->adaptive, it->err_all);
-        if ((e = enqueue_force(d, d->x, d->y, d->z, F_NEED_K0)) != hipSuccess) return e;
+        if ((e = enqueue_force(d, d->x, d->y, d->z, F_K0_FORCE)) != hipSuccess) return e;
         if ((e = launch_stage(d->stream, it->method, 0, a)) != hipSuccess) return e;
         if (it->n_stage > 1) {
             StageArgs a1 = make_args(d, 0, it->adaptive, it->err_all);   // comb1 writes P[1]
@@ -873,6 +877,7 @@ int rd_disk_destroy(rd_disk* d)
     d->integrators.clear();
     (void)hipStreamSynchronize(d->stream);
     free_buffers(d);
+    if (d->ev_err) (void)hipEventDestroy(d->ev_err);
     (void)hipStreamDestroy(d->stream);
     delete d;
     return RD_OK;
@@ -1333,20 +1338,31 @@ static int after_accept(rd_integrator* it, const std::vector<char>& acc, const s
 // attempt for each disk that still owes steps, so a disk that accepts starts
 // its next step in the next round instead of waiting for the disks that
 // rejected; each disk's own sequence of attempts is the same either way.
-static int step_disks(rd_integrator* it, const std::vector<uint32_t>& want)
+// With `overlap` (rd_steps), an attempt is followed on the stream by the
+// device's own accept decision (k_decide, the host's arithmetic), the commit
+// of accepted disks and, for disks that go on, the next step's stage-0 pair
+// pass; the host waits only for the error words, so its step-size control
+// runs while that pass does.  The next attempt then skips its stage-0 pair
+// pass for those disks (F_K0_FORCE clear).  The operations and their inputs
+// are the same, so the results are too.
+static int step_disks(rd_integrator* it, const std::vector<uint32_t>& want, bool overlap = false)
 {
     rd_disk* d = it->d;
     if (int rc = ensure_lists(d)) return rc;
     const uint32_t nd = d->n_disk;
+    const bool early = overlap && it->overlap && it->adaptive;
+    if (early && !d->ev_err) HIP_TRY(hipEventCreateWithFlags(&d->ev_err, hipEventDisableTiming));
     std::vector<uint32_t> remaining(want);
     std::vector<char> pending(nd);
     for (uint32_t k = 0; k < nd; ++k) pending[k] = remaining[k] > 0;
     std::vector<int> iter(nd, 0);
     std::vector<char> need_k0(nd, 1);
+    std::vector<char> k0_ahead(nd, 0);   // the stage-0 pair pass already ran on the device
     for (uint32_t k = 0; k < nd; ++k)
         if (pending[k] && !it->fixed_dt) it->dt_try[k] = it->dt_next[k];
     const double lambda = it->method == RD_RKF78 ? 41.0 / 840.0 : 1.0 / 10.0;
     uint32_t commit_flags_pending = 0;
+    bool any_acc = false;
     std::vector<uint32_t> commit(nd, 0);
     std::vector<char> acc(nd, 0);
     std::vector<double> err_ratio(nd, 0.0);
@@ -1365,10 +1381,12 @@ static int step_disks(rd_integrator* it, const std::vector<uint32_t>& want)
             }
             uint32_t fl = commit[k];
             if (pending[k]) {
-                fl |= F_ACTIVE | (need_k0[k] ? F_NEED_K0 : 0u);
+                fl |= F_ACTIVE | (need_k0[k] ? F_NEED_K0 : 0u) | (need_k0[k] && !k0_ahead[k] ? F_K0_FORCE : 0u) |
+                      (early && remaining[k] > 1 ? F_MORE : 0u);
                 it->dt_did[k] = it->dt_try[k];
                 any = true;
             }
+            k0_ahead[k] = 0;
             d->h_ctl[k] = DiskCtl{d->hd[k].t, it->dt_try[k], fl, 0};
         }
         HIP_TRY(hipMemcpyAsync(d->ctl, d->h_ctl, nd * sizeof(DiskCtl), hipMemcpyHostToDevice, d->stream));
@@ -1394,7 +1412,17 @@ static int step_disks(rd_integrator* it, const std::vector<uint32_t>& want)
         else if (it->adaptive)   // max bits and non-finite flags of every disk
             HIP_TRY(hipMemcpyAsync(d->h_err, d->err, 2 * nd * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                    d->stream));
-        HIP_TRY(hipStreamSynchronize(d->stream));
+        if (early) {
+            HIP_TRY(hipEventRecord(d->ev_err, d->stream));
+            HIP_TRY(launch_decide(d->stream, nd == 1 ? d->errw : nullptr, (uint32_t)nw, d->err, nd, d->ctl, lambda,
+                                  it->tol, it->reject_nonfinite ? 1 : 0));
+            StageArgs ca = make_args(d, 0, it->adaptive, it->err_all);
+            HIP_TRY(launch_commit(d->stream, ca, 0, d->ctl));
+            HIP_TRY(enqueue_force(d, d->x, d->y, d->z, F_SPEC_K0));
+            HIP_TRY(hipEventSynchronize(d->ev_err));
+        } else {
+            HIP_TRY(hipStreamSynchronize(d->stream));
+        }
         if (it->adaptive && nd == 1) {
             unsigned long long m = 0, bad = 0;
             for (size_t w = 0; w < nw; ++w) {
@@ -1450,13 +1478,20 @@ static int step_disks(rd_integrator* it, const std::vector<uint32_t>& want)
                 d->hd[k].t += it->dt_did[k];
                 pending[k] = 0;
                 remaining[k]--;
-                commit[k] = F_COMMIT;
-                commit_flags_pending = 1;
+                if (early) {
+                    // committed on the device; its next k0 pass ran if it goes on
+                    k0_ahead[k] = remaining[k] > 0;
+                } else {
+                    commit[k] = F_COMMIT;
+                    commit_flags_pending = 1;
+                }
                 acc[k] = 1;
+                any_acc = true;
             }
         }
         (void)n_act;
-        if (commit_flags_pending && it->adaptive) {
+        if (any_acc && it->adaptive) {
+            any_acc = false;
             if (int rc = after_accept(it, acc, err_ratio, iter)) return rc;
             for (uint32_t k = 0; k < nd; ++k) {
                 if (!acc[k] || remaining[k] == 0) continue;
@@ -1498,7 +1533,7 @@ int rd_steps(rd_integrator* it, uint32_t n_steps)
             if (!it->fixed_dt) it->dt_try[k] = it->dt_next[k];
             it->dt_next[k] = it->dt_try[k];
             it->dt_did[k] = it->dt_try[k];
-            d->h_ctl[k] = DiskCtl{d->hd[k].t, it->dt_try[k], F_ACTIVE | F_NEED_K0 | F_COMMIT, 0};
+            d->h_ctl[k] = DiskCtl{d->hd[k].t, it->dt_try[k], F_ACTIVE | F_NEED_K0 | F_K0_FORCE | F_COMMIT, 0};
         }
         HIP_TRY(hipMemcpyAsync(d->ctl, d->h_ctl, nd * sizeof(DiskCtl), hipMemcpyHostToDevice, d->stream));
         StageArgs a = make_args(d, 0, 0, it->err_all);
@@ -1527,7 +1562,7 @@ int rd_steps(rd_integrator* it, uint32_t n_steps)
         return RD_OK;
     }
     std::vector<uint32_t> want(d->n_disk, n_steps);
-    return step_disks(it, want);
+    return step_disks(it, want, true);
 }
 
 int rd_get_dt_next(rd_integrator* it, uint32_t k, double* dt)
@@ -1575,6 +1610,13 @@ int rd_integrator_set_guard(rd_integrator* it, uint32_t max_attempts, double min
     it->max_attempts = max_attempts;
     it->min_dt = min_dt;
     it->reject_nonfinite = nonfinite != 0;
+    return RD_OK;
+}
+
+int rd_integrator_set_overlap(rd_integrator* it, int on)
+{
+    if (!it) return fail(RD_EINVAL, "NULL integrator");
+    it->overlap = on != 0;
     return RD_OK;
 }
 

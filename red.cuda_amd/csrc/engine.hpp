@@ -39,6 +39,9 @@ enum : uint32_t {
     F_NEED_K0 = 2u,  // k0 = f(t, y_n) must be (re)computed
     F_COMMIT = 4u,   // copy y_{n+1} -> y_n
     F_CHECK = 8u,    // event detection runs on this disk
+    F_K0_FORCE = 16u,  // the attempt's own stage-0 pair pass runs (not done ahead of it)
+    F_MORE = 32u,      // the disk goes on with another step if this attempt is accepted
+    F_SPEC_K0 = 64u,   // set on the device: accepted, committed, next k0 pass runs now
 };
 
 // Per-disk static layout.  Indices are local to the disk except off.
@@ -132,6 +135,11 @@ size_t force_sym_lds_bytes(uint32_t R);
 hipError_t launch_stage(hipStream_t s, int method, int r, const StageArgs& a);
 hipError_t launch_comb1(hipStream_t s, int method, const StageArgs& a);
 hipError_t launch_commit(hipStream_t s, const StageArgs& a, int advance_time, DiskCtl* ctl_rw);
+// The host's accept decision made on the device: errw (nw per-wave slots) for
+// one disk, else err[k] / err[n_disk + k]; sets F_COMMIT, and F_SPEC_K0 where
+// F_MORE was set, on accepted disks; clears every other flag.
+hipError_t launch_decide(hipStream_t s, const unsigned long long* errw, uint32_t nw, const unsigned long long* err,
+                         uint32_t n_disk, DiskCtl* ctl, double lambda, double tol, int nonfinite_guard);
 hipError_t launch_operator(hipStream_t s, const StageArgs& a, double t, uint32_t disk, double* ax, double* ay,
                            double* az, const double* vx, const double* vy, const double* vz);
 hipError_t launch_collision_grid(hipStream_t s, uint32_t n_all, const double* x, const double* y, const double* z,

@@ -1317,6 +1317,60 @@ __global__ __launch_bounds__(kBodyBlock) void k_commit(StageArgs a, int advance,
     if (advance && q == 0 && i == a.info[d].off) ctl[d].t = ctl[d].t + ctl[d].h;
 }
 
+// The accept decision of an adaptive attempt, on the device, with the host's
+// arithmetic: max_err = |(lambda h) raw|, rejected when max_err > tol
+// (int_rungekutta8.cu:693-708), refused by the opt-in guard when a term was
+// not finite.  Only comparisons and two products, so the outcome is the
+// host's bit for bit.  Accepted disks get F_COMMIT, and F_SPEC_K0 when they go
+// on (F_MORE): k_commit and the next step's stage-0 pair pass then run while
+// the host is still reading the error.  One block: a single disk reduces its
+// per-wave slots; a batch takes one thread per disk.
+__global__ __launch_bounds__(256) void k_decide(const unsigned long long* __restrict__ errw, uint32_t nw,
+                                                const unsigned long long* __restrict__ err, uint32_t n_disk,
+                                                DiskCtl* __restrict__ ctl, double lambda, double tol,
+                                                int nonfinite_guard)
+{
+    __shared__ unsigned long long smax[256], sbad[256];
+    const uint32_t t = threadIdx.x;
+    if (errw) {
+        unsigned long long m = 0, bad = 0;
+        for (uint32_t w = t; w < nw; w += blockDim.x) {
+            const unsigned long long v = errw[w];
+            m = max(m, v & ~kErrNonFinite);
+            bad |= v & kErrNonFinite;
+        }
+        smax[t] = m;
+        sbad[t] = bad;
+        __syncthreads();
+        for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+            if (t < s) {
+                smax[t] = max(smax[t], smax[t + s]);
+                sbad[t] |= sbad[t + s];
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t k = t; k < n_disk; k += blockDim.x) {
+        const DiskCtl c = ctl[k];
+        uint32_t fl = 0;
+        if (c.flags & F_ACTIVE) {
+            const unsigned long long raw_bits = errw ? smax[0] : err[k];
+            const bool bad = errw ? sbad[0] != 0 : err[n_disk + k] != 0;
+            const double max_err = fabs(lambda * c.h * __longlong_as_double((long long)raw_bits));
+            const bool refused = nonfinite_guard && (bad || !isfinite(max_err));
+            if (!refused && !(max_err > tol)) fl = F_COMMIT | ((c.flags & F_MORE) ? F_SPEC_K0 : 0u);
+        }
+        ctl[k].flags = fl;
+    }
+}
+
+hipError_t launch_decide(hipStream_t s, const unsigned long long* errw, uint32_t nw, const unsigned long long* err,
+                         uint32_t n_disk, DiskCtl* ctl, double lambda, double tol, int nonfinite_guard)
+{
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(256), 0, s, errw, nw, err, n_disk, ctl, lambda, tol, nonfinite_guard);
+    return hipGetLastError();
+}
+
 // pp_disk::calc_dydx(1, ...) for one disk: acceleration at time t from the
 // partials, written to acc[q].
 __global__ __launch_bounds__(kBodyBlock) void k_operator(StageArgs a, double t, uint32_t disk, double* ax,

@@ -219,6 +219,8 @@ class integrator {
     {
         rd_check(rd_integrator_set_guard(h_, max_attempts, min_dt, nonfinite ? 1 : 0), "set_guard");
     }
+    // rd_steps overlaps the host's step-size control with the device (on by default)
+    void set_overlap(bool on) { rd_check(rd_integrator_set_overlap(h_, on ? 1 : 0), "set_overlap"); }
     rd_integrator* handle() { return h_; }
 
     // learned step proposer: first try of each step from m (nullptr: reference controller only)

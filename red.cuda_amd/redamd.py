@@ -74,6 +74,7 @@ _SIGS = {
     "rd_counters": (C.c_int, [_P, _u32, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u64)]),
     "rd_set_error_check_for_tp": (C.c_int, [_P, C.c_int]),
     "rd_integrator_set_guard": (C.c_int, [_P, _u32, _f64, C.c_int]),
+    "rd_integrator_set_overlap": (C.c_int, [_P, C.c_int]),
     "rd_bench_stages": (C.c_int, [_P, C.c_int, _u32, C.POINTER(_f64)]),
     "rd_calc_integral_noalloc": (C.c_int, [_P, _u32, _P]),
     "rd_work_counters": (C.c_int, [_P, C.POINTER(_u64), C.POINTER(_f64)]),
@@ -407,6 +408,11 @@ class Integrator:
     def set_guard(self, max_attempts=0, min_dt=0.0, nonfinite=False):
         """Opt-in bound on the retry loop (rd_integrator_set_guard)."""
         check(lib.rd_integrator_set_guard(self.h, max_attempts, min_dt, 1 if nonfinite else 0))
+
+    def set_overlap(self, on=True):
+        """rd_steps: accept, commit and the next k0 pass on the device while the
+        host reads the error (rd_integrator_set_overlap; on by default)."""
+        check(lib.rd_integrator_set_overlap(self.h, 1 if on else 0))
 
     def step_rc(self):
         """(status, dt_did) of rd_step without raising: the guard's RD_ESTATE."""

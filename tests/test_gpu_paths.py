@@ -236,3 +236,84 @@ def test_event_buffer_overflow_is_reported():
     r2["x"][1:] = np.arange(1, n) * 0.1
     d.set_state(0.0, r2, v)
     assert d.check_collision() == 0
+
+
+def _steps_with_overlap(make, method, h0, tol, n, overlap, precision, guard=None):
+    """rd_steps(n) with the device-side decision, commit and k0 pass on or off:
+    (status, per-disk states, counters, dt_next, work counters)."""
+    d = make()
+    d.set_precision(precision)
+    it = R.Integrator(d, method, h0, adaptive=True, tol=tol)
+    it.set_overlap(overlap)
+    if guard:
+        it.set_guard(**guard)
+    rc = 0
+    try:
+        it.steps(n)
+    except R.RedError as e:
+        rc = e.code
+    nd = d.n_disk()
+    return (rc, [d.state(k) for k in range(nd)], [it.counters(k) for k in range(nd)],
+            [it.dt_next(k) for k in range(nd)], it.work())
+
+
+def _batch(parts):
+    cat = [np.concatenate([q[i] for q in parts]) for i in range(4)]
+    return R.Disk(*cat, np.stack([q[4] for q in parts]), batch=True)
+
+
+OVERLAP_CASES = {
+    # name: (disk factory, method, h0, tol, steps, precision, guard)
+    "one_disk_mixed_rkf78": (lambda: R.Disk(*R.make_disk(2000, 71)), R.RD_RKF78, 0.1 * KDAY, 1e-10, 40,
+                             R.RD_PREC_MIXED, None),
+    "one_disk_fp64_rkf78": (lambda: R.Disk(*R.make_disk(300, 72)), R.RD_RKF78, 0.1 * KDAY, 1e-10, 40,
+                            R.RD_PREC_FP64, None),
+    "one_disk_mixed_rk4": (lambda: R.Disk(*R.make_disk(1000, 73)), R.RD_RK4, 0.1 * KDAY, 1e-9, 40,
+                           R.RD_PREC_MIXED, None),
+    "batch_mixed_rkf78": (lambda: _batch([R.make_disk(n, 80 + n) for n in (64, 300, 999, 257, 1500)]), R.RD_RKF78,
+                          0.1 * KDAY, 1e-10, 25, R.RD_PREC_MIXED, None),
+    "batch_fp64_nan_guard": (lambda: _batch([R.make_disk(99, 61), coincident(), R.make_disk(299, 62)]), R.RD_RKF78,
+                             0.01, 1e-11, 3, R.RD_PREC_FP64, dict(nonfinite=True)),
+    "one_disk_nan_accepted": (lambda: R.Disk(*coincident()), R.RD_RKF78, 0.01, 1e-10, 3, R.RD_PREC_FP64, None),
+}
+
+
+@pytest.mark.parametrize("case", sorted(OVERLAP_CASES))
+def test_overlap_changes_no_result(case):
+    """rd_steps with the accept decision, commit and next k0 pass made on the
+    device while the host computes the step size (rd_integrator_set_overlap)
+    gives the results of the host-only sequence bit for bit: states,
+    counters, next step sizes, force passes and pair counts, also where a
+    disk of a batch fails the non-finite guard or a NaN state is accepted."""
+    print(f"# {case}", flush=True)
+    make, method, h0, tol, n, prec, guard = OVERLAP_CASES[case]
+    on = _steps_with_overlap(make, method, h0, tol, n, True, prec, guard)
+    off = _steps_with_overlap(make, method, h0, tol, n, False, prec, guard)
+    assert on[0] == off[0]
+    if case == "batch_fp64_nan_guard":
+        assert on[0] == R.RD_ESTATE
+    for (ta, ra, va, pa, ba), (tb, rb, vb, pb, bb) in zip(on[1], off[1]):
+        assert ta == tb and bits_equal(pa, pb) and bits_equal(ba, bb)
+        for c in "xyz":
+            assert np.array_equal(ra[c], rb[c], equal_nan=True) and bits_equal(ra[c][~np.isnan(ra[c])],
+                                                                                 rb[c][~np.isnan(rb[c])])
+            assert np.array_equal(va[c], vb[c], equal_nan=True)
+    assert on[2] == off[2]
+    assert np.array_equal(np.array(on[3]), np.array(off[3]), equal_nan=True)
+    assert on[4] == off[4]
+
+
+def test_overlapped_steps_match_the_oracle():
+    """FP64 rd_steps with the overlap on against the oracle's step loop, bit for
+    bit, over a run with rejections (the device's decision must be the host's)."""
+    r, v, p, b, nbt = R.make_disk(200, 74)
+    d = R.Disk(r, v, p, b, nbt)
+    o = O.ODisk(r, v, p, b, nbt)
+    it = R.Integrator(d, R.RD_RKF78, 5.0 * KDAY, adaptive=True, tol=1e-12)
+    oi = O.OInteg(o, R.RD_RKF78, 5.0 * KDAY, adaptive=True, tol=1e-12)
+    it.steps(30)
+    for _ in range(30):
+        oi.step()
+    assert it.counters() == oi.counters() and it.counters()[2] > 0
+    assert it.dt_next() == oi.dt_next()
+    assert same_state(d, o)
