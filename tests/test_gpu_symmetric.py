@@ -37,8 +37,10 @@ def check_bounds(got, ref, mut):
 
 
 # bodies -> 256-body blocks -> chunks of R blocks: 301 -> 2 blocks (partial),
-# 5000 -> 20 (R = 1), 70000 -> 274 (R = 2, 137 chunks), 262200 -> 1025 (R = 8)
-@pytest.mark.parametrize("n,seed", [(300, 31), (4999, 32), (69999, 33), (262199, 34)])
+# 5000 -> 20 (R = 1), 70000 -> 274 (R = 2, 137 chunks), 262200 -> 1025 (R = 8),
+# 1048576 -> 4096 (R at its cap of 8: 512 chunks, past the 255 the cap-free
+# rule allows, so 512 partial-sum slots per body)
+@pytest.mark.parametrize("n,seed", [(300, 31), (4999, 32), (69999, 33), (262199, 34), (1048575, 36)])
 def test_symmetric_force_bounds(n, seed):
     r, v, p, b, nbt = R.make_disk(n, seed)
     ref, mut = fp64_and_mutual(r, v, p, b, nbt)
