@@ -1830,6 +1830,10 @@ __global__ __launch_bounds__(kBlock) void k_feat_pairs(const WorkItem* __restric
         if (!(ctl[w.disk].flags & F_CHECK)) continue;
         const uint32_t s = info[w.disk].off;
         const double sx = x[s], sy = y[s], sz = z[s], svx = vx[s], svy = vy[s], svz = vz[s];
+        // Inactive or padding bodies sit at far sentinels, sinks at -3e19 and
+        // sources at +3e19 AU: the squares overflow, r^2 is infinite and never
+        // the minimum, so the inner loop needs no activity test.  Only tiles
+        // that reach back to the sinks' own indices test j > i.
         uint32_t si[kFeatS];
         float xi[kFeatS], yi[kFeatS], zi[kFeatS], ui[kFeatS], vi[kFeatS], wi[kFeatS];
 #pragma unroll
@@ -1837,39 +1841,51 @@ __global__ __launch_bounds__(kBlock) void k_feat_pairs(const WorkItem* __restric
             const uint32_t i = w.sink_lo + lane + 64u * u;
             const bool valid = i < w.sink_hi && id[i] > 0;
             si[u] = valid ? i : 0xffffffffu;     // an invalid sink matches no j > i
-            xi[u] = valid ? (float)(x[i] - sx) : 0.0f;
-            yi[u] = valid ? (float)(y[i] - sy) : 0.0f;
-            zi[u] = valid ? (float)(z[i] - sz) : 0.0f;
+            xi[u] = valid ? (float)(x[i] - sx) : -3.0e19f;
+            yi[u] = valid ? (float)(y[i] - sy) : -3.0e19f;
+            zi[u] = valid ? (float)(z[i] - sz) : -3.0e19f;
             ui[u] = valid ? (float)(vx[i] - svx) : 0.0f;
             vi[u] = valid ? (float)(vy[i] - svy) : 0.0f;
             wi[u] = valid ? (float)(vz[i] - svz) : 0.0f;
         }
         float best = 3.0e38f;
+        auto pair = [&](const float4& pj, const float4& qj, uint32_t u) {
+            const float dx = pj.x - xi[u], dy = pj.y - yi[u], dz = pj.z - zi[u];
+            const float du = qj.x - ui[u], dv = qj.y - vi[u], dw = qj.z - wi[u];
+            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            const float u2 = __builtin_fmaf(dw, dw, __builtin_fmaf(dv, dv, __builtin_fmaf(du, du, 1.0e-30f)));
+            return r2 * __builtin_amdgcn_rcpf(u2);
+        };
         for (uint32_t t0 = w.src_lo; t0 < w.src_hi; t0 += kTile) {
             __syncthreads();
             const uint32_t j = t0 + tid;
             if (j < w.src_hi) {
                 const bool on = id[j] > 0;
-                tp[tid] = make_float4((float)(x[j] - sx), (float)(y[j] - sy), (float)(z[j] - sz), on ? 0.0f : 1.0f);
+                tp[tid] = on ? make_float4((float)(x[j] - sx), (float)(y[j] - sy), (float)(z[j] - sz), 0.0f)
+                             : make_float4(3.0e19f, 3.0e19f, 3.0e19f, 0.0f);
                 tv[tid] = make_float4((float)(vx[j] - svx), (float)(vy[j] - svy), (float)(vz[j] - svz), 0.0f);
             }
             __syncthreads();
             const uint32_t nt = min((uint32_t)kTile, w.src_hi - t0);
             const uint32_t k0 = wave * 64u, k1 = min(k0 + 64u, nt);
+            if (t0 < w.sink_hi) {
+                // the tile may hold j <= i: keep i < j only
 #pragma unroll 4
-            for (uint32_t k = k0; k < k1; ++k) {
-                const float4 pj = tp[k];
-                const float4 qj = tv[k];
-                const uint32_t jj = t0 + k;
+                for (uint32_t k = k0; k < k1; ++k) {
+                    const float4 pj = tp[k], qj = tv[k];
+                    const uint32_t jj = t0 + k;
 #pragma unroll
-                for (uint32_t u = 0; u < kFeatS; ++u) {
-                    const float dx = pj.x - xi[u], dy = pj.y - yi[u], dz = pj.z - zi[u];
-                    const float du = qj.x - ui[u], dv = qj.y - vi[u], dw = qj.z - wi[u];
-                    const float r2 = dx * dx + dy * dy + dz * dz;
-                    const float u2 = du * du + dv * dv + dw * dw + 1.0e-30f;
-                    const float qq = r2 * __builtin_amdgcn_rcpf(u2);
-                    // si = 0xffffffff (invalid sink) never satisfies jj > si
-                    best = (jj > si[u] && pj.w == 0.0f) ? fminf(best, qq) : best;
+                    for (uint32_t u = 0; u < kFeatS; ++u) {
+                        const float qq = pair(pj, qj, u);
+                        best = jj > si[u] ? fminf(best, qq) : best;
+                    }
+                }
+            } else {
+#pragma unroll 4
+                for (uint32_t k = k0; k < k1; ++k) {
+                    const float4 pj = tp[k], qj = tv[k];
+#pragma unroll
+                    for (uint32_t u = 0; u < kFeatS; ++u) best = fminf(best, pair(pj, qj, u));
                 }
             }
         }
