@@ -1490,8 +1490,9 @@ static int step_disks(rd_integrator* it, const std::vector<uint32_t>& want, bool
             }
         }
         (void)n_act;
-        if (any_acc && it->adaptive) {
-            any_acc = false;
+        const bool accepted_some = any_acc;
+        any_acc = false;
+        if (accepted_some && it->adaptive) {
             if (int rc = after_accept(it, acc, err_ratio, iter)) return rc;
             for (uint32_t k = 0; k < nd; ++k) {
                 if (!acc[k] || remaining[k] == 0) continue;
