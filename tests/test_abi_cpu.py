@@ -115,3 +115,17 @@ def test_golden_config1_reproduced_by_oracle_prefix():
     assert np.array_equal(np.stack([r["x"], r["y"], r["z"]], 1), g["r0"])
     assert np.array_equal(np.stack([v["x"], v["y"], v["z"]], 1), g["v0"])
     assert g["t"] == pytest.approx(10000 * 0.1 * 0.01720209895, rel=1e-12)
+
+
+def test_binding_refuses_to_load_without_the_native_library(tmp_path):
+    """No CPU fallback: the binding, copied next to no librdamd.so, raises at
+    import instead of running anything."""
+    import shutil
+    import sys
+    src = os.path.join(ROOT, "red.cuda_amd")
+    for f in ("redamd.py", "redamd_types.py"):
+        shutil.copy(os.path.join(src, f), tmp_path / f)
+    res = subprocess.run([sys.executable, "-c", "import redamd"], cwd=str(tmp_path), capture_output=True, text=True,
+                         timeout=120, env={**os.environ, "PYTHONPATH": str(tmp_path)})
+    assert res.returncode != 0
+    assert "ImportError" in res.stderr and "is not built" in res.stderr
