@@ -167,6 +167,7 @@ struct rd_disk {
     WorkItem* items_feat = nullptr;
     uint32_t n_items_feat = 0;
     bool exact_lane = false;       // FP64 pair kernel: one sink per lane (see build_lists)
+    bool exact_small = false;      // FP64 pair kernel: 4-sink items (small disks, see build_lists)
     bool lists_valid = false;
     double pairs_per_eval = 0.0;
 
@@ -485,18 +486,23 @@ int build_lists(rd_disk* d)
     const uint32_t S = d->tuning.sinks_per_lane;
     const uint32_t C = std::max<uint32_t>(kTile, d->tuning.chunk / kTile * kTile);
     // FP64: one sink per lane when 256-sink items give >= 4 blocks per CU,
-    // else the term-parallel kernel on 16-sink items (same bits either way)
+    // else the term-parallel kernel on 16-sink items, or on 4-sink items when
+    // 16-sink items would be fewer than the CUs (same bits in every case)
     if (d->precision == RD_PREC_FP64) {
-        size_t groups = 0;
+        size_t groups = 0, groups16 = 0;
         for (auto& o : order)
             for (int cls = 0; cls < 3; ++cls) {
                 uint32_t b[4];
                 class_bounds(d->hd[o.second], cls, b);
-                if (b[1] > b[0]) groups += (b[1] - b[0] + kBlock - 1) / kBlock;
+                if (b[1] > b[0]) {
+                    groups += (b[1] - b[0] + kBlock - 1) / kBlock;
+                    groups16 += (b[1] - b[0] + kExactSinks - 1) / kExactSinks;
+                }
             }
         d->exact_lane = groups >= 4 * (size_t)d->n_cu;
+        d->exact_small = !d->exact_lane && groups16 < (size_t)d->n_cu;
     }
-    const uint32_t exact_group = d->exact_lane ? (uint32_t)kBlock : kExactSinks;
+    const uint32_t exact_group = d->exact_lane ? (uint32_t)kBlock : (d->exact_small ? kExactSinksSmall : kExactSinks);
     const bool sym = d->precision == RD_PREC_MIXED && d->tuning.pair_mode == RD_PAIRS_SYMMETRIC;
     std::vector<std::pair<uint32_t, WorkItem>> sym_items;   // (tiles, item)
     d->sym_R = 1;
@@ -668,7 +674,8 @@ StageArgs make_args(rd_disk* d, int r, int adaptive, int err_all)
 hipError_t enqueue_force(rd_disk* d, const double* fx, const double* fy, const double* fz, uint32_t need)
 {
     if (d->precision == RD_PREC_FP64)
-        return launch_force_exact(d->stream, d->exact_lane ? 1 : 0, d->items_force, d->n_items_force, d->grid_force,
+        return launch_force_exact(d->stream, d->exact_lane ? 1 : (d->exact_small ? 2 : 0), d->items_force,
+                                  d->n_items_force, d->grid_force,
                                   fx, fy, fz, d->m_ex, d->ctl, need, d->part, d->n_all);
     // mixed precision: FP32 slot totals, in float units of the same buffer
     float* part32 = reinterpret_cast<float*>(d->part);

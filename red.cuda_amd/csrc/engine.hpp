@@ -23,6 +23,8 @@ constexpr int kBodyBlock = 64;       // per-body kernels: one wave per block, so
 constexpr int kTile = 256;           // sources staged in LDS per tile
 // sinks per work item of the exact (FP64, reference-order) pair kernel
 constexpr uint32_t kExactSinks = 16;
+// ... and 4 for disks whose 16-sink items would leave CUs idle
+constexpr uint32_t kExactSinksSmall = 4;
 // bodies per block of the symmetric pair kernel (64 lanes x 4 sinks)
 constexpr uint32_t kSymBlockBodies = 256;
 // largest chunk (in blocks) of a symmetric-kernel work item; its reaction
@@ -118,6 +120,7 @@ struct BodyArrays {
 };
 
 // Launchers (kernels.hip).
+// lane: 0 term-parallel 16-sink items, 1 one sink per lane, 2 term-parallel 4-sink items
 hipError_t launch_force_exact(hipStream_t s, int lane, const WorkItem* items, uint32_t n_items, uint32_t grid,
                               const double* x, const double* y, const double* z, const double* m_ex,
                               const DiskCtl* ctl, uint32_t need, double* part, uint32_t n_all);

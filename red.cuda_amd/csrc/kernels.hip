@@ -75,14 +75,16 @@ __host__ __device__ constexpr double rk4_c(int r) { return (r == 1 || r == 2) ? 
 // rounded separately (the reference builds -fmad=false).
 //
 // The sum per (sink, component) is a chain of FP64 adds in j order, so it
-// cannot be split; the pair terms feeding it can.  A block takes kExactSinks
-// sinks: for each 64-source LDS tile, all 256 threads compute the 16 x 64
-// terms (m_j / (d * d^2)) * dx (sqrt and divide are the costly part) into LDS,
-// then 48 lanes of wave 0 -- one per (sink, component) -- add the tile's
-// terms in j order to their running sums.  Same terms, same order: the result
-// is bit-identical to one lane summing alone, while the term work spreads
-// over N / 16 blocks of 4 waves instead of N lanes.
+// cannot be split; the pair terms feeding it can.  A block takes kS sinks
+// (16, or 4 for disks whose 16-sink items would not reach every CU): for each
+// 64-source LDS tile, all 256 threads compute the kS x 64 terms
+// (m_j / (d * d^2)) * dx (sqrt and divide are the costly part) into LDS, then
+// 3 kS lanes of wave 0 -- one per (sink, component) -- add the tile's terms in
+// j order to their running sums.  Same terms, same order: the result is
+// bit-identical to one lane summing alone, for either kS, while the term work
+// spreads over N / kS blocks of 4 waves instead of N lanes.
 constexpr uint32_t kExactTile = 64;
+template <uint32_t kS>
 __global__ __launch_bounds__(kBlock) void k_force_exact(const WorkItem* __restrict__ items, uint32_t n_items,
                                                         const double* __restrict__ x, const double* __restrict__ y,
                                                         const double* __restrict__ z,
@@ -90,14 +92,16 @@ __global__ __launch_bounds__(kBlock) void k_force_exact(const WorkItem* __restri
                                                         const DiskCtl* __restrict__ ctl, uint32_t need,
                                                         double* __restrict__ part, uint32_t n_all)
 {
-    static_assert(kExactSinks * kExactTile == 4 * kBlock, "4 terms per thread");
+    constexpr uint32_t kLanesPerSink = kBlock / kS;              // threads sharing one sink's terms
+    constexpr uint32_t kT = kExactTile / kLanesPerSink;          // terms per thread per tile
+    static_assert(kS * kExactTile == kT * kBlock && kT >= 1 && 3 * kS <= 64, "tile shape");
     __shared__ double4 src[kExactTile];
-    __shared__ double term[3][kExactSinks][kExactTile + 1];   // +1: sums read with a 65-double stride
+    __shared__ double term[3][kS][kExactTile + 1];   // +1: sums read with a 65-double stride
     const uint32_t tid = threadIdx.x;
-    const uint32_t g = tid >> 4;              // sink of this thread's terms
-    const uint32_t k_lo = tid & 15u;          // its sources: k_lo + 16 q, q = 0..3
-    const uint32_t sc_c = tid >> 4 & 3u;      // summing lanes (tid < 48): component
-    const uint32_t sc_g = tid & 15u;          //                             sink
+    const uint32_t g = tid / kLanesPerSink;   // sink of this thread's terms
+    const uint32_t k_lo = tid % kLanesPerSink;    // its sources: k_lo + kLanesPerSink q, q < kT
+    const uint32_t sc_c = tid / kS % 3u;      // summing lanes (tid < 3 kS): component
+    const uint32_t sc_g = tid % kS;           //                               sink
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         const WorkItem w = items[it];
         if (!(ctl[w.disk].flags & need)) continue;
@@ -122,8 +126,8 @@ __global__ __launch_bounds__(kBlock) void k_force_exact(const WorkItem* __restri
             }
             __syncthreads();
 #pragma unroll
-            for (uint32_t q = 0; q < 4; ++q) {
-                const uint32_t k = k_lo + 16u * q;
+            for (uint32_t q = 0; q < kT; ++q) {
+                const uint32_t k = k_lo + kLanesPerSink * q;
                 const double4 sj = src[k];
                 const double dx = sj.x - xi;
                 const double dy = sj.y - yi;
@@ -140,13 +144,13 @@ __global__ __launch_bounds__(kBlock) void k_force_exact(const WorkItem* __restri
                 term[2][g][k] = skip ? 0.0 : sc * dz;
             }
             __syncthreads();
-            if (tid < 3 * kExactSinks) {
+            if (tid < 3 * kS) {
                 const double* tr = &term[sc_c][sc_g][0];
 #pragma unroll
                 for (uint32_t k = 0; k < kExactTile; ++k) acc += tr[k];
             }
         }
-        if (tid < 3 * kExactSinks && si < w.sink_hi) part[(size_t)(w.slot * 3 + sc_c) * n_all + si] = acc;
+        if (tid < 3 * kS && si < w.sink_hi) part[(size_t)(w.slot * 3 + sc_c) * n_all + si] = acc;
     }
 }
 
@@ -605,12 +609,15 @@ hipError_t launch_force_exact(hipStream_t st, int lane, const WorkItem* items, u
                               const DiskCtl* ctl, uint32_t need, double* part, uint32_t n_all)
 {
     if (n_items == 0) return hipSuccess;
-    if (lane)
+    if (lane == 1)
         hipLaunchKernelGGL(k_force_exact_lane, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, m_ex, ctl,
                            need, part, n_all);
+    else if (lane == 2)
+        hipLaunchKernelGGL(k_force_exact<kExactSinksSmall>, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z,
+                           m_ex, ctl, need, part, n_all);
     else
-        hipLaunchKernelGGL(k_force_exact, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, m_ex, ctl, need,
-                           part, n_all);
+        hipLaunchKernelGGL(k_force_exact<kExactSinks>, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z,
+                           m_ex, ctl, need, part, n_all);
     return hipGetLastError();
 }
 

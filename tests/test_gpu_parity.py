@@ -25,10 +25,13 @@ def pair(r, v, p, b, nbt, thr=(0.0, 0.0, 0.0), gas=None, t=0.0):
     return d, o
 
 
-@pytest.mark.parametrize("case", ["disk256", "inactive1000", "classes700", "ragged1"])
+# below ~4096 bodies the FP64 kernel takes 4-sink items, above 16-sink ones
+@pytest.mark.parametrize("case", ["disk256", "inactive1000", "classes700", "ragged1", "disk5000"])
 def test_force_fp64_bitexact(case):
     if case == "disk256":
         r, v, p, b, nbt = R.make_disk(255, 11)
+    elif case == "disk5000":
+        r, v, p, b, nbt = R.make_disk(4999, 15)
     elif case == "inactive1000":
         r, v, p, b, nbt = small_system(1000, 12, inactive=(1, 2, 17, 256, 511, 999))
     elif case == "classes700":
