@@ -699,16 +699,19 @@ int flag_check_only(rd_disk* d, const std::vector<char>& on)
     return RD_OK;
 }
 
-// One attempt for every disk whose ctl flags say so.
-hipError_t enqueue_attempt(rd_integrator* it)
+// One attempt for every disk whose ctl flags say so; with commit_advance >= 0
+// (fixed-step chains) the commit follows in the same sequence.
+hipError_t enqueue_attempt(rd_integrator* it, int commit_advance)
 {
     rd_disk* d = it->d;
     hipError_t e;
     {
         StageArgs a = make_args(d, 0, it->adaptive, it->err_all);
+        // fixed-step RK4 never retries with k0 kept, so stage 0 forms stage 1 itself
+        a.fuse_comb1 = it->method == RD_RK4 && !it->adaptive;
         if ((e = enqueue_force(d, d->x, d->y, d->z, F_K0_FORCE)) != hipSuccess) return e;
         if ((e = launch_stage(d->stream, it->method, 0, a)) != hipSuccess) return e;
-        if (it->n_stage > 1) {
+        if (it->n_stage > 1 && !a.fuse_comb1) {
             StageArgs a1 = make_args(d, 0, it->adaptive, it->err_all);   // comb1 writes P[1]
             if ((e = launch_comb1(d->stream, it->method, a1)) != hipSuccess) return e;
         }
@@ -718,18 +721,22 @@ hipError_t enqueue_attempt(rd_integrator* it)
         if ((e = enqueue_force(d, a.fpos[0], a.fpos[1], a.fpos[2], F_ACTIVE)) != hipSuccess) return e;
         if ((e = launch_stage(d->stream, it->method, r, a)) != hipSuccess) return e;
     }
+    if (commit_advance >= 0) {
+        StageArgs a = make_args(d, 0, it->adaptive, it->err_all);
+        if ((e = launch_commit(d->stream, a, commit_advance, d->ctl)) != hipSuccess) return e;
+    }
     return hipSuccess;
 }
 
-int run_attempt(rd_integrator* it, bool use_graph)
+int run_attempt(rd_integrator* it, bool use_graph, int commit_advance = -1)
 {
     rd_disk* d = it->d;
     if (use_graph) {
-        const int kind = it->method * 16 + it->adaptive * 2 + it->err_all;
+        const int kind = ((it->method * 16 + it->adaptive * 2 + it->err_all) * 4) + (commit_advance + 1);
         if (!it->gexec || it->graph_gen != d->generation || it->graph_kind != kind) {
             release_graph(it);
             HIP_TRY(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
-            hipError_t e = enqueue_attempt(it);
+            hipError_t e = enqueue_attempt(it, commit_advance);
             hipGraph_t g = nullptr;
             hipError_t e2 = hipStreamEndCapture(d->stream, &g);
             if (e != hipSuccess || e2 != hipSuccess) {
@@ -744,7 +751,7 @@ int run_attempt(rd_integrator* it, bool use_graph)
         }
         HIP_TRY(hipGraphLaunch(it->gexec, d->stream));
     } else {
-        HIP_TRY(enqueue_attempt(it));
+        HIP_TRY(enqueue_attempt(it, commit_advance));
     }
     return RD_OK;
 }
@@ -1544,10 +1551,9 @@ int rd_steps(rd_integrator* it, uint32_t n_steps)
             d->h_ctl[k] = DiskCtl{d->hd[k].t, it->dt_try[k], F_ACTIVE | F_NEED_K0 | F_K0_FORCE | F_COMMIT, 0};
         }
         HIP_TRY(hipMemcpyAsync(d->ctl, d->h_ctl, nd * sizeof(DiskCtl), hipMemcpyHostToDevice, d->stream));
-        StageArgs a = make_args(d, 0, 0, it->err_all);
         for (uint32_t s = 0; s < n_steps; ++s) {
-            if (int rc = run_attempt(it, d->tuning.use_graph != 0)) return rc;
-            HIP_TRY(launch_commit(d->stream, a, 1, d->ctl));
+            // the attempt and its commit (t += h on the device) as one sequence
+            if (int rc = run_attempt(it, d->tuning.use_graph != 0, 1)) return rc;
             it->force_evals += (uint64_t)it->n_stage;
             // at most 64 attempts (each a graph launch and a commit) queued
             // between host waits

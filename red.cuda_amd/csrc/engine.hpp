@@ -103,6 +103,7 @@ struct StageArgs {
     GasDev gas;
     int mixed;
     int adaptive;
+    int fuse_comb1;              // RK4 stage 0 also forms stage 1 (fixed step: k0 is never reused)
     int one_disk;                // every body belongs to disk 0
     DiskInfo info0;              // info[0] when one_disk (kernel arguments are scalar loads)
     double* trash;               // [3][kBodyBlock]: stores of a single-disk stage that does not apply

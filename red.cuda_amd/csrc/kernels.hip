@@ -1149,7 +1149,13 @@ __device__ __forceinline__ void stage_rk4_one(const StageArgs& a)
     double* const trash = a.trash + (size_t)q * kBodyBlock + threadIdx.x;
     auto at = [&](double* p) { return valid ? p + i : trash; };
     *at(ka_w(a, R, q)) = kRa;
-    if constexpr (R == 1 || R == 2) {
+    if constexpr (R == 0) {
+        if (a.fuse_comb1) {   // k_comb1's stage-1 combination, same operands
+            const double fac = rk4_a(1) * h;
+            *at(a.npos[q]) = yy + fac * vv;
+            *at(kv_w(a, 1, q)) = vv + fac * kRa;
+        }
+    } else if constexpr (R == 1 || R == 2) {
         const double fac = rk4_a(R + 1) * h;
         *at(a.npos[q]) = yy + fac * kk;
         *at(kv_w(a, R + 1, q)) = vv + fac * kRa;
@@ -1210,7 +1216,13 @@ __global__ __launch_bounds__(kBodyBlock) void k_stage_rk4(StageArgs a)
         const double b = prefix_accel(a, i, q, di, a.fpos);
         const double kRa = stage_accel(a, i, q, di, t, b, partial_sum(a, i, q, nchunk_of(di, i)), a.fpos, vR);
         ka_w(a, R, q)[i] = kRa;
-        if constexpr (R == 1 || R == 2) {
+        if constexpr (R == 0) {
+            if (a.fuse_comb1) {   // k_comb1's stage-1 combination, same operands
+                const double fac = rk4_a(1) * h;
+                a.npos[q][i] = yy + fac * vv;
+                kv_w(a, 1, q)[i] = vv + fac * kRa;
+            }
+        } else if constexpr (R == 1 || R == 2) {
             const double fac = rk4_a(R + 1) * h;
             a.npos[q][i] = yy + fac * kk;
             kv_w(a, R + 1, q)[i] = vv + fac * kRa;
