@@ -224,7 +224,7 @@ class integrator {
     rd_integrator* handle() { return h_; }
 
     // learned step proposer: first try of each step from m (nullptr: reference controller only)
-    void set_proposer(mlp* m, float safety = 0.85f)
+    void set_proposer(mlp* m, float safety = 1.0f)
     {
         rd_check(rd_integrator_set_proposer(h_, m ? m->handle() : nullptr, safety), "set_proposer");
     }
