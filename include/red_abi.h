@@ -93,6 +93,20 @@ typedef struct rd_gas_params {                                          /* analy
     double t0, t1, e_folding_time;
     double alpha, mean_molecular_weight, particle_diameter;
 } rd_gas_params;
+/* Frame 0 of a FARGO gas run (fargo_gas_disk.h:8-60; the frame the reference
+ * loads, fargo_gas_disk.cu:190-206): n_rad rings x n_sec sectors, the cell of
+ * ring k and sector s at k * n_sec + s.  used_rad holds the n_rad + 1 ring
+ * interface radii [AU], increasing (FARGO's used_rad.dat).  Values are taken in
+ * the solver's units -- density [Msun/AU^3], velocities [AU/tau] -- since the
+ * reference's unit transforms are not implemented (fargo_gas_disk.cu:256-272);
+ * vtheta is the gas's azimuthal velocity on top of the circular one. */
+typedef struct rd_gas_grid {
+    uint32_t n_rad, n_sec;
+    const double* used_rad;   /* n_rad + 1 */
+    const double* density;    /* n_rad * n_sec */
+    const double* vrad;       /* n_rad * n_sec */
+    const double* vtheta;     /* n_rad * n_sec */
+} rd_gas_grid;
 
 /* Launch shapes of the pair kernels (chosen by rd_autotune or by hand). */
 typedef struct rd_tuning {
@@ -141,6 +155,12 @@ int rd_batch_create(uint32_t n_disk, const uint32_t* n_by_type,
 int rd_disk_destroy(rd_disk* d);
 int rd_disk_set_precision(rd_disk* d, int precision);
 int rd_disk_set_drag_mode(rd_disk* d, int mode);
+/* GAS_DISK_MODEL_FARGO (options.cpp:149-153, 232-234; drag pp_disk.cu:778-814):
+ * copy the grid to the device and drag the gas-feeling bodies through it
+ * (every disk of a batch, the same grid), in place of the analytic disk.
+ * NULL removes the grid; the disk returns to the gas model it was created
+ * with.  The arrays are copied, so the caller may free them on return. */
+int rd_disk_set_gas_grid(rd_disk* d, const rd_gas_grid* grid);
 int rd_disk_set_tuning(rd_disk* d, const rd_tuning* tuning);
 int rd_disk_get_tuning(rd_disk* d, rd_tuning* tuning);
 /* pp_disk::benchmark picks n_tpb by timing (pp_disk.cu:553-632); this times

@@ -21,6 +21,13 @@
  *                    long double (inside / outside the taper, off the
  *                    midplane); decrease laws; decr<1e-6 cut; the gas
  *                    parameters integrate over z to the stated Sigma
+ *   grid drag        pp_disk.cu:158-193, 778-814: rotate_2D_vector KATs
+ *                    (red_test.cpp:170-250); the cell against numpy's
+ *                    searchsorted and floor(angle / dalpha) at ring and
+ *                    sector centres and on the edges; a uniform grid at rest
+ *                    relative to Keplerian rotation equals the analytic
+ *                    disk's drag with eta = 0 and a flat midplane density;
+ *                    gas velocity from unit vectors in long double
  *   RK4 / RKF78      int_rungekutta4.cu, int_rungekutta8.cu: order of
  *                    convergence on Kepler; the oracle's tableau against the
  *                    published fractions, row sums == c, quadrature order 7
@@ -516,6 +523,62 @@ void or_drag(double t, const or_gas* g, int drag_mode, uint32_t n, const uint32_
 }
 
 /* ------------------------------------------------------------------------ */
+/* FARGO grid drag (pp_disk.cu:158-193, 778-814; util.cu:76-83)              */
+/* ------------------------------------------------------------------------ */
+or_var4 or_rotate_2d(double theta, or_var4 rv)   /* rotate_2D_vector, util.cu:76-83 */
+{
+    double ct = cos(theta);
+    double st = sin(theta);
+    or_var4 res = {ct * rv.x - st * rv.y, st * rv.x + ct * rv.y, 0.0, 0.0};
+    return res;
+}
+/* calc_linear_index (pp_disk.cu:158-193), with the ring found by the
+ * reference's linear scan over used_rad[0..n_rad].  Where the reference's
+ * index is undefined or past the grid (a negative alpha from atan2(-0, x < 0)
+ * converted to unsigned; sector n_sec in the last ring) this returns sector 0
+ * and the last cell. */
+uint32_t or_grid_cell(const or_grid* g, double x, double y)
+{
+    const double dalpha = OR_TWOPI / g->n_sec;
+    double r = sqrt(SQR(x) + SQR(y));
+    if (g->used_rad[0] > r) return 0;
+    else if (g->used_rad[g->n_rad] < r) return g->n_rad * g->n_sec - 1;
+    uint32_t i_rad = 0, i_sec = 0;
+    for (uint32_t k = 0; k < g->n_rad; k++) {
+        if (g->used_rad[k] <= r && r < g->used_rad[k + 1]) { i_rad = k; break; }
+    }
+    double alpha = y >= 0.0 ? atan2(y, x) : OR_TWOPI + atan2(y, x);
+    double q = alpha / dalpha;
+    i_sec = q >= 0.0 ? (uint32_t)q : 0u;
+    uint32_t i_linear = i_rad * g->n_sec + i_sec;
+    return i_linear < g->n_rad * g->n_sec ? i_linear : g->n_rad * g->n_sec - 1;
+}
+/* The reference's CPU FARGO loop (pp_disk.cu:778-814) over the drag range. */
+void or_grid_drag(const or_grid* g, int drag_mode, const uint32_t pl[OR_NTYPE], const or_param* p,
+                  const or_var4* r, const or_var4* v, or_var4* a)
+{
+    uint32_t lo, hi;
+    if (drag_mode == OR_DRAG_NSI) { lo = n_si_of(pl); hi = lo + n_nsi_of(pl); }
+    else { lo = pl[OR_STAR]; hi = n_si_of(pl) + n_nsi_of(pl); }
+    double m_star = p[0].mass;
+    for (uint32_t i = lo; i < hi; i++) {
+        uint32_t idx = or_grid_cell(g, r[i].x, r[i].y);
+        double r_norm = sqrt(SQR(r[i].x) + SQR(r[i].y));
+        double vc_theta = sqrt(m_star / r_norm);
+        double v_g_theta = vc_theta + g->vtheta[idx];
+        double v_g_rad = g->vrad[idx];
+        or_var4 v_g_polar = {v_g_rad, v_g_theta, 0.0, 0.0};
+        double theta = r[i].y >= 0.0 ? atan2(r[i].y, r[i].x) : OR_TWOPI + atan2(r[i].y, r[i].x);
+        or_var4 v_g = or_rotate_2d(theta, v_g_polar);
+        double density_g = g->density[idx];
+        double ux = v_g.x - v[i].x, uy = v_g.y - v[i].y, uz = v_g.z - v[i].z;
+        double u_n = sqrt(SQR(ux) + SQR(uy) + SQR(uz));
+        double f = (3.0 * p[i].cd * density_g * u_n) / (8.0 * p[i].radius * p[i].density);
+        a[i].x += f * ux; a[i].y += f * uy; a[i].z += f * uz;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* pp_disk object                                                            */
 /* ------------------------------------------------------------------------ */
 struct or_disk {
@@ -528,6 +591,8 @@ struct or_disk {
     double thr[3];
     int has_gas;
     or_gas gas;
+    int has_grid;       /* a FARGO grid replaces the analytic disk */
+    or_grid grid;       /* owns its arrays: radii, density, vrad, vtheta in one block */
     int drag_mode, formula;
     or_event* events; uint32_t event_counter, event_cap;
     or_event* log; uint32_t n_log, log_cap;
@@ -558,7 +623,27 @@ void or_disk_destroy(or_disk* d)
 {
     if (!d) return;
     free(d->r); free(d->v); free(d->rout); free(d->vout); free(d->p); free(d->bmd);
-    free(d->events); free(d->log); free(d);
+    free(d->events); free(d->log); free((void*)d->grid.used_rad); free(d);
+}
+int or_disk_set_gas_grid(or_disk* d, uint32_t n_rad, uint32_t n_sec, const double* used_rad, const double* density,
+                         const double* vrad, const double* vtheta)
+{
+    free((void*)d->grid.used_rad);
+    memset(&d->grid, 0, sizeof(d->grid));
+    d->has_grid = 0;
+    if (!used_rad) return OR_OK;
+    if (n_rad == 0 || n_sec == 0) return OR_EINVAL;
+    size_t nr = (size_t)n_rad + 1, nc = (size_t)n_rad * n_sec;
+    double* b = (double*)malloc((nr + 3 * nc) * sizeof(double));
+    if (!b) return OR_EINVAL;
+    memcpy(b, used_rad, nr * sizeof(double));
+    memcpy(b + nr, density, nc * sizeof(double));
+    memcpy(b + nr + nc, vrad, nc * sizeof(double));
+    memcpy(b + nr + 2 * nc, vtheta, nc * sizeof(double));
+    d->grid.n_rad = n_rad; d->grid.n_sec = n_sec;
+    d->grid.used_rad = b; d->grid.density = b + nr; d->grid.vrad = b + nr + nc; d->grid.vtheta = b + nr + 2 * nc;
+    d->has_grid = 1;
+    return OR_OK;
 }
 void or_disk_set_formula(or_disk* d, int f) { d->formula = f; }
 void or_disk_set_drag_mode(or_disk* d, int m) { d->drag_mode = m; }
@@ -595,7 +680,8 @@ int or_calc_dydx(or_disk* d, uint32_t i, uint32_t rr, double t, const or_var4* r
     if (d->formula == OR_FORMULA_GPU) or_grav_gpu(d->n, d->playing, d->bmd, d->p, r, dy);
     else rc = or_grav_cpu(d->n, d->playing, d->bmd, d->p, r, dy);
     if (rc) return rc;
-    if (d->has_gas) or_drag(t, &d->gas, d->drag_mode, d->n, d->playing, d->p, r, v, dy);
+    if (d->has_grid) or_grid_drag(&d->grid, d->drag_mode, d->playing, d->p, r, v, dy);
+    else if (d->has_gas) or_drag(t, &d->gas, d->drag_mode, d->n, d->playing, d->p, r, v, dy);
     return OR_OK;
 }
 

@@ -112,6 +112,17 @@ int  or_grav_cpu(uint32_t n, const uint32_t playing[OR_NTYPE], const or_body_md*
 void or_drag(double t, const or_gas* g, int drag_mode, uint32_t n, const uint32_t playing[OR_NTYPE],
              const or_param* p, const or_var4* r, const or_var4* v, or_var4* a);
 
+/* FARGO gas grid, frame 0 (fargo_gas_disk.h:8-60): n_rad + 1 ring radii,
+ * n_rad * n_sec cells of density, radial and extra azimuthal velocity. */
+typedef struct or_grid {
+    uint32_t n_rad, n_sec;
+    const double *used_rad, *density, *vrad, *vtheta;
+} or_grid;
+or_var4  or_rotate_2d(double theta, or_var4 r);                 /* util.cu:76-83 */
+uint32_t or_grid_cell(const or_grid* g, double x, double y);   /* pp_disk.cu:158-193 */
+void     or_grid_drag(const or_grid* g, int drag_mode, const uint32_t playing[OR_NTYPE], const or_param* p,
+                      const or_var4* r, const or_var4* v, or_var4* a);   /* pp_disk.cu:778-814 */
+
 /* ---- disk + integrator objects ------------------------------------------ */
 typedef struct or_disk or_disk;
 typedef struct or_integ or_integ;
@@ -122,6 +133,9 @@ or_disk* or_disk_create(uint32_t n, const uint32_t n_by_type[OR_NTYPE],
 void     or_disk_destroy(or_disk* d);
 void     or_disk_set_formula(or_disk* d, int formula);
 void     or_disk_set_drag_mode(or_disk* d, int mode);
+/* copy a FARGO grid into the disk; used_rad NULL removes it */
+int      or_disk_set_gas_grid(or_disk* d, uint32_t n_rad, uint32_t n_sec, const double* used_rad,
+                              const double* density, const double* vrad, const double* vtheta);
 uint32_t or_disk_n(const or_disk* d);
 double   or_disk_t(const or_disk* d);
 void     or_disk_get(const or_disk* d, or_var4* r, or_var4* v, or_param* p, or_body_md* bmd);

@@ -198,6 +198,9 @@ struct rd_disk {
     rd_tuning tuning{2, 1024, 4, 0, 1, 1};
     double thr[3] = {0, 0, 0};
     GasDev gas{};
+    int32_t gas_created = kGasNone;  // the model given at creation (a grid's removal returns to it)
+    // FARGO grid on the device: ring radii, then density, v_rad, v_theta
+    double* grid = nullptr;
     uint64_t generation = 0;       // bumps when buffers / lists change (invalidates graphs)
 };
 
@@ -304,6 +307,7 @@ void free_buffers(rd_disk* d)
     dev_free(d->esum);
     dev_free(d->fx);
     dev_free(d->fy);
+    dev_free(d->grid);
     for (void** h : {(void**)&d->h_ctl, (void**)&d->h_err, (void**)&d->h_feat, (void**)&d->h_esum, (void**)&d->h_fx,
                      (void**)&d->h_fy, (void**)&d->h_errw}) {
         if (*h) (void)hipHostFree(*h);
@@ -837,7 +841,7 @@ int rd_batch_create(uint32_t n_disk, const uint32_t* n_by_type, const rd_var4* r
     d->bmd.assign(bmd, bmd + off);
     if (threshold) std::memcpy(d->thr, threshold, sizeof(d->thr));
     if (gas) {
-        d->gas.on = 1;
+        d->gas.on = d->gas_created = kGasAnalytic;
         d->gas.rho0 = gas->rho[0]; d->gas.rho1 = gas->rho[1];
         d->gas.sch0 = gas->sch[0]; d->gas.sch1 = gas->sch[1];
         d->gas.eta0 = gas->eta[0]; d->gas.eta1 = gas->eta[1];
@@ -913,6 +917,44 @@ int rd_disk_set_drag_mode(rd_disk* d, int mode)
     d->drag_mode = mode;
     d->lists_valid = false;
     return build_lists(d);
+}
+
+int rd_disk_set_gas_grid(rd_disk* d, const rd_gas_grid* g)
+{
+    DeviceScope dev_scope(d ? d->device : -1);
+    if (!d) return fail(RD_EINVAL, "NULL disk");
+    if (g) {
+        if (!g->used_rad || !g->density || !g->vrad || !g->vtheta) return fail(RD_EINVAL, "NULL grid array");
+        if (g->n_rad == 0 || g->n_sec == 0) return fail(RD_EINVAL, "the grid needs n_rad >= 1 and n_sec >= 1");
+        if ((uint64_t)g->n_rad * g->n_sec > (uint64_t)INT32_MAX)
+            return fail(RD_EINVAL, "the grid has more than 2^31 - 1 cells");
+        for (uint32_t k = 0; k <= g->n_rad; ++k) {
+            if (!std::isfinite(g->used_rad[k])) return fail(RD_EINVAL, "used_rad holds a value that is not finite");
+            if (k > 0 && g->used_rad[k] < g->used_rad[k - 1]) return fail(RD_EINVAL, "used_rad is not increasing");
+        }
+    }
+    // kernels queued on the disk's stream may still read the old grid
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    dev_free(d->grid);
+    d->gas.g_rad = d->gas.g_dens = d->gas.g_vrad = d->gas.g_vtheta = nullptr;
+    d->gas.n_rad = d->gas.n_sec = 0;
+    d->gas.on = d->gas_created;
+    d->generation++;
+    if (!g) return RD_OK;
+    const size_t nr = (size_t)g->n_rad + 1, nc = (size_t)g->n_rad * g->n_sec;
+    if (int rc = dev_alloc(&d->grid, nr + 3 * nc)) return rc;
+    HIP_TRY(hipMemcpy(d->grid, g->used_rad, nr * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d->grid + nr, g->density, nc * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d->grid + nr + nc, g->vrad, nc * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d->grid + nr + 2 * nc, g->vtheta, nc * sizeof(double), hipMemcpyHostToDevice));
+    d->gas.g_rad = d->grid;
+    d->gas.g_dens = d->grid + nr;
+    d->gas.g_vrad = d->grid + nr + nc;
+    d->gas.g_vtheta = d->grid + nr + 2 * nc;
+    d->gas.n_rad = g->n_rad;
+    d->gas.n_sec = g->n_sec;
+    d->gas.on = kGasGrid;
+    return RD_OK;
 }
 
 int rd_disk_set_tuning(rd_disk* d, const rd_tuning* t)

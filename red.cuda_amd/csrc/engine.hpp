@@ -75,12 +75,20 @@ struct WorkItem {
     uint32_t pad;
 };
 
-// Analytic gas disk (analytic_gas_disk_params_t red_type.h:464-485), device copy.
+// Gas disk, device copy: the analytic model (analytic_gas_disk_params_t
+// red_type.h:464-485) or frame 0 of a FARGO grid (fargo_gas_disk.h:8-60).
+enum : int32_t { kGasNone = 0, kGasAnalytic = 1, kGasGrid = 2 };
 struct GasDev {
     double rho0, rho1, sch0, sch1, eta0, eta1;
     double t0, t1, e_folding_time;
     int32_t gas_decrease;
-    int32_t on;
+    int32_t on;                  // kGasNone, kGasAnalytic or kGasGrid
+    // grid: n_rad rings x n_sec sectors, cell (ring, sector) at ring * n_sec + sector
+    const double* g_rad;         // n_rad + 1 ring interface radii, increasing
+    const double* g_dens;        // volume density
+    const double* g_vrad;        // radial velocity
+    const double* g_vtheta;      // azimuthal velocity on top of the circular one
+    uint32_t n_rad, n_sec;
 };
 
 struct StageArgs {

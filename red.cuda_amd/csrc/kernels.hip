@@ -14,6 +14,9 @@ namespace rdamd {
 
 #define SQR(x) ((x) * (x))
 
+// 2 pi rounded to double, the value of the reference's TWOPI (red_macro.h:17)
+constexpr double kTwoPi = 6.2831853071795864769253;
+
 // ---------------------------------------------------------------------------
 // Runge-Kutta tableaux
 // ---------------------------------------------------------------------------
@@ -698,6 +701,65 @@ __device__ double drag_component(const GasDev& g, double decr, double m_star, do
     return f * (q == 0 ? ux : (q == 1 ? uy : uz));
 }
 
+// Cell of the FARGO grid under (x, y) (calc_linear_index, pp_disk.cu:158-193),
+// from r = sqrt(x^2 + y^2) and alpha = atan2(y, x) taken into [0, 2 pi] as in
+// the reference.  Inside the grid, ring k holds g_rad[k] <= r < g_rad[k + 1]
+// (bisection over the increasing radii finds the k of the reference's linear
+// scan) and the sector is floor(alpha / (2 pi / n_sec)).  r below the first
+// interface reads cell 0, above the last reads the last cell; r equal to the
+// last interface (or NaN) fits no half-open ring, and the reference's scan
+// then leaves ring 0.  A sector of n_sec (alpha rounded to 2 pi) reads the
+// next ring's first cell, as the flat index does; the one index the
+// reference would read past the grid (there, and a negative alpha from
+// atan2(-0, x < 0), which it converts with undefined behaviour) is held
+// inside it: the last cell and sector 0.
+__device__ uint32_t grid_cell(const GasDev& g, double r, double alpha)
+{
+    const uint32_t n_cell = g.n_rad * g.n_sec;
+    if (g.g_rad[0] > r) return 0;
+    if (g.g_rad[g.n_rad] < r) return n_cell - 1;
+    uint32_t ring = 0;
+    if (g.g_rad[g.n_rad] != r) {
+        uint32_t lo = 0, hi = g.n_rad;   // g_rad[lo] <= r < g_rad[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (g.g_rad[mid] <= r) lo = mid;
+            else hi = mid;
+        }
+        ring = lo;
+    }
+    const double q = alpha / (kTwoPi / g.n_sec);
+    const uint32_t sec = q >= 0.0 ? (uint32_t)q : 0u;
+    const uint32_t cell = ring * g.n_sec + sec;
+    return cell < n_cell ? cell : n_cell - 1;
+}
+
+// Quadratic drag a += f u in a FARGO gas grid, component q (the reference's
+// CPU loop, pp_disk.cu:778-814; its GPU kernel reads only the analytic
+// disk's parameters).  The gas moves at the circular speed sqrt(m_star / r)
+// plus the cell's azimuthal velocity, with the cell's radial velocity,
+// rotated from (r, theta) to (x, y) (rotate_2D_vector, util.cu:76-83); no
+// reduction factor applies.
+__device__ double grid_drag_component(const GasDev& g, double m_star, double rx, double ry, double vx, double vy,
+                                      double vz, double cd, double radius, double density, int q)
+{
+    const double r = sqrt(SQR(rx) + SQR(ry));
+    const double at = atan2(ry, rx);
+    const double theta = ry >= 0.0 ? at : kTwoPi + at;
+    const uint32_t c = grid_cell(g, r, theta);
+    const double vc_theta = sqrt(m_star / r);
+    const double vg_theta = vc_theta + g.g_vtheta[c];
+    const double vg_rad = g.g_vrad[c];
+    const double ct = cos(theta), st = sin(theta);
+    const double gx = ct * vg_rad - st * vg_theta;
+    const double gy = st * vg_rad + ct * vg_theta;
+    const double rho = g.g_dens[c];
+    const double ux = gx - vx, uy = gy - vy, uz = 0.0 - vz;
+    const double u_n = sqrt(SQR(ux) + SQR(uy) + SQR(uz));
+    const double f = (3.0 * cd * rho * u_n) / (8.0 * radius * density);
+    return f * (q == 0 ? ux : (q == 1 ? uy : uz));
+}
+
 __device__ inline const double* kv_of(const StageArgs& a, int r, int q)
 {
     return r == 0 ? a.vel[q] : a.K + ((size_t)r * 6 + q) * a.n_all;
@@ -813,11 +875,14 @@ __device__ double stage_accel(const StageArgs& a, uint32_t i, int q, const DiskI
     const uint32_t li = i - di.off;
     if (a.mixed) s = b + s;
     if (!(a.id[i] > 0)) s = 0.0;
-    if (a.gas.on && li >= di.drag_lo && li < di.drag_hi) {
+    if (a.gas.on == kGasAnalytic && li >= di.drag_lo && li < di.drag_hi) {
         const double decr = reduction_factor(a.gas, t);
         if (!(1.0e-6 > decr))
             s += drag_component(a.gas, decr, a.mass[di.off], pos[0][i], pos[1][i], pos[2][i], vel[0][i], vel[1][i],
                                 vel[2][i], a.cd[i], a.radius[i], a.density[i], q);
+    } else if (a.gas.on == kGasGrid && li >= di.drag_lo && li < di.drag_hi) {
+        s += grid_drag_component(a.gas, a.mass[di.off], pos[0][i], pos[1][i], vel[0][i], vel[1][i], vel[2][i],
+                                 a.cd[i], a.radius[i], a.density[i], q);
     }
     return s;
 }

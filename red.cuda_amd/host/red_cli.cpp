@@ -4,13 +4,14 @@
 // the same data, dump, event, integral, info and log files out, so that an
 // existing red run directory can be continued with this build.
 //
-//   red -gpu -idir <dir> -p <parameter file> -i <start files> [-ga <gas disk file>]
+//   red -gpu -idir <dir> -p <parameter file> -i <start files> [-ga <gas disk file> | -gf <FARGO file>]
 //       [-odir <dir>] [-id_dev <n>] [-v] [-pts] [-pe] [-ef] [-t] [-h]
 //       [-prec fp64|mixed]
 //
-// -gpu runs the engine; the reference's CPU path (-cpu, -n_chg) and the FARGO
-// gas disk (-gf) are not part of this build and are refused.  -prec is this
-// build's addition (default fp64: the reference's GPU arithmetic, bit for bit).
+// -gpu runs the engine; the reference's CPU path (-cpu, -n_chg) is not part of
+// this build and is refused.  -gf reads the FARGO run in -idir (fargo_gas_disk.hpp
+// says how that differs from the reference).  -prec is this build's addition
+// (default fp64: the reference's GPU arithmetic, bit for bit).
 #include <sys/stat.h>
 
 #include <algorithm>
@@ -23,10 +24,12 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
 
+#include "fargo_gas_disk.hpp"
 #include "red_host.hpp"
 
 using namespace redhost;
@@ -404,6 +407,7 @@ struct Options {
     bool test = false, verbose = false, print_to_screen = false, print_dbe = false, ef = false;
     int id_dev = 0;
     std::string idir, odir, start_files, param_file, gas_file;
+    bool fargo = false;   // gas_file names a FARGO parameter file (-gf), not an analytic disk (-ga)
     int precision = RD_PREC_FP64;
 };
 
@@ -423,9 +427,11 @@ void usage()
                  "  -i <file>         The input file containing the filenames of the input data\n"
                  "  -p <file>         The input file containing the parameters of the simulation\n"
                  "  -ga <file>        The input file containing the parameters of an analytic gas disk\n"
+                 "  -gf <file>        The input file containing the details of the gas disk resulted from FARGO simulations\n"
+                 "                    (in -idir with used_rad.dat and frame 0, taken in AU, Msun/AU^3 and AU/tau)\n"
                  "  -ef               Prefix the output files with the run configuration\n\n"
                  "  -h                Display this information\n\n"
-                 "The CPU path (-cpu, -n_chg) and FARGO gas disks (-gf) are not part of this build.\n";
+                 "The CPU path (-cpu, -n_chg) is not part of this build.\n";
 }
 
 Options parse_options(int argc, char** argv)
@@ -458,8 +464,13 @@ Options parse_options(int argc, char** argv)
         else if (p == "-odir") o.odir = need(i, p);
         else if (p == "-i") o.start_files = need(i, p);
         else if (p == "-p") o.param_file = need(i, p);
-        else if (p == "-ga") o.gas_file = need(i, p);
-        else if (p == "-gf") throw std::string("FARGO gas disks (-gf) are not part of this build.");
+        else if (p == "-ga") {
+            o.gas_file = need(i, p);
+            o.fargo = false;
+        } else if (p == "-gf") {
+            o.gas_file = need(i, p);
+            o.fargo = true;
+        }
         else if (p == "-h") {
             usage();
             std::exit(EXIT_SUCCESS);
@@ -708,9 +719,13 @@ int main(int argc, char** argv)
         run.dt_cpu_offset = info.dt_cpu;
 
         rd_gas_params gas;
-        const bool has_gas = !run.opt.gas_file.empty();
+        const bool has_gas = !run.opt.gas_file.empty() && !run.opt.fargo;
         if (has_gas) gas = load_gas(combine_path(run.opt.idir, run.opt.gas_file), n ? B.p[0].mass : 1.0);
+        // the FARGO files are read before the GPU is touched, as the other inputs
+        std::unique_ptr<fargo_gas_disk> fgd;
+        if (run.opt.fargo) fgd.reset(new fargo_gas_disk(run.opt.idir, run.opt.gas_file));
         pp_disk ppd(B.r, B.v, B.p, B.b, info.n, info.t, run.par.thr, has_gas ? &gas : nullptr, run.opt.id_dev);
+        if (fgd) ppd.set_gas_grid(fgd->grid());
         if (run.opt.precision == RD_PREC_MIXED) ppd.set_precision(RD_PREC_MIXED);
         double dt = info.dt;
         if (run.par.length < 0.0 && dt > 0.0) dt = -dt;   // backward integration

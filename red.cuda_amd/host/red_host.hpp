@@ -59,6 +59,9 @@ class pp_disk {
     rd_disk* handle() { return h_; }
     void set_precision(int prec) { rd_check(rd_disk_set_precision(h_, prec), "set_precision"); }
     void set_drag_mode(int mode) { rd_check(rd_disk_set_drag_mode(h_, mode), "set_drag_mode"); }
+    // GAS_DISK_MODEL_FARGO: drag through a FARGO grid (fargo_gas_disk.hpp) in
+    // place of the analytic disk; the grid is copied to the device
+    void set_gas_grid(const rd_gas_grid& g) { rd_check(rd_disk_set_gas_grid(h_, &g), "set_gas_grid"); }
     // pp_disk::benchmark (pp_disk.cu:553-611): picks the launch shape by timing
     rd_tuning benchmark()
     {

@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "librdamd.so")
 if HERE not in sys.path:
     sys.path.insert(0, HERE)
 
-from redamd_types import (VAR4, PARAM, BODY_MD, INTEGRAL, EVENT, GasParams, Tuning, DiskSpec,  # noqa: F401
+from redamd_types import (VAR4, PARAM, BODY_MD, INTEGRAL, EVENT, GasParams, GasGrid, Tuning, DiskSpec,  # noqa: F401
                           RD_OK, RD_EINVAL, RD_EDEVICE, RD_ENOTIMPL, RD_EOVERFLOW, RD_ESTATE, RD_ENOMEM,
                           RD_EULER, RD_RK2, RD_RK4, RD_RKF78, RD_PREC_FP64, RD_PREC_MIXED, RD_DRAG_NSI, RD_DRAG_ALL_MASSIVE,
                           BODY_STAR, BODY_GIANT, BODY_ROCKY, BODY_PROTO, BODY_SUPERPL, BODY_PL, BODY_TP,
@@ -52,6 +52,7 @@ _SIGS = {
     "rd_disk_destroy": (C.c_int, [_P]),
     "rd_disk_set_precision": (C.c_int, [_P, C.c_int]),
     "rd_disk_set_drag_mode": (C.c_int, [_P, C.c_int]),
+    "rd_disk_set_gas_grid": (C.c_int, [_P, _P]),
     "rd_disk_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "rd_disk_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "rd_autotune": (C.c_int, [_P, C.POINTER(Tuning)]),
@@ -232,6 +233,21 @@ class Disk:
 
     def set_drag_mode(self, mode):
         check(lib.rd_disk_set_drag_mode(self.h, mode))
+
+    def set_gas_grid(self, used_rad, density, vrad, vtheta):
+        """Drag through frame 0 of a FARGO grid: used_rad holds the n_rad + 1
+        ring interface radii, the three fields are (n_rad, n_sec) arrays in the
+        solver's units.  used_rad=None removes the grid."""
+        if used_rad is None:
+            check(lib.rd_disk_set_gas_grid(self.h, None))
+            return
+        ur = np.ascontiguousarray(used_rad, np.float64)
+        f = [np.ascontiguousarray(x, np.float64) for x in (density, vrad, vtheta)]
+        n_rad = ur.size - 1
+        if n_rad < 1 or f[0].ndim != 2 or f[0].shape[0] != n_rad or any(x.shape != f[0].shape for x in f):
+            raise ValueError("grid fields must be (n_rad, n_sec) with n_rad + 1 radii")
+        g = GasGrid(n_rad, f[0].shape[1], ur.ctypes.data, f[0].ctypes.data, f[1].ctypes.data, f[2].ctypes.data)
+        check(lib.rd_disk_set_gas_grid(self.h, C.byref(g)))
 
     def tuning(self):
         t = Tuning()

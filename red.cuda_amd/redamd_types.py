@@ -51,6 +51,12 @@ class GasParams(C.Structure):
                 ("particle_diameter", C.c_double)]
 
 
+class GasGrid(C.Structure):
+    """rd_gas_grid: frame 0 of a FARGO run (fargo_gas_disk.h:8-60)."""
+    _fields_ = [("n_rad", C.c_uint32), ("n_sec", C.c_uint32), ("used_rad", C.c_void_p),
+                ("density", C.c_void_p), ("vrad", C.c_void_p), ("vtheta", C.c_void_p)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("sinks_per_lane", C.c_uint32), ("chunk", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("use_graph", C.c_uint32), ("pair_mode", C.c_uint32),

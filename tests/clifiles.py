@@ -88,3 +88,56 @@ def read_run(directory, var4, param, body_md):
         r["x"][i], r["y"][i], r["z"][i] = (float(x) for x in f[9:12])
         v["x"][i], v["y"][i], v["z"][i] = (float(x) for x in f[12:15])
     return r, v, p, b, nbt, t, dt
+
+
+FARGO_PAR = """### Disk parameters
+AspectRatio     0.05            Thickness over Radius in the disc
+Sigma0          6.3661977237e-4 Surface Density at r=1
+AlphaViscosity  0.001           Uniform kinematic viscosity
+SigmaSlope      0.5             Slope of surface density profile.
+FlaringIndex    0.0
+ExcludeHill     YES
+
+### Planet parameters
+PlanetConfig    in/Jup.cfg
+ThicknessSmoothing 0.6          Smoothing parameters in disk thickness
+
+### Numerical method parameters
+Transport       FARGO
+InnerBoundary   OPEN            choose : OPEN or RIGID or NONREFLECTING
+Disk            YES
+OmegaFrame      0
+Frame           COROTATING
+IndirectTerm    YES
+
+### Mesh parameters
+Nrad            {n_rad}             Radial number of zones
+Nsec            {n_sec}             Azimuthal number of zones (sectors)
+Rmin            {r_min}             Inner boundary radius
+Rmax            {r_max}             Outer boundary radius
+RadialSpacing   LOGARITHMIC     Zone interfaces evenly spaced
+
+### Output control parameters
+Ntot            1000            Total number of time steps
+Ninterm         20              Time steps between outputs
+DT              0.314159265359  Time step length. 2PI = 1 orbit
+OutputDir       out
+"""
+
+
+def write_fargo(directory, used_rad, dens, vrad, vtheta, par="fargo.par", crlf=False, par_text=None):
+    """A FARGO output directory as the reference reads it (fargo_gas_disk.cu:15-255):
+    the parameter file, used_rad.dat (n_rad + 1 ring radii, one per line) and frame 0
+    (gasdens0.dat, gasvrad0.dat, gasvtheta0.dat: n_rad * n_sec doubles each)."""
+    n_rad, n_sec = dens.shape
+    text = par_text if par_text is not None else FARGO_PAR.format(n_rad=n_rad, n_sec=n_sec, r_min=used_rad[0],
+                                                                  r_max=used_rad[-1])
+    nl = "\r\n" if crlf else "\n"
+    with open(os.path.join(directory, par), "w", newline="") as f:
+        f.write(text.replace("\n", nl))
+    with open(os.path.join(directory, "used_rad.dat"), "w", newline="") as f:
+        for x in used_rad:
+            f.write(f"{x:.17g}{nl}")
+    for name, a in (("gasdens0.dat", dens), ("gasvrad0.dat", vrad), ("gasvtheta0.dat", vtheta)):
+        with open(os.path.join(directory, name), "wb") as f:
+            f.write(np.ascontiguousarray(a, "<f8").tobytes())

@@ -16,12 +16,21 @@ lib = C.CDLL(LIB)
 _P, _u32, _u64, _i32, _f64 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32, C.c_double
 
 
+class VAR4C(C.Structure):
+    _fields_ = [(n, C.c_double) for n in "xyzw"]
+
+
 class OrRng(C.Structure):
     _fields_ = [("I", C.c_uint32 * 624), ("idx", C.c_uint32)]
 
 
 class OrOrbelem(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("sma", "ecc", "inc", "peri", "node", "mean")]
+
+
+class OrGrid(C.Structure):
+    _fields_ = [("n_rad", C.c_uint32), ("n_sec", C.c_uint32), ("used_rad", C.c_void_p),
+                ("density", C.c_void_p), ("vrad", C.c_void_p), ("vtheta", C.c_void_p)]
 
 
 class OrDiskSpec(C.Structure):
@@ -55,6 +64,10 @@ _SIGS = {
     "or_grav_gpu": (None, [_u32, _P, _P, _P, _P, _P]),
     "or_grav_cpu": (C.c_int, [_u32, _P, _P, _P, _P, _P]),
     "or_drag": (None, [_f64, C.POINTER(GasParams), C.c_int, _u32, _P, _P, _P, _P, _P]),
+    "or_rotate_2d": (VAR4C, [_f64, VAR4C]),
+    "or_grid_cell": (_u32, [C.POINTER(OrGrid), _f64, _f64]),
+    "or_grid_drag": (None, [C.POINTER(OrGrid), C.c_int, _P, _P, _P, _P, _P]),
+    "or_disk_set_gas_grid": (C.c_int, [_P, _u32, _u32, _P, _P, _P, _P]),
     "or_disk_create": (_P, [_u32, _P, _P, _P, _P, _P, _f64, _P, _P]),
     "or_disk_destroy": (None, [_P]),
     "or_disk_set_formula": (None, [_P, C.c_int]),
@@ -109,6 +122,18 @@ def make_disk(n, seed, body_type=3, a_min=0.5, a_max=5.0, a_power=-0.5, sig_e=0.
     return r, v, p, b, nbt
 
 
+class OGrid:
+    """A FARGO grid for the oracle's free functions (keeps its arrays alive)."""
+
+    def __init__(self, used_rad, density, vrad, vtheta):
+        self.arrays = [np.ascontiguousarray(x, np.float64) for x in (used_rad, density, vrad, vtheta)]
+        ur, de, vr, vt = self.arrays
+        self.g = OrGrid(ur.size - 1, de.shape[1], ur.ctypes.data, de.ctypes.data, vr.ctypes.data, vt.ctypes.data)
+
+    def cell(self, x, y):
+        return lib.or_grid_cell(C.byref(self.g), x, y)
+
+
 class ODisk:
     """Oracle pp_disk."""
 
@@ -145,6 +170,13 @@ class ODisk:
         a, b, c = (np.zeros(7, np.uint32) for _ in range(3))
         lib.or_disk_counts(self.h, ptr(a), ptr(b), ptr(c))
         return a, b, c
+
+    def set_gas_grid(self, used_rad, density, vrad, vtheta):
+        if used_rad is None:
+            assert lib.or_disk_set_gas_grid(self.h, 0, 0, None, None, None, None) == 0
+            return
+        ur, de, vr, vt = (np.ascontiguousarray(x, np.float64) for x in (used_rad, density, vrad, vtheta))
+        assert lib.or_disk_set_gas_grid(self.h, ur.size - 1, de.shape[1], ptr(ur), ptr(de), ptr(vr), ptr(vt)) == 0
 
     def calc_dydx(self, i, rr, t, r, v):
         dy = np.zeros(len(r), VAR4)

@@ -49,3 +49,18 @@ def bits_equal(a, b):
 
 def as3(a):
     return np.stack([a["x"], a["y"], a["z"]], 1)
+
+
+def fargo_grid(n_rad, n_sec, seed, r_min=0.4, r_max=2.5, rho1=2e-3, vr=1e-3, vt=3e-3):
+    """A frame-0 gas grid shaped like a FARGO run: log-spaced ring interfaces
+    (n_rad + 1 radii, FARGO's logarithmic RadialSpacing) and (n_rad, n_sec)
+    cells of midplane density ~ rho1 r^-2.75 [Msun/AU^3; 2e-3 is a minimum-mass
+    nebula's at 1 AU], radial velocity and
+    sub-Keplerian azimuthal offset [AU/tau], each with seeded cell noise."""
+    rng = np.random.default_rng(seed)
+    used_rad = r_min * (r_max / r_min) ** (np.arange(n_rad + 1) / n_rad)
+    rc = np.sqrt(used_rad[:-1] * used_rad[1:])[:, None]
+    dens = rho1 * rc ** -2.75 * rng.uniform(0.5, 1.5, (n_rad, n_sec))
+    vrad = vr * rng.uniform(-1.0, 1.0, (n_rad, n_sec))
+    vtheta = -vt * rc ** -0.5 * rng.uniform(0.5, 1.5, (n_rad, n_sec))
+    return used_rad, dens, vrad, vtheta

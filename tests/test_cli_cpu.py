@@ -20,14 +20,13 @@ def red(*args, cwd=None):
 def test_usage():
     res = red("-h")
     assert res.returncode == 0
-    for sw in ("-gpu", "-idir", "-odir", "-i", "-p", "-ga", "-id_dev", "-ef", "-pts", "-pe", "-prec"):
+    for sw in ("-gpu", "-idir", "-odir", "-i", "-p", "-ga", "-gf", "-id_dev", "-ef", "-pts", "-pe", "-prec"):
         assert sw in res.stdout
 
 
 @pytest.mark.parametrize("args,msg", [
     (["-cpu"], "CPU path"),
     (["-n_chg", "4"], "CPU path"),
-    (["-gf", "x"], "FARGO"),
     (["-bogus"], "Invalid switch"),
     (["-prec", "fp16"], "-prec"),
     (["-id_dev", "x"], "Invalid number"),
@@ -103,3 +102,76 @@ def test_units_of_the_parameter_file():
     # parameter.cpp: years -> days (365.25) -> k days (Gauss' constant)
     assert CF.internal_time("1.0") == 365.25 * 1.720209895e-2
     assert CF.K_GAUSS == R.K_GAUSS
+
+
+def _fargo_dir(tmp_path, **kw):
+    """A valid run directory with a FARGO run beside it (fargo.par, used_rad.dat, frame 0)."""
+    from systems import fargo_grid
+    d = _dir(tmp_path, {"length": "1.0"})
+    grid = fargo_grid(6, 10, 3)
+    CF.write_fargo(d, *grid, **kw)
+    return d, grid
+
+
+def _red_fargo(d):
+    return red("-gpu", "-idir", d, "-p", "parameters.txt", "-i", "start_files.txt", "-gf", "fargo.par")
+
+
+FARGO_ERRORS = [
+    # (what to break, the reference's message)
+    ("missing_par", "could not opened"),
+    ("bad_key", "Invalid parameter :bogus."),
+    ("bad_number", "Invalid number at: nrad."),
+    ("no_value", "Invalid key/value pair: Nsec."),
+    ("no_nrad", "does not set Nrad and Nsec"),
+    ("bad_radius", "Invalid number (0.4x) in file"),
+    ("few_radii", "radii; Nrad + 1 = 7 expected"),
+    ("short_frame", "gasvrad0.dat' has different number of data than expected!"),
+    ("missing_frame", "gasvtheta0.dat' could not opened!"),
+]
+
+
+@pytest.mark.parametrize("case,msg", FARGO_ERRORS)
+def test_fargo_file_errors(tmp_path, case, msg):
+    """fargo_gas_disk's checks (fargo_gas_disk.cu:15-43, 190-249, 293-549;
+    file_util.cpp:101-153), raised before the GPU is touched."""
+    d, grid = _fargo_dir(tmp_path)
+    par = os.path.join(d, "fargo.par")
+    text = open(par).read()
+    if case == "missing_par":
+        os.remove(par)
+    elif case == "bad_key":
+        open(par, "w").write(text + "Bogus 1\n")
+    elif case == "bad_number":
+        open(par, "w").write(text.replace("Nrad            6", "Nrad            six"))
+    elif case == "no_value":
+        open(par, "w").write(text + "Nsec\n")
+    elif case == "no_nrad":
+        open(par, "w").write(text.replace("Nrad            6", "Ntot 5"))
+    elif case == "bad_radius":
+        ur = open(os.path.join(d, "used_rad.dat")).read().splitlines()
+        ur[0] = "0.4x"
+        open(os.path.join(d, "used_rad.dat"), "w").write("\n".join(ur) + "\n")
+    elif case == "few_radii":
+        ur = open(os.path.join(d, "used_rad.dat")).read().splitlines()
+        open(os.path.join(d, "used_rad.dat"), "w").write("\n".join(ur[:-1]) + "\n")
+    elif case == "short_frame":
+        b = open(os.path.join(d, "gasvrad0.dat"), "rb").read()
+        open(os.path.join(d, "gasvrad0.dat"), "wb").write(b[:-8])
+    elif case == "missing_frame":
+        os.remove(os.path.join(d, "gasvtheta0.dat"))
+    res = _red_fargo(d)
+    assert res.returncode != 0
+    assert msg in res.stderr, res.stderr
+
+
+@pytest.mark.parametrize("crlf", [False, True])
+def test_fargo_files_are_read(tmp_path, crlf):
+    """A FARGO directory written as FARGO writes it -- comment lines, trailing
+    words after each value, CRLF line ends or not -- passes every file check:
+    whatever the run then reports, it is none of fargo_gas_disk's errors."""
+    d, _ = _fargo_dir(tmp_path, crlf=crlf)
+    res = _red_fargo(d)
+    for _, msg in FARGO_ERRORS:
+        assert msg not in res.stderr, res.stderr
+    assert "Invalid" not in res.stderr and "could not opened" not in res.stderr, res.stderr

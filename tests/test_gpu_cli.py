@@ -211,6 +211,48 @@ def test_cli_mixed_precision_run(tmp_path):
     assert np.abs(rr - np.stack([orr["x"], orr["y"], orr["z"]], 1)).max() < 1e-7
 
 
+def test_cli_fargo_gas_run(tmp_path):
+    """red -gf: planetesimals dragged through frame 0 of a FARGO run in -idir
+    (written as FARGO writes it, CRLF line ends) follow the oracle's driver loop
+    with the same grid -- to 1e-10 AU, the drag taking device atan2, sin and
+    cos -- and end far from the same run without gas."""
+    from systems import fargo_grid
+    r, v, p, b, nbt = R.make_disk(96, 5, body_type=R.BODY_PL, a_min=0.8, a_max=1.2, sig_e=0.02, sig_i=0.001,
+                                  size_mode=1, lo=1.0, hi=100.0, cd=1.0)
+    d = str(tmp_path)
+    params = dict(PARAMS, length="0.2")
+    params["radii enhance factor"] = "1"
+    dt = 1.0 * R.K_GAUSS
+    CF.write_run(d, r, v, p, b, nbt, 0.0, dt, params)
+    grid = fargo_grid(48, 128, 12, rho1=2e-3)
+    CF.write_fargo(d, *grid, crlf=True)
+    _run(d, "-gf", "fargo.par")
+    last = sorted(glob.glob(os.path.join(d, "data_*[0-9].txt")))[-1]
+    _, ids, rr, vv, _ = CF.read_data(last, R.BODY_MD)
+    t = float(open(last.replace(".txt", ".info.txt")).read().split()[0])
+    thr = (float(params["hit centrum"]), float(params["ejection"]), float(params["radii enhance factor"]))
+    ends = []
+    for with_grid in (True, False):
+        o = O.ODisk(r, v, p, b, nbt, thr=thr)
+        if with_grid:
+            o.set_gas_grid(*grid)
+        oi = O.OInteg(o, 4, dt, adaptive=True, tol=float(params["tolerance"]))
+        rc, _ = oi.run(CF.internal_time(params["length"]), CF.internal_time(params["output interval"]), 2 ** 62)
+        assert rc == 0
+        # the loop ends with the first step past the length: ulp-level drag
+        # differences may move that step, and without gas it differs by steps;
+        # carry the oracle to the run's end time with one fixed step
+        if with_grid:
+            assert abs(t - o.t) < 1e-9 * o.t
+        if o.t != t:
+            O.OInteg(o, 4, t - o.t, adaptive=False).step()
+        ot, orr, ovv, _, obb = o.state()
+        assert np.array_equal(ids, obb["id"])
+        ends.append((np.stack([orr["x"], orr["y"], orr["z"]], 1), np.stack([ovv["x"], ovv["y"], ovv["z"]], 1)))
+    assert np.abs(rr - ends[0][0]).max() < 1e-10 and np.abs(vv - ends[0][1]).max() < 1e-10
+    assert np.abs(vv - ends[1][1]).max() > 1e-8
+
+
 def test_cli_self_test():
     res = subprocess.run([RED, "-t"], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0 and "ok" in res.stdout, res.stdout + res.stderr
