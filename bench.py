@@ -116,10 +116,12 @@ def cpu_rk4_baseline(r, v, p, b, nbt, h, steps):
                       f"{cpu_model()} (nproc {os.cpu_count()})"}
 
 
-def cpu_force_baseline(r, v, p, b, nbt, gas=None, drag_mode=0, pairs_per_step=None):
+def cpu_force_baseline(r, v, p, b, nbt, gas=None, drag_mode=0, pairs_per_step=None, grid=None):
     """One force evaluation (gravity + drag) of the reference CPU path on the same disk (1 thread)."""
     O = _oracle()
     o = O.ODisk(r, v, p, b, nbt, formula=1, gas=gas, drag_mode=drag_mode)
+    if grid is not None:
+        o.set_gas_grid(*grid)
     t0 = time.perf_counter()
     o.calc_dydx(1, 0, 0.0, r, v)
     el = time.perf_counter() - t0
@@ -220,10 +222,12 @@ def roofline(d, precision, traffic=None):
             "ms_per_launch": ms, "pairs_per_launch": pairs_launch}
 
 
-def roofline_stage(d, n_bodies, method=R.RD_RKF78, traffic=None):
-    """HBM roofline of one attempt's RK passes (rd_bench_stages, HIP events)."""
+def roofline_stage(d, n_bodies, method=R.RD_RKF78, traffic=None, drag_bytes=0):
+    """HBM roofline of one attempt's RK passes (rd_bench_stages, HIP events);
+    drag_bytes: what the drag reads per body and attempt on top of the RK
+    vectors."""
     ms = d.bench_stages(method, 20)
-    alg = STAGE_BYTES[method] * n_bodies
+    alg = (STAGE_BYTES[method] + drag_bytes) * n_bodies
     achieved = alg / (ms * 1e-3)
     return {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
             "frac": achieved / PEAK_HBM, "traffic": traffic, "algorithmic_bytes": alg,
@@ -310,14 +314,35 @@ def bench_config1(args):
     emit(out)
 
 
+# a FARGO-shaped frame 0 (512 rings x 1024 sectors, the reference's FARGO test
+# size, src/red.code_snippet.cuda.test/main.cu:1142-1143) over 0.4 - 6 AU
+C3_GRID = (512, 1024, 0.4, 6.0)
+
+
+def grid_from_analytic(g, n_rad, n_sec, r_min, r_max, m_star=1.0):
+    """Frame 0 sampled from the analytic disk g at the cell centres: midplane
+    density rho_c r^rho_p (pp_disk.cu:87-105 with z = 0), no radial velocity,
+    and the azimuthal offset sqrt(m/r) (sqrt(1 - 2 eta_c r^eta_p) - 1) of the
+    sub-Keplerian gas (pp_disk.cu:145-155)."""
+    used_rad = r_min * (r_max / r_min) ** (np.arange(n_rad + 1) / n_rad)
+    rc = np.sqrt(used_rad[:-1] * used_rad[1:])
+    dens = np.repeat((g.rho[0] * rc ** g.rho[1])[:, None], n_sec, 1)
+    vt = np.sqrt(m_star / rc) * (np.sqrt(1.0 - 2.0 * g.eta[0] * rc ** g.eta[1]) - 1.0)
+    return used_rad, dens, np.zeros_like(dens), np.repeat(vt[:, None], n_sec, 1)
+
+
 def bench_config3(args):
     """Config 3: N = 131072 in a gas disk, drag on all massive bodies, perfect merging
-    (radius factor 1), hit-centrum 0.1 AU, mixed precision, RKF7(8)."""
+    (radius factor 1), hit-centrum 0.1 AU, mixed precision, RKF7(8).  --gas grid
+    drags through a FARGO frame sampled from the same analytic disk."""
     n = 131072 if args.n == 16384 else args.n
     steps = args.steps if args.steps != 1000 else 40
     r, v, p, b, nbt = R.make_disk(n, 3, size_mode=1, lo=1.0, hi=100.0, cd=1.0)
     g = R.gas_params()
-    d = R.Disk(r, v, p, b, nbt, threshold=(0.1, 0.0, 1.0), gas=g)
+    grid = grid_from_analytic(g, *C3_GRID) if args.gas == "grid" else None
+    d = R.Disk(r, v, p, b, nbt, threshold=(0.1, 0.0, 1.0), gas=g if grid is None else None)
+    if grid is not None:
+        d.set_gas_grid(*grid)
     d.set_drag_mode(R.RD_DRAG_ALL_MASSIVE)
     d.set_precision(R.RD_PREC_MIXED)
     tuning = set_tuning_arg(d, args.tuning) if args.tuning else (d.autotune() if not args.no_autotune else d.tuning())
@@ -341,20 +366,24 @@ def bench_config3(args):
            "unit": "pair interactions/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
            "ms_per_step": el * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64 state / f32 pair force", "data": "synthetic seeded disk",
-           "config": {"workload": f"N={n} in a gas disk (Sigma=1700 r^-1.5, H/r=0.05 r^0.25, eta=0.002), drag on "
-                                  "all massive bodies, R 1-100 km, merging f=1, hit-centrum 0.1 AU, mixed, RKF7(8) "
-                                  f"tol {args.tol:g}",
-                      "tuning": tuning_json(tuning)},
+           "config": {"workload": f"N={n} in a gas disk (Sigma=1700 r^-1.5, H/r=0.05 r^0.25, eta=0.002"
+                                  + (", as a FARGO frame 0 of %dx%d cells over %g-%g AU" % C3_GRID
+                                     if grid is not None else "")
+                                  + "), drag on all massive bodies, R 1-100 km, merging f=1, hit-centrum 0.1 AU, "
+                                  f"mixed, RKF7(8) tol {args.tol:g}",
+                      "gas": args.gas, "tuning": tuning_json(tuning)},
            "rkf78_steps_per_s": steps / el, "attempts": c1[0] - c0[0], "rejected": c1[2] - c0[2],
            "events": int(len(d.event_log())), "rel_energy_error": rel_e,
            "note": "energy is not conserved with gas drag; the error column is the drag dissipation",
            "roofline": roofline(d, "mixed", traffic=profiled_traffic(
                pair_kernel("mixed", tuning), f"{pair_kernel('mixed', tuning)}_disk131072_gas_rkf78.txt", tuning)
-               if n == 131072 else None)}
+               if n == 131072 and grid is None else None),
+           # the drag reads R, rho_p and C_D per body and stage, the grid also its cell's three fields
+           "roofline_stage": roofline_stage(d, n + 1, drag_bytes=13 * (24 + (24 if grid is not None else 0)))}
     if not args.no_cpu:
         print("# cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_force_baseline(r, v, p, b, nbt, gas=g, drag_mode=1,
-                                                 pairs_per_step=(p1 - p0) / steps)
+        out["cpu_baseline"] = cpu_force_baseline(r, v, p, b, nbt, gas=g if grid is None else None, drag_mode=1,
+                                                 pairs_per_step=(p1 - p0) / steps, grid=grid)
     emit(out)
 
 
@@ -555,6 +584,8 @@ def main():
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5],
                     help="1 RK4 N=256, 2 RKF78 N=16384 (default), 3 gas+collisions N=131072, "
                          "4 batch of 512 disks, 5 learned step proposer")
+    ap.add_argument("--gas", choices=["analytic", "grid"], default="analytic",
+                    help="config 3: the analytic gas disk, or a FARGO frame 0 sampled from it")
     ap.add_argument("--log-steps", type=int, default=2000, help="config 5: logged steps per disk")
     ap.add_argument("--epochs", type=int, default=10, help="config 5: training epochs")
     ap.add_argument("--safety", type=float, default=None, help="config 5: proposer safety factor (default: "

@@ -94,11 +94,11 @@ typedef struct rd_gas_params {                                          /* analy
     double alpha, mean_molecular_weight, particle_diameter;
 } rd_gas_params;
 /* Frame 0 of a FARGO gas run (fargo_gas_disk.h:8-60; the frame the reference
- * loads, fargo_gas_disk.cu:190-206): n_rad rings x n_sec sectors, the cell of
+ * loads, fargo_gas_disk.cu:190-205): n_rad rings x n_sec sectors, the cell of
  * ring k and sector s at k * n_sec + s.  used_rad holds the n_rad + 1 ring
  * interface radii [AU], increasing (FARGO's used_rad.dat).  Values are taken in
  * the solver's units -- density [Msun/AU^3], velocities [AU/tau] -- since the
- * reference's unit transforms are not implemented (fargo_gas_disk.cu:256-272);
+ * reference's unit transforms are not implemented (fargo_gas_disk.cu:252-274);
  * vtheta is the gas's azimuthal velocity on top of the circular one. */
 typedef struct rd_gas_grid {
     uint32_t n_rad, n_sec;

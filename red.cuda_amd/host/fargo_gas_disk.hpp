@@ -11,7 +11,7 @@
 //   - the frames are taken as they are, in the solver's units (density
 //     [Msun/AU^3], velocities [AU/tau]), where the reference's transform_time,
 //     transform_velocity and transform_density throw "not yet implemented"
-//     (fargo_gas_disk.cu:256-272).  FARGO's own units with r0 = 1 AU and
+//     (fargo_gas_disk.cu:252-274).  FARGO's own units with r0 = 1 AU and
 //     M = 1 Msun make the time and velocity transforms the identity; the
 //     density must be the midplane volume density.
 //   - used_rad.dat must hold n_rad + 1 radii (FARGO writes the n_rad + 1 ring
@@ -116,7 +116,7 @@ class fargo_gas_disk {
     }
     // Lines split at '\r' / '\n' (empty ones skipped), each "key value ..." with
     // blanks or tabs between; words past the second are ignored
-    // (fargo_gas_disk::parse, fargo_gas_disk.cu:293-327).
+    // (fargo_gas_disk::parse, fargo_gas_disk.cu:290-328).
     void parse(const std::string& text)
     {
         size_t pos = 0;
@@ -139,7 +139,7 @@ class fargo_gas_disk {
             set_param(words[0], words[1]);
         }
     }
-    // fargo_gas_disk::set_param (fargo_gas_disk.cu:329-549)
+    // fargo_gas_disk::set_param (fargo_gas_disk.cu:330-555)
     void set_param(std::string key, std::string value)
     {
         key = trimmed(key);
@@ -190,7 +190,7 @@ class fargo_gas_disk {
     }
     // One radius per line, at most 30 characters each; a line longer than
     // that is read in pieces of 30 with the character between them dropped,
-    // as the reference's reader does (fargo_gas_disk.cu:223-249).
+    // as the reference's reader does (fargo_gas_disk.cu:225-250).
     void load_used_rad(const std::string& path)
     {
         const std::string text = load_ascii(path);

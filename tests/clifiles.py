@@ -126,7 +126,7 @@ OutputDir       out
 
 
 def write_fargo(directory, used_rad, dens, vrad, vtheta, par="fargo.par", crlf=False, par_text=None):
-    """A FARGO output directory as the reference reads it (fargo_gas_disk.cu:15-255):
+    """A FARGO output directory as the reference reads it (fargo_gas_disk.cu:15-250):
     the parameter file, used_rad.dat (n_rad + 1 ring radii, one per line) and frame 0
     (gasdens0.dat, gasvrad0.dat, gasvtheta0.dat: n_rad * n_sec doubles each)."""
     n_rad, n_sec = dens.shape

@@ -133,7 +133,7 @@ FARGO_ERRORS = [
 
 @pytest.mark.parametrize("case,msg", FARGO_ERRORS)
 def test_fargo_file_errors(tmp_path, case, msg):
-    """fargo_gas_disk's checks (fargo_gas_disk.cu:15-43, 190-249, 293-549;
+    """fargo_gas_disk's checks (fargo_gas_disk.cu:15-42, 190-250, 290-555;
     file_util.cpp:101-153), raised before the GPU is touched."""
     d, grid = _fargo_dir(tmp_path)
     par = os.path.join(d, "fargo.par")
