@@ -79,81 +79,101 @@ __host__ __device__ constexpr double rk4_c(int r) { return (r == 1 || r == 2) ? 
 //
 // The sum per (sink, component) is a chain of FP64 adds in j order, so it
 // cannot be split; the pair terms feeding it can.  A block takes kS sinks
-// (16, or 4 for disks whose 16-sink items would not reach every CU): for each
-// 64-source LDS tile, all 256 threads compute the kS x 64 terms
-// (m_j / (d * d^2)) * dx (sqrt and divide are the costly part) into LDS, then
-// 3 kS lanes of wave 0 -- one per (sink, component) -- add the tile's terms in
-// j order to their running sums.  Same terms, same order: the result is
-// bit-identical to one lane summing alone, for either kS, while the term work
-// spreads over N / kS blocks of 4 waves instead of N lanes.
-constexpr uint32_t kExactTile = 64;
+// (16, or 4 for disks whose 16-sink items would not reach every CU) and has
+// five waves.  Waves 0-3 compute, for each LDS tile of kTile sources, the
+// kS x kTile terms (m_j / (d * d^2)) * dx (sqrt and divide are the costly
+// part) into LDS; 3 kS lanes of wave 4 -- one per (sink, component) -- add a
+// tile's terms in j order to their running sums.  Terms and sources are
+// double-buffered: wave 4 adds tile t - 1 while waves 0-3 compute tile t, one
+// barrier per tile, so a tile costs the longer of the two, not their sum.
+// Same terms, same order: the result is bit-identical to one lane summing
+// alone, for either kS, while the term work spreads over N / kS blocks instead
+// of N lanes.  Threads of sinks past the item's end (a disk's last item)
+// compute nothing.
+constexpr int kExactBlock = kBlock + 64;   // 4 term waves + 1 summing wave
 template <uint32_t kS>
-__global__ __launch_bounds__(kBlock) void k_force_exact(const WorkItem* __restrict__ items, uint32_t n_items,
-                                                        const double* __restrict__ x, const double* __restrict__ y,
-                                                        const double* __restrict__ z,
-                                                        const double* __restrict__ m_ex,
-                                                        const DiskCtl* __restrict__ ctl, uint32_t need,
-                                                        double* __restrict__ part, uint32_t n_all)
+__global__ __launch_bounds__(kExactBlock) void k_force_exact(const WorkItem* __restrict__ items, uint32_t n_items,
+                                                             const double* __restrict__ x,
+                                                             const double* __restrict__ y,
+                                                             const double* __restrict__ z,
+                                                             const double* __restrict__ m_ex,
+                                                             const DiskCtl* __restrict__ ctl, uint32_t need,
+                                                             double* __restrict__ part, uint32_t n_all)
 {
-    constexpr uint32_t kLanesPerSink = kBlock / kS;              // threads sharing one sink's terms
-    constexpr uint32_t kT = kExactTile / kLanesPerSink;          // terms per thread per tile
-    static_assert(kS * kExactTile == kT * kBlock && kT >= 1 && 3 * kS <= 64, "tile shape");
-    __shared__ double4 src[kExactTile];
-    __shared__ double term[3][kS][kExactTile + 1];   // +1: sums read with a 65-double stride
+    constexpr uint32_t kLanesPerSink = kBlock / kS;              // term threads sharing one sink
+    // sources per tile: 32 for 16-sink items (two term buffers in 26 KB of
+    // LDS), one per term thread of a sink otherwise
+    constexpr uint32_t kTile = kS >= 16 ? 32u : kLanesPerSink;
+    constexpr uint32_t kT = kTile / kLanesPerSink;               // terms per thread per tile
+    static_assert(kS * kTile == kT * kBlock && kT >= 1 && 3 * kS <= 64 && kTile <= kBlock, "tile shape");
+    __shared__ double4 src[2][kTile];
+    __shared__ double term[2][3][kS][kTile + 1];   // +1: summing lanes read with an odd stride
     const uint32_t tid = threadIdx.x;
-    const uint32_t g = tid / kLanesPerSink;   // sink of this thread's terms
-    const uint32_t k_lo = tid % kLanesPerSink;    // its sources: k_lo + kLanesPerSink q, q < kT
-    const uint32_t sc_c = tid / kS % 3u;      // summing lanes (tid < 3 kS): component
-    const uint32_t sc_g = tid % kS;           //                               sink
+    const bool termer = tid < (uint32_t)kBlock;   // waves 0-3
+    const uint32_t g = tid / kLanesPerSink;   // term threads: sink of this thread's terms
+    const uint32_t k_lo = tid % kLanesPerSink;    //            its sources: k_lo + kLanesPerSink q, q < kT
+    const uint32_t sl = tid - (uint32_t)kBlock;   // wave 4: summing lane (sl < 3 kS)
+    const uint32_t sc_c = sl / kS % 3u;      //         component
+    const uint32_t sc_g = sl % kS;           //         sink
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         const WorkItem w = items[it];
         if (!(ctl[w.disk].flags & need)) continue;
         const uint32_t i = w.sink_lo + g;
-        const bool valid = i < w.sink_hi;
+        const bool valid = termer && i < w.sink_hi;
         const double xi = valid ? x[i] : 0.0, yi = valid ? y[i] : 0.0, zi = valid ? z[i] : 0.0;
         const uint32_t si = w.sink_lo + sc_g;  // sink of a summing lane
+        const bool summer = !termer && sl < 3 * kS && si < w.sink_hi;
+        const uint32_t n_tiles = (w.src_hi - w.src_lo + kTile - 1) / kTile;
+        auto load_src = [&](uint32_t j) {
+            return j < w.src_hi ? make_double4(x[j], y[j], z[j], m_ex[j]) : make_double4(0.0, 0.0, 0.0, -1.0);
+        };
         double acc = 0.0;
-        // the next tile's sources are loaded into registers while the current
-        // tile is worked on, so global-load latency is paid once per item
+        // tile 0 into LDS, tile 1 into registers; the loop stages tile t + 1
+        // and loads tile t + 2 while tile t's terms are computed
         double4 nxt = make_double4(0.0, 0.0, 0.0, -1.0);
-        if (tid < kExactTile && w.src_lo + tid < w.src_hi) {
-            const uint32_t j = w.src_lo + tid;
-            nxt = make_double4(x[j], y[j], z[j], m_ex[j]);
+        if (tid < kTile) {
+            src[0][tid] = load_src(w.src_lo + tid);
+            nxt = load_src(w.src_lo + kTile + tid);
         }
-        for (uint32_t t0 = w.src_lo; t0 < w.src_hi; t0 += kExactTile) {
-            __syncthreads();
-            if (tid < kExactTile) {
-                src[tid] = nxt;
-                const uint32_t j = t0 + kExactTile + tid;
-                nxt = j < w.src_hi ? make_double4(x[j], y[j], z[j], m_ex[j]) : make_double4(0.0, 0.0, 0.0, -1.0);
-            }
-            __syncthreads();
+        __syncthreads();
+        for (uint32_t t = 0; t <= n_tiles; ++t) {
+            const uint32_t b = t & 1u;
+            if (termer) {
+                if (t < n_tiles) {
+                    const uint32_t t0 = w.src_lo + t * kTile;
+                    if (valid) {
 #pragma unroll
-            for (uint32_t q = 0; q < kT; ++q) {
-                const uint32_t k = k_lo + kLanesPerSink * q;
-                const double4 sj = src[k];
-                const double dx = sj.x - xi;
-                const double dy = sj.y - yi;
-                const double dz = sj.z - zi;
-                const double r2 = SQR(dx) + SQR(dy) + SQR(dz);
-                const double d = sqrt(r2);
-                const double sc = sj.w / (d * r2);
-                // a skipped pair (self, inactive or padding source) adds +0:
-                // the running sum starts at +0 and round-to-nearest never
-                // makes it -0, so acc + 0 == acc bit for bit
-                const bool skip = sj.w < 0.0 || t0 + k == i;
-                term[0][g][k] = skip ? 0.0 : sc * dx;
-                term[1][g][k] = skip ? 0.0 : sc * dy;
-                term[2][g][k] = skip ? 0.0 : sc * dz;
+                        for (uint32_t q = 0; q < kT; ++q) {
+                            const uint32_t k = k_lo + kLanesPerSink * q;
+                            const double4 sj = src[b][k];
+                            const double dx = sj.x - xi;
+                            const double dy = sj.y - yi;
+                            const double dz = sj.z - zi;
+                            const double r2 = SQR(dx) + SQR(dy) + SQR(dz);
+                            const double d = sqrt(r2);
+                            const double sc = sj.w / (d * r2);
+                            // a skipped pair (self, inactive or padding source) adds +0:
+                            // the running sum starts at +0 and round-to-nearest never
+                            // makes it -0, so acc + 0 == acc bit for bit
+                            const bool skip = sj.w < 0.0 || t0 + k == i;
+                            term[b][0][g][k] = skip ? 0.0 : sc * dx;
+                            term[b][1][g][k] = skip ? 0.0 : sc * dy;
+                            term[b][2][g][k] = skip ? 0.0 : sc * dz;
+                        }
+                    }
+                    if (tid < kTile) {
+                        src[b ^ 1u][tid] = nxt;
+                        nxt = load_src(w.src_lo + (t + 2) * kTile + tid);
+                    }
+                }
+            } else if (t > 0 && summer) {
+                const double* tr = &term[b ^ 1u][sc_c][sc_g][0];
+#pragma unroll
+                for (uint32_t k = 0; k < kTile; ++k) acc += tr[k];
             }
             __syncthreads();
-            if (tid < 3 * kS) {
-                const double* tr = &term[sc_c][sc_g][0];
-#pragma unroll
-                for (uint32_t k = 0; k < kExactTile; ++k) acc += tr[k];
-            }
         }
-        if (tid < 3 * kS && si < w.sink_hi) part[(size_t)(w.slot * 3 + sc_c) * n_all + si] = acc;
+        if (summer) part[(size_t)(w.slot * 3 + sc_c) * n_all + si] = acc;
     }
 }
 
@@ -616,11 +636,11 @@ hipError_t launch_force_exact(hipStream_t st, int lane, const WorkItem* items, u
         hipLaunchKernelGGL(k_force_exact_lane, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z, m_ex, ctl,
                            need, part, n_all);
     else if (lane == 2)
-        hipLaunchKernelGGL(k_force_exact<kExactSinksSmall>, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z,
-                           m_ex, ctl, need, part, n_all);
+        hipLaunchKernelGGL(k_force_exact<kExactSinksSmall>, dim3(grid), dim3(kExactBlock), 0, st, items, n_items, x,
+                           y, z, m_ex, ctl, need, part, n_all);
     else
-        hipLaunchKernelGGL(k_force_exact<kExactSinks>, dim3(grid), dim3(kBlock), 0, st, items, n_items, x, y, z,
-                           m_ex, ctl, need, part, n_all);
+        hipLaunchKernelGGL(k_force_exact<kExactSinks>, dim3(grid), dim3(kExactBlock), 0, st, items, n_items, x, y,
+                           z, m_ex, ctl, need, part, n_all);
     return hipGetLastError();
 }
 
