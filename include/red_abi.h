@@ -22,8 +22,9 @@
 extern "C" {
 #endif
 
-/* 2: rd_tuning gained sym_chunk_min */
-#define RD_ABI_VERSION 2
+/* 2: rd_tuning gained sym_chunk_min; 3: rd_gas_grid, rd_disk_set_gas_grid, rd_integrator_set_overlap,
+ * rd_mlp_set_target */
+#define RD_ABI_VERSION 3
 
 /* Status codes.  The reference throws std::string (red.type/red_macro.h:37-95,
  * src/red.cuda/pp_disk.cu:688, 974-982); here each throw site maps to a code. */

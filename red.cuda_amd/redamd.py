@@ -112,7 +112,7 @@ for _name, (_res, _args) in _SIGS.items():
     _f.argtypes = _args
 
 EXPORTED = sorted(_SIGS)
-ABI_VERSION = 2   # include/red_abi.h RD_ABI_VERSION: the structs below follow that header
+ABI_VERSION = 3   # include/red_abi.h RD_ABI_VERSION: the structs below follow that header
 if lib.rd_version() != ABI_VERSION:
     raise ImportError(f"{LIB_PATH} has ABI version {lib.rd_version()}, this binding expects {ABI_VERSION}: rebuild it")
 

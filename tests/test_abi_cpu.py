@@ -31,7 +31,7 @@ def test_every_declared_symbol_is_exported():
     assert not missing, missing
     for n in names:
         getattr(R.lib, n)
-    assert R.lib.rd_version() == R.ABI_VERSION == 2
+    assert R.lib.rd_version() == R.ABI_VERSION == 3
 
 
 def test_binding_covers_the_header():
