@@ -214,3 +214,34 @@ def test_disk_uses_the_grid_in_place_of_the_analytic_gas():
     d.set_gas_grid(None, None, None, None)
     _, back = d.calc_dydx(1, 0, 0.0, r, v)
     assert np.array_equal(as3(back), as3(anal))
+
+
+def test_bench_grid_reproduces_the_analytic_disk():
+    """bench.py --config 3 --gas grid drags through a frame sampled from the
+    analytic disk (grid_from_analytic): at the cell centres, in the midplane
+    and outside the 0.1 AU taper, its drag equals the analytic disk's."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    import redamd as R
+    g = R.gas_params()
+    n_rad, n_sec, r_min, r_max = bench.C3_GRID
+    grid = bench.grid_from_analytic(g, *bench.C3_GRID)
+    ur = grid[0]
+    assert ur.size == n_rad + 1 and grid[1].shape == (n_rad, n_sec)
+    n = 300
+    rng = np.random.default_rng(4)
+    k, s = rng.integers(0, n_rad, n), rng.integers(0, n_sec, n)
+    rc, th = np.sqrt(ur[k] * ur[k + 1]), (s + 0.5) * TWO_PI / n_sec
+    r, v, p, b, nbt = small_system(n + 1, 5)
+    r["x"][1:], r["y"][1:], r["z"] = rc * np.cos(th), rc * np.sin(th), 0.0
+    v["x"][1:], v["y"][1:] = -np.sin(th) * np.sqrt(1 / rc) * 1.01, np.cos(th) * np.sqrt(1 / rc) * 1.01
+    p["radius"], p["density"] = 1e-8, 3.4e6
+    a_grid, a_anal = np.zeros(n + 1, VAR4), np.zeros(n + 1, VAR4)
+    og = O.OGrid(*grid)
+    O.lib.or_grid_drag(C.byref(og.g), 1, O.ptr(nbt), O.ptr(p), O.ptr(r), O.ptr(v), O.ptr(a_grid))
+    O.lib.or_drag(0.0, C.byref(g), 1, n + 1, O.ptr(nbt), O.ptr(p), O.ptr(r), O.ptr(v), O.ptr(a_anal))
+    ag, aa = as3(a_grid)[1:], as3(a_anal)[1:]
+    assert np.all(np.abs(aa).max(axis=1) > 0)
+    assert np.all(np.abs(ag - aa) <= 1e-12 * np.abs(aa).max(axis=1, keepdims=True))
