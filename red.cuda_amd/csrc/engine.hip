@@ -199,7 +199,7 @@ struct rd_disk {
     double thr[3] = {0, 0, 0};
     GasDev gas{};
     int32_t gas_created = kGasNone;  // the model given at creation (a grid's removal returns to it)
-    // FARGO grid on the device: ring radii, then density, v_rad, v_theta
+    // FARGO grid on the device: cell records (density, v_rad, v_theta, 0), then ring radii
     double* grid = nullptr;
     uint64_t generation = 0;       // bumps when buffers / lists change (invalidates graphs)
 };
@@ -936,21 +936,26 @@ int rd_disk_set_gas_grid(rd_disk* d, const rd_gas_grid* g)
     // kernels queued on the disk's stream may still read the old grid
     HIP_TRY(hipStreamSynchronize(d->stream));
     dev_free(d->grid);
-    d->gas.g_rad = d->gas.g_dens = d->gas.g_vrad = d->gas.g_vtheta = nullptr;
+    d->gas.g_rad = nullptr;
+    d->gas.g_cell = nullptr;
     d->gas.n_rad = d->gas.n_sec = 0;
     d->gas.on = d->gas_created;
     d->generation++;
     if (!g) return RD_OK;
+    // one block: the cell records (32 B each, 32-byte aligned), then the radii
     const size_t nr = (size_t)g->n_rad + 1, nc = (size_t)g->n_rad * g->n_sec;
-    if (int rc = dev_alloc(&d->grid, nr + 3 * nc)) return rc;
-    HIP_TRY(hipMemcpy(d->grid, g->used_rad, nr * sizeof(double), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d->grid + nr, g->density, nc * sizeof(double), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d->grid + nr + nc, g->vrad, nc * sizeof(double), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d->grid + nr + 2 * nc, g->vtheta, nc * sizeof(double), hipMemcpyHostToDevice));
-    d->gas.g_rad = d->grid;
-    d->gas.g_dens = d->grid + nr;
-    d->gas.g_vrad = d->grid + nr + nc;
-    d->gas.g_vtheta = d->grid + nr + 2 * nc;
+    std::vector<double> h(4 * nc + nr);
+    for (size_t c = 0; c < nc; ++c) {
+        h[4 * c + 0] = g->density[c];
+        h[4 * c + 1] = g->vrad[c];
+        h[4 * c + 2] = g->vtheta[c];
+        h[4 * c + 3] = 0.0;
+    }
+    std::memcpy(h.data() + 4 * nc, g->used_rad, nr * sizeof(double));
+    if (int rc = dev_alloc(&d->grid, h.size())) return rc;
+    HIP_TRY(hipMemcpy(d->grid, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    d->gas.g_cell = reinterpret_cast<const double4*>(d->grid);
+    d->gas.g_rad = d->grid + 4 * nc;
     d->gas.n_rad = g->n_rad;
     d->gas.n_sec = g->n_sec;
     d->gas.on = kGasGrid;

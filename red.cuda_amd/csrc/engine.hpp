@@ -10,6 +10,8 @@
 //                    kernel (one slot), FP32 chunk totals in mixed precision
 //   sources        : m_ex (FP64 mass, -1 if inactive), m32 (FP32 mass, 0 if
 //                    inactive or summed in FP64)
+//   gas grid       : FARGO cells as 32-byte records (density, v_rad, v_theta, 0),
+//                    ring-major, then the n_rad + 1 ring radii
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -85,9 +87,9 @@ struct GasDev {
     int32_t on;                  // kGasNone, kGasAnalytic or kGasGrid
     // grid: n_rad rings x n_sec sectors, cell (ring, sector) at ring * n_sec + sector
     const double* g_rad;         // n_rad + 1 ring interface radii, increasing
-    const double* g_dens;        // volume density
-    const double* g_vrad;        // radial velocity
-    const double* g_vtheta;      // azimuthal velocity on top of the circular one
+    // per cell (density, radial velocity, azimuthal velocity on top of the
+    // circular one, 0): one 32-byte record, so a lookup reads one segment
+    const double4* g_cell;
     uint32_t n_rad, n_sec;
 };
 

@@ -766,14 +766,14 @@ __device__ double grid_drag_component(const GasDev& g, double m_star, double rx,
     const double r = sqrt(SQR(rx) + SQR(ry));
     const double at = atan2(ry, rx);
     const double theta = ry >= 0.0 ? at : kTwoPi + at;
-    const uint32_t c = grid_cell(g, r, theta);
+    const double4 cell = g.g_cell[grid_cell(g, r, theta)];
     const double vc_theta = sqrt(m_star / r);
-    const double vg_theta = vc_theta + g.g_vtheta[c];
-    const double vg_rad = g.g_vrad[c];
+    const double vg_theta = vc_theta + cell.z;
+    const double vg_rad = cell.y;
     const double ct = cos(theta), st = sin(theta);
     const double gx = ct * vg_rad - st * vg_theta;
     const double gy = st * vg_rad + ct * vg_theta;
-    const double rho = g.g_dens[c];
+    const double rho = cell.x;
     const double ux = gx - vx, uy = gy - vy, uz = 0.0 - vz;
     const double u_n = sqrt(SQR(ux) + SQR(uy) + SQR(uz));
     const double f = (3.0 * cd * rho * u_n) / (8.0 * radius * density);
