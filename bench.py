@@ -379,7 +379,10 @@ def bench_config3(args):
                pair_kernel("mixed", tuning), f"{pair_kernel('mixed', tuning)}_disk131072_gas_rkf78.txt", tuning)
                if n == 131072 and grid is None else None),
            # the drag reads R, rho_p and C_D per body and stage, the grid also its cell's three fields
-           "roofline_stage": roofline_stage(d, n + 1, drag_bytes=13 * (24 + (24 if grid is not None else 0)))}
+           "roofline_stage": roofline_stage(d, n + 1, drag_bytes=13 * (24 + (24 if grid is not None else 0)),
+                                            traffic=profiled_stage_traffic(
+                                                "k_stage_rk8_disk131072_gridgas_rkf78.txt" if grid is not None
+                                                else "k_force_sym_disk131072_gas_rkf78.txt") if n == 131072 else None)}
     if not args.no_cpu:
         print("# cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_force_baseline(r, v, p, b, nbt, gas=g if grid is None else None, drag_mode=1,
