@@ -23,7 +23,7 @@ extern "C" {
 #endif
 
 /* 2: rd_tuning gained sym_chunk_min; 3: rd_gas_grid, rd_disk_set_gas_grid, rd_integrator_set_overlap,
- * rd_mlp_set_target */
+ * rd_mlp_set_target, rd_calc_dydx_device */
 #define RD_ABI_VERSION 3
 
 /* Status codes.  The reference throws std::string (red.type/red_macro.h:37-95,
@@ -185,6 +185,14 @@ int rd_sync(rd_disk* d);
  * Copies in and out over PCIe; the integrators never go through it. */
 int rd_calc_dydx(rd_disk* d, uint32_t k, uint32_t i, uint32_t rr, double t,
                  const rd_var4* r, const rd_var4* v, rd_var4* dy);
+/* The same on device arrays (the reference's calc_dydx works on the GPU copies
+ * it owns, pp_disk.cu:1450-1502): r, v and dy point to device memory of the
+ * disk's device, each holding disk k's bodies as rd_var4 records (w of dy set
+ * to 0).  Pointers that are not device memory of that device, or whose
+ * allocation ends before the last record, are refused (RD_EINVAL) before any
+ * kernel runs.  Returns when dy is written. */
+int rd_calc_dydx_device(rd_disk* d, uint32_t k, uint32_t i, uint32_t rr, double t,
+                        const rd_var4* r, const rd_var4* v, rd_var4* dy);
 /* Time n_iter launches of the pair kernel on the current state of all disks
  * (pp_disk::benchmark_calc_grav_accel, pp_disk.cu:613-632).  Returns the mean
  * kernel time in ms (HIP events on the engine's stream) and the ordered

@@ -1079,6 +1079,61 @@ int rd_calc_dydx(rd_disk* d, uint32_t k, uint32_t i, uint32_t rr, double t, cons
     return RD_OK;
 }
 
+// p holds n rd_var4 records in device memory of the disk's device
+static int check_device_var4(const rd_disk* d, const void* p, uint32_t n, const char* what)
+{
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();   // leave no error for the next call to report
+        return fail(RD_EINVAL, std::string(what) + " is not device memory");
+    }
+    if ((at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged) || at.device != d->device)
+        return fail(RD_EINVAL, std::string(what) + " is not device memory of the disk's device");
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<void*>(p)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(RD_EINVAL, std::string(what) + ": its allocation could not be found");
+    }
+    if ((const char*)p + (size_t)n * sizeof(rd_var4) > (const char*)base + size)
+        return fail(RD_EINVAL, std::string(what) + " holds fewer records than the disk has bodies");
+    return RD_OK;
+}
+
+int rd_calc_dydx_device(rd_disk* d, uint32_t k, uint32_t i, uint32_t rr, double t, const rd_var4* r,
+                        const rd_var4* v, rd_var4* dy)
+{
+    DeviceScope dev_scope(d ? d->device : -1);
+    (void)rr;
+    if (!d || k >= d->n_disk || !r || !v || !dy) return fail(RD_EINVAL, "bad argument");
+    const HostDisk& h = d->hd[k];
+    if (i != 0 && i != 1) return fail(RD_EINVAL, "Parameter 'i' is out of range.");
+    if (int rc = check_device_var4(d, v, h.n, "v")) return rc;
+    if (int rc = check_device_var4(d, dy, h.n, "dy")) return rc;
+    if (i == 0) {
+        HIP_TRY(hipMemcpyAsync(dy, v, h.n * sizeof(rd_var4), hipMemcpyDeviceToDevice, d->stream));
+        HIP_TRY(hipStreamSynchronize(d->stream));
+        return RD_OK;
+    }
+    if (int rc = check_device_var4(d, r, h.n, "r")) return rc;
+    if (int rc = ensure_lists(d)) return rc;
+    // positions into P[0], velocities into the stage-1 velocity slot, as rd_calc_dydx
+    double* const p[3] = {d->P[0][0] + h.off, d->P[0][1] + h.off, d->P[0][2] + h.off};
+    double* const vel[3] = {kv_ptr(d, 1, 0) + h.off, kv_ptr(d, 1, 1) + h.off, kv_ptr(d, 1, 2) + h.off};
+    HIP_TRY(launch_var4_split(d->stream, h.n, reinterpret_cast<const double*>(r), reinterpret_cast<const double*>(v),
+                              p, vel));
+    for (uint32_t q = 0; q < d->n_disk; ++q) d->h_ctl[q] = DiskCtl{d->hd[q].t, 0.0, q == k ? F_NEED_K0 : 0u, 0};
+    HIP_TRY(hipMemcpyAsync(d->ctl, d->h_ctl, d->n_disk * sizeof(DiskCtl), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(enqueue_force(d, d->P[0][0], d->P[0][1], d->P[0][2], F_NEED_K0));
+    StageArgs a = make_args(d, 2, 0, 0);   // fx = P[0]
+    HIP_TRY(launch_operator(d->stream, a, t, k, ka_ptr(d, 0, 0), ka_ptr(d, 0, 1), ka_ptr(d, 0, 2), kv_ptr(d, 1, 0),
+                            kv_ptr(d, 1, 1), kv_ptr(d, 1, 2)));
+    const double* const acc[3] = {ka_ptr(d, 0, 0) + h.off, ka_ptr(d, 0, 1) + h.off, ka_ptr(d, 0, 2) + h.off};
+    HIP_TRY(launch_var4_join(d->stream, h.n, acc, reinterpret_cast<double*>(dy)));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    return RD_OK;
+}
+
 int rd_bench_force(rd_disk* d, uint32_t n_iter, double* ms, double* pairs)
 {
     DeviceScope dev_scope(d ? d->device : -1);

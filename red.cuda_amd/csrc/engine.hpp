@@ -156,6 +156,10 @@ hipError_t launch_decide(hipStream_t s, const unsigned long long* errw, uint32_t
                          uint32_t n_disk, DiskCtl* ctl, double lambda, double tol, int nonfinite_guard);
 hipError_t launch_operator(hipStream_t s, const StageArgs& a, double t, uint32_t disk, double* ax, double* ay,
                            double* az, const double* vx, const double* vy, const double* vz);
+// var4 records (x, y, z, w) <-> component arrays, for calc_dydx on device arrays
+hipError_t launch_var4_split(hipStream_t s, uint32_t n, const double* r, const double* v, double* const p[3],
+                             double* const vel[3]);
+hipError_t launch_var4_join(hipStream_t s, uint32_t n, const double* const a[3], double* dy);
 hipError_t launch_collision_grid(hipStream_t s, uint32_t n_all, const double* x, const double* y, const double* z,
                                  const double* radius, const int32_t* id, const uint32_t* disk_of,
                                  const DiskCtl* ctl, double f, uint32_t table, uint32_t* head, uint32_t* next,

@@ -1562,6 +1562,55 @@ hipError_t launch_operator(hipStream_t st, const StageArgs& a, double t, uint32_
     return hipGetLastError();
 }
 
+// calc_dydx on device arrays: the caller's var4 records (x, y, z, w; 32 B)
+// split into the engine's component arrays, and the accelerations joined back
+// into var4 records with w = 0.  Scalar loads and stores: a caller's record
+// array need only be 8-byte aligned.
+__global__ __launch_bounds__(kBodyBlock) void k_var4_split(uint32_t n, const double* __restrict__ r,
+                                                           const double* __restrict__ v, double* __restrict__ px,
+                                                           double* __restrict__ py, double* __restrict__ pz,
+                                                           double* __restrict__ vx, double* __restrict__ vy,
+                                                           double* __restrict__ vz)
+{
+    const uint32_t q = blockIdx.x * kBodyBlock + threadIdx.x;
+    if (q >= n) return;
+    px[q] = r[4 * (size_t)q + 0];
+    py[q] = r[4 * (size_t)q + 1];
+    pz[q] = r[4 * (size_t)q + 2];
+    vx[q] = v[4 * (size_t)q + 0];
+    vy[q] = v[4 * (size_t)q + 1];
+    vz[q] = v[4 * (size_t)q + 2];
+}
+
+__global__ __launch_bounds__(kBodyBlock) void k_var4_join(uint32_t n, const double* __restrict__ ax,
+                                                          const double* __restrict__ ay,
+                                                          const double* __restrict__ az, double* __restrict__ dy)
+{
+    const uint32_t q = blockIdx.x * kBodyBlock + threadIdx.x;
+    if (q >= n) return;
+    dy[4 * (size_t)q + 0] = ax[q];
+    dy[4 * (size_t)q + 1] = ay[q];
+    dy[4 * (size_t)q + 2] = az[q];
+    dy[4 * (size_t)q + 3] = 0.0;
+}
+
+hipError_t launch_var4_split(hipStream_t st, uint32_t n, const double* r, const double* v, double* const p[3],
+                             double* const vel[3])
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_var4_split, dim3((n + kBodyBlock - 1) / kBodyBlock), dim3(kBodyBlock), 0, st, n, r, v, p[0],
+                       p[1], p[2], vel[0], vel[1], vel[2]);
+    return hipGetLastError();
+}
+
+hipError_t launch_var4_join(hipStream_t st, uint32_t n, const double* const a[3], double* dy)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_var4_join, dim3((n + kBodyBlock - 1) / kBodyBlock), dim3(kBodyBlock), 0, st, n, a[0], a[1],
+                       a[2], dy);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Sources: FP64 mass with -1 for inactive bodies, FP32 mass with 0 for
 // inactive bodies and for the FP64 prefix.

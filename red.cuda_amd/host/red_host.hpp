@@ -96,6 +96,12 @@ class pp_disk {
     {
         rd_check(rd_calc_dydx(h_, 0, i, rr, curr_t, r, v, dy), "calc_dydx");
     }
+    // ... on device arrays of the disk's device (no PCIe copies)
+    void calc_dydx_device(uint32_t i, uint32_t rr, double curr_t, const rd_var4* r_dev, const rd_var4* v_dev,
+                          rd_var4* dy_dev)
+    {
+        rd_check(rd_calc_dydx_device(h_, 0, i, rr, curr_t, r_dev, v_dev, dy_dev), "calc_dydx_device");
+    }
     // event hooks (pp_disk.h:83-102)
     bool check_for_ejection_hit_centrum()
     {

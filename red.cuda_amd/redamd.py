@@ -53,6 +53,7 @@ _SIGS = {
     "rd_disk_set_precision": (C.c_int, [_P, C.c_int]),
     "rd_disk_set_drag_mode": (C.c_int, [_P, C.c_int]),
     "rd_disk_set_gas_grid": (C.c_int, [_P, _P]),
+    "rd_calc_dydx_device": (C.c_int, [_P, _u32, _u32, _u32, _f64, _P, _P, _P]),
     "rd_disk_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "rd_disk_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "rd_autotune": (C.c_int, [_P, C.POINTER(Tuning)]),
@@ -307,6 +308,12 @@ class Disk:
         dy = np.zeros(len(r), VAR4)
         check(lib.rd_calc_dydx(self.h, k, i, rr, t, _ptr(r), _ptr(v), _ptr(dy)))
         return dy
+
+    def calc_dydx_device(self, i, rr, t, r_dev, v_dev, dy_dev, k=0):
+        """rd_calc_dydx_device: r_dev, v_dev and dy_dev are device addresses
+        (integers) of rd_var4 record arrays on the disk's device."""
+        check(lib.rd_calc_dydx_device(self.h, k, i, rr, t, C.c_void_p(r_dev), C.c_void_p(v_dev),
+                                      C.c_void_p(dy_dev)))
 
     def bench_force(self, n_iter=10):
         ms, pairs = _f64(), _f64()
