@@ -709,6 +709,27 @@ hipError_t enqueue_attempt(rd_integrator* it, int commit_advance)
 {
     rd_disk* d = it->d;
     hipError_t e;
+    // FP64, one small gas-free disk (4-sink items: launches, not pairs, set
+    // the time), fixed-step RK4: each stage is done by the summing lanes of
+    // its FP64 pair pass (launch_force_exact_rk4), which hold every body's
+    // complete sum; the same operations in the same order as pair pass + stage
+    // kernel, one launch per stage instead of two
+    if (d->precision == RD_PREC_FP64 && !d->exact_lane && d->exact_small && d->n_disk == 1 &&
+        d->gas.on == kGasNone && it->method == RD_RK4 && !it->adaptive) {
+        for (int r = 0; r <= last_stage(it); ++r) {
+            StageArgs a = make_args(d, r, it->adaptive, it->err_all);
+            a.fuse_comb1 = r == 0;   // stage 0 forms stage 1 itself, as in the separate kernel
+            if ((e = launch_force_exact_rk4(d->stream, r, d->items_force, d->n_items_force, d->grid_force,
+                                            a.fpos[0], a.fpos[1], a.fpos[2], d->m_ex, d->ctl,
+                                            r == 0 ? F_K0_FORCE : F_ACTIVE, a)) != hipSuccess)
+                return e;
+        }
+        if (commit_advance >= 0) {
+            StageArgs a = make_args(d, 0, it->adaptive, it->err_all);
+            if ((e = launch_commit(d->stream, a, commit_advance, d->ctl)) != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
     {
         StageArgs a = make_args(d, 0, it->adaptive, it->err_all);
         // fixed-step RK4 never retries with k0 kept, so stage 0 forms stage 1 itself

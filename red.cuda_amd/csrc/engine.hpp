@@ -135,6 +135,12 @@ struct BodyArrays {
 hipError_t launch_force_exact(hipStream_t s, int lane, const WorkItem* items, uint32_t n_items, uint32_t grid,
                               const double* x, const double* y, const double* z, const double* m_ex,
                               const DiskCtl* ctl, uint32_t need, double* part, uint32_t n_all);
+// The FP64 pair pass (4-sink items) of single-disk, gas-free, fixed-step RK4
+// stage r (0-3) with the stage combination done by the kernel's summing lanes
+// (no stage kernel, no partial-sum slot).
+hipError_t launch_force_exact_rk4(hipStream_t s, int r, const WorkItem* items, uint32_t n_items, uint32_t grid,
+                                  const double* x, const double* y, const double* z, const double* m_ex,
+                                  const DiskCtl* ctl, uint32_t need, const StageArgs& a);
 hipError_t launch_force_mixed(hipStream_t s, int sinks_per_lane, const WorkItem* items, uint32_t n_items,
                               uint32_t grid, const double* x, const double* y, const double* z,
                               const float* m32, const DiskCtl* ctl, uint32_t need_mask, float* part,

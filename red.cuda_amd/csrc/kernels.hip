@@ -91,14 +91,25 @@ __host__ __device__ constexpr double rk4_c(int r) { return (r == 1 || r == 2) ? 
 // of N lanes.  Threads of sinks past the item's end (a disk's last item)
 // compute nothing.
 constexpr int kExactBlock = kBlock + 64;   // 4 term waves + 1 summing wave
-template <uint32_t kS>
-__global__ __launch_bounds__(kExactBlock) void k_force_exact(const WorkItem* __restrict__ items, uint32_t n_items,
-                                                             const double* __restrict__ x,
-                                                             const double* __restrict__ y,
-                                                             const double* __restrict__ z,
-                                                             const double* __restrict__ m_ex,
-                                                             const DiskCtl* __restrict__ ctl, uint32_t need,
-                                                             double* __restrict__ part, uint32_t n_all)
+
+// Single-disk classical RK4 stage R for body i, component q (defined with the
+// stage kernels below).
+template <int R, bool kFused>
+__device__ void stage_rk4_body(const StageArgs& a, uint32_t i, int q, bool in, double ps_fused);
+struct NoStage {};
+
+// FR < 0: each summing lane stores its sum in the partial-sum slot.  FR >= 0
+// (a single gas-free disk, fixed-step RK4, FP64): the lane holds body si's complete
+// component-sc_c sum -- the value the stage kernel would read back from that
+// slot -- and runs stage FR for the body itself: it reads only that body's
+// state and k rows and writes the other position buffer, which no pair term
+// of this launch reads.
+template <uint32_t kS, int FR, typename SA>
+__device__ __forceinline__ void force_exact_body(const WorkItem* __restrict__ items, uint32_t n_items,
+                                                 const double* __restrict__ x, const double* __restrict__ y,
+                                                 const double* __restrict__ z, const double* __restrict__ m_ex,
+                                                 const DiskCtl* __restrict__ ctl, uint32_t need,
+                                                 double* __restrict__ part, uint32_t n_all, const SA& sa)
 {
     constexpr uint32_t kLanesPerSink = kBlock / kS;              // term threads sharing one sink
     // sources per tile: 32 for 16-sink items (two term buffers in 26 KB of
@@ -173,8 +184,37 @@ __global__ __launch_bounds__(kExactBlock) void k_force_exact(const WorkItem* __r
             }
             __syncthreads();
         }
-        if (summer) part[(size_t)(w.slot * 3 + sc_c) * n_all + si] = acc;
+        if constexpr (FR >= 0) {
+            if (summer) stage_rk4_body<FR, true>(sa, si, (int)sc_c, true, acc);
+        } else {
+            if (summer) part[(size_t)(w.slot * 3 + sc_c) * n_all + si] = acc;
+        }
     }
+}
+
+template <uint32_t kS>
+__global__ __launch_bounds__(kExactBlock) void k_force_exact(const WorkItem* __restrict__ items, uint32_t n_items,
+                                                             const double* __restrict__ x,
+                                                             const double* __restrict__ y,
+                                                             const double* __restrict__ z,
+                                                             const double* __restrict__ m_ex,
+                                                             const DiskCtl* __restrict__ ctl, uint32_t need,
+                                                             double* __restrict__ part, uint32_t n_all)
+{
+    force_exact_body<kS, -1>(items, n_items, x, y, z, m_ex, ctl, need, part, n_all, NoStage{});
+}
+
+// The FP64 pair pass of RK4 stage R with the stage itself done by the summing lanes.
+template <uint32_t kS, int R>
+__global__ __launch_bounds__(kExactBlock) void k_force_exact_rk4(const WorkItem* __restrict__ items,
+                                                                 uint32_t n_items, const double* __restrict__ x,
+                                                                 const double* __restrict__ y,
+                                                                 const double* __restrict__ z,
+                                                                 const double* __restrict__ m_ex,
+                                                                 const DiskCtl* __restrict__ ctl, uint32_t need,
+                                                                 StageArgs sa)
+{
+    force_exact_body<kS, R>(items, n_items, x, y, z, m_ex, ctl, need, nullptr, sa.n_all, sa);
 }
 
 // The same sum with one sink per lane, each lane adding its terms in j order
@@ -641,6 +681,25 @@ hipError_t launch_force_exact(hipStream_t st, int lane, const WorkItem* items, u
     else
         hipLaunchKernelGGL(k_force_exact<kExactSinks>, dim3(grid), dim3(kExactBlock), 0, st, items, n_items, x, y,
                            z, m_ex, ctl, need, part, n_all);
+    return hipGetLastError();
+}
+
+hipError_t launch_force_exact_rk4(hipStream_t st, int r, const WorkItem* items, uint32_t n_items, uint32_t grid,
+                                  const double* x, const double* y, const double* z, const double* m_ex,
+                                  const DiskCtl* ctl, uint32_t need, const StageArgs& a)
+{
+    if (n_items == 0) return hipSuccess;
+#define EXACT_RK4(R)                                                                                            \
+    hipLaunchKernelGGL((k_force_exact_rk4<kExactSinksSmall, R>), dim3(grid), dim3(kExactBlock), 0, st, items,   \
+                       n_items, x, y, z, m_ex, ctl, need, a)
+    switch (r) {
+    case 0: EXACT_RK4(0); break;
+    case 1: EXACT_RK4(1); break;
+    case 2: EXACT_RK4(2); break;
+    case 3: EXACT_RK4(3); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef EXACT_RK4
     return hipGetLastError();
 }
 
@@ -1201,14 +1260,17 @@ __global__ __launch_bounds__(kBodyBlock) void k_stage_rk8(StageArgs a)
     if constexpr (R == 12) error_reduce(a, valid, d, e);
 }
 
-// Classical RK4 stage R over a single disk.
-template <int R>
-__device__ __forceinline__ void stage_rk4_one(const StageArgs& a)
+// Classical RK4 stage R over a single disk, for body i and component q.
+// kFused: called by the FP64 pair kernel's summing lane with the body's
+// complete FP64 sum, for a body of a gas-free disk (fixed step: no stage 4),
+// and returns early where the stage does not apply; stores go to the arrays.
+// Otherwise a stage-kernel thread: the sums come from the partial-sum slots,
+// and a thread past the end or outside the attempt stores into its discard
+// slot, so no branch sits between the loads and the stores.
+template <int R, bool kFused>
+__device__ __forceinline__ void stage_rk4_body(const StageArgs& a, uint32_t i, int q, bool in, double ps_fused)
 {
-    const uint32_t i0 = blockIdx.x * kBodyBlock + threadIdx.x;
-    const bool in = i0 < a.n_all;
-    const uint32_t i = in ? i0 : a.n_all - 1;
-    const int q = blockIdx.y;
+    static_assert(!kFused || R <= 3, "fused stages are the fixed-step ones");
     const DiskInfo& di = a.info0;
     const DiskCtl c = a.ctl[0];
     if constexpr (R == 0)
@@ -1225,14 +1287,33 @@ __device__ __forceinline__ void stage_rk4_one(const StageArgs& a)
         k3a = ka_of(a, 3, q)[i];
     }
     double ps, b;
-    one_disk_sums(a, i, q, ps, b);
+    if constexpr (kFused) {
+        ps = ps_fused;   // the FP64 kernel's one slot: no prefix in FP64 mode
+        b = 0.0;
+    } else {
+        one_disk_sums(a, i, q, ps, b);
+    }
     const bool valid = in && (c.flags & (R == 0 ? F_NEED_K0 : F_ACTIVE)) != 0;
+    if constexpr (kFused)
+        if (!valid) return;
     const double h = c.h;
     const double t = c.t + rk4_c(R) * h;
     const double* vR[3] = {kv_of(a, R, 0), kv_of(a, R, 1), kv_of(a, R, 2)};
-    const double kRa = stage_accel(a, i, q, di, t, b, ps, a.fpos, vR);
-    double* const trash = a.trash + (size_t)q * kBodyBlock + threadIdx.x;
-    auto at = [&](double* p) { return valid ? p + i : trash; };
+    // the fused form runs only without gas (its drag code would take the
+    // registers the pair pass needs): the reference's sum, zero for an
+    // inactive sink (stage_accel without the drag)
+    double kRa;
+    if constexpr (kFused) {
+        (void)vR;
+        (void)t;
+        kRa = a.id[i] > 0 ? ps : 0.0;
+    } else {
+        kRa = stage_accel(a, i, q, di, t, b, ps, a.fpos, vR);
+    }
+    auto at = [&](double* p) -> double* {
+        if constexpr (kFused) return p + i;
+        else return valid ? p + i : a.trash + (size_t)q * kBodyBlock + threadIdx.x;
+    };
     *at(ka_w(a, R, q)) = kRa;
     if constexpr (R == 0) {
         if (a.fuse_comb1) {   // k_comb1's stage-1 combination, same operands
@@ -1261,6 +1342,14 @@ __device__ __forceinline__ void stage_rk4_one(const StageArgs& a)
         if (valid && (a.err_all || li < di.err_hi)) e = fmax(fabs(k3v - kk), fabs(k3a - kRa));   // |f4 - f5|
         error_store_wave(a, valid, e);
     }
+}
+
+template <int R>
+__device__ __forceinline__ void stage_rk4_one(const StageArgs& a)
+{
+    const uint32_t i0 = blockIdx.x * kBodyBlock + threadIdx.x;
+    const bool in = i0 < a.n_all;
+    stage_rk4_body<R, false>(a, in ? i0 : a.n_all - 1, (int)blockIdx.y, in, 0.0);
 }
 
 // Classical RK4 stage kernel (int_rungekutta4.cu:37-73, 130-247).

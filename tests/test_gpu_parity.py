@@ -116,6 +116,37 @@ def test_rk4_fixed_bitexact():
     assert t == ot and bits_equal(rr, orr) and bits_equal(vv, ovv)
 
 
+@pytest.mark.parametrize("case", ["classes", "gas", "graph", "disk5000"])
+def test_rk4_fixed_paths_bitexact(case):
+    """Fixed-step RK4 in FP64 takes one launch per stage on a small gas-free
+    disk (the FP64 pair kernel's summing lanes do the stage) and pair pass +
+    stage kernel otherwise (gas, 16-sink items above ~4096 bodies): both are
+    the oracle's steps bit for bit, with every body class and inactive bodies,
+    direct launches and graph replays."""
+    if case == "disk5000":
+        r, v, p, b, nbt = R.make_disk(4999, 15)
+    else:
+        r, v, p, b, nbt = small_system(600, 61, mixed_types(600), inactive=(4, 77, 599))
+    gas = R.gas_params() if case == "gas" else None
+    d, o = pair(r, v, p, b, nbt, gas=gas)
+    if case == "graph":
+        d.set_tuning(use_graph=1)
+    h = 0.05 * KDAY
+    it = R.Integrator(d, R.RD_RK4, h, adaptive=False)
+    oi = O.OInteg(o, 2, h, adaptive=False)
+    n = 3 if case == "disk5000" else 12
+    it.steps(n)
+    for _ in range(n):
+        oi.step()
+    t, rr, vv, _, _ = d.state()
+    ot, orr, ovv, _, _ = o.state()
+    assert t == ot and bits_equal(rr, orr) and bits_equal(vv, ovv)
+    assert it.step() == oi.step()
+    t, rr, vv, _, _ = d.state()
+    ot, orr, ovv, _, _ = o.state()
+    assert t == ot and bits_equal(rr, orr) and bits_equal(vv, ovv)
+
+
 @pytest.mark.parametrize("method", [R.RD_RKF78, R.RD_RK4])
 def test_adaptive_bitexact_with_rejections(method):
     r, v, p, b, nbt = R.make_disk(64, 2, a_min=0.8, a_max=1.2)
