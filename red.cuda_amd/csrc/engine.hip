@@ -158,6 +158,7 @@ struct rd_disk {
     uint32_t n_items_force = 0, grid_force = 0;
     WorkItem* items_sym = nullptr;   // mixed precision, symmetric SI x SI pairs
     uint32_t n_items_sym = 0, grid_sym = 0, sym_R = 1;
+    bool sym_self = false;           // diagonal tiles with reactions (launch_force_sym)
     // collision grid: bucket heads, per-body list links and cells, largest radius
     uint32_t* coll_head = nullptr;
     uint32_t* coll_next = nullptr;
@@ -540,13 +541,15 @@ int build_lists(rd_disk* d)
                     const uint32_t R = std::min<uint32_t>(kSymMaxR, std::max<uint32_t>(rmin, nblk / kSymChunks));
                     const uint32_t nc = (nblk + R - 1) / R, span = R * kSymBlockBodies;
                     d->sym_R = std::max(d->sym_R, R);
+                    // a decision of the disk alone: results never depend on the batch
+                    const uint32_t self = nblk <= kSymSelfMaxBlocks ? kItemSelf : 0u;
                     for (uint32_t a = 0; a < nc; ++a)
                         for (uint32_t c = a; c < nc; ++c) {
                             const uint32_t ia = std::min(nblk - a * R, R), ic = std::min(nblk - c * R, R);
                             const uint32_t tiles = a == c ? ia * (ia + 1) / 2 : ia * ic;
                             sym_items.push_back({tiles, WorkItem{k, h.off + a * span, h.off + std::min(b[1], (a + 1) * span),
                                                                  h.off + c * span, h.off + std::min(b[1], (c + 1) * span),
-                                                                 c, a, 0}});
+                                                                 c, a, a == c ? self : 0u}});
                         }
                     base = nc;
                     lo = std::max(lo, b[1]);
@@ -592,6 +595,8 @@ int build_lists(rd_disk* d)
     }
     // largest items first; ties keep the disk order
     std::stable_sort(sym_items.begin(), sym_items.end(), [](auto& a, auto& b) { return a.first > b.first; });
+    d->sym_self = false;
+    for (auto& q : sym_items) d->sym_self |= (q.second.flags & kItemSelf) != 0;
     std::vector<WorkItem> symv;
     for (auto& q : sym_items) symv.push_back(q.second);
     dev_free(d->items_force);
@@ -683,8 +688,8 @@ hipError_t enqueue_force(rd_disk* d, const double* fx, const double* fy, const d
                                   fx, fy, fz, d->m_ex, d->ctl, need, d->part, d->n_all);
     // mixed precision: FP32 slot totals, in float units of the same buffer
     float* part32 = reinterpret_cast<float*>(d->part);
-    if (hipError_t e = launch_force_sym(d->stream, d->sym_R, d->items_sym, d->n_items_sym, d->grid_sym, fx, fy, fz,
-                                        d->m32, d->ctl, need, part32, d->n_all))
+    if (hipError_t e = launch_force_sym(d->stream, d->sym_R, d->sym_self, d->items_sym, d->n_items_sym, d->grid_sym,
+                                        fx, fy, fz, d->m32, d->ctl, need, part32, d->n_all))
         return e;
     return launch_force_mixed(d->stream, (int)d->tuning.sinks_per_lane, d->items_force, d->n_items_force,
                               d->grid_force, fx, fy, fz, d->m32, d->ctl, need, part32, d->n_all);

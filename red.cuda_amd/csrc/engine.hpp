@@ -36,6 +36,14 @@ constexpr uint32_t kSymMaxR = 8;
 // (at least 1, at most kSymMaxR): a disk has kSymChunks..2 kSymChunks - 1
 // chunks (partial-sum slots per body) once it has that many blocks
 constexpr uint32_t kSymChunks = 128;
+// the symmetric kernel takes the diagonal tiles of a disk of at most this many
+// 256-body blocks (2 / (blocks + 1) of its tiles, 8% or more) with reactions,
+// the others one-sided: the kernel instance that holds the reaction sweep runs
+// its main loop about 2% slower (measured on one MI355X: a batch of 512 disks
+// of 64..4096 bodies, 16% diagonal tiles, runs 2.7% faster with it, a single
+// 16384-body disk, 3%, 2% slower), and the sweep saves a third of a diagonal
+// tile's work
+constexpr uint32_t kSymSelfMaxBlocks = 24;
 
 // Per-disk control flags (DiskCtl::flags).
 enum : uint32_t {
@@ -74,8 +82,11 @@ struct WorkItem {
     uint32_t src_lo, src_hi;
     uint32_t slot;       // partial-sum slot of the sinks
     uint32_t src_slot;   // symmetric kernel: partial-sum slot of the sources
-    uint32_t pad;
+    uint32_t flags;      // symmetric kernel: kItemSelf
 };
+// diagonal item whose full-block diagonal tiles take each pair once with its
+// reaction (a disk of at most kSymSelfMaxBlocks blocks)
+constexpr uint32_t kItemSelf = 1u;
 
 // Gas disk, device copy: the analytic model (analytic_gas_disk_params_t
 // red_type.h:464-485) or frame 0 of a FARGO grid (fargo_gas_disk.h:8-60).
@@ -148,8 +159,10 @@ hipError_t launch_force_mixed(hipStream_t s, int sinks_per_lane, const WorkItem*
 // Symmetric pair kernel over (chunk a, chunk c) items: sink_lo/hi = chunk a,
 // src_lo/hi = chunk c (256-body blocks, R per chunk), slot = c (sink side),
 // src_slot = a (source side).
-hipError_t launch_force_sym(hipStream_t s, uint32_t R, const WorkItem* items, uint32_t n_items, uint32_t grid,
-                            const double* x, const double* y, const double* z, const float* m32,
+// self: some item carries kItemSelf (the kernel instance with the diagonal
+// reaction sweep).
+hipError_t launch_force_sym(hipStream_t s, uint32_t R, bool self, const WorkItem* items, uint32_t n_items,
+                            uint32_t grid, const double* x, const double* y, const double* z, const float* m32,
                             const DiskCtl* ctl, uint32_t need_mask, float* part, uint32_t n_all);
 size_t force_sym_lds_bytes(uint32_t R);
 hipError_t launch_stage(hipStream_t s, int method, int r, const StageArgs& a);

@@ -397,7 +397,10 @@ __global__ __launch_bounds__(kBlock) void k_force_mixed(const WorkItem* __restri
 // touch 64 different sources, and the reaction sum of each source rides one
 // lane up per step (a DPP wave rotation folded into the accumulating
 // subtract).  After 64 steps lane l holds the reaction on source (l + 1) mod 64
-// over all 256 sinks.  Diagonal tiles (I = J) are one-sided (no reaction).
+// over all 256 sinks.  Diagonal tiles (I = J) are one-sided, except, in a
+// disk of at most kSymSelfMaxBlocks blocks, those of full blocks: they take
+// each pair once with its reaction through a per-wave choice of rows
+// (sym_sweep_self).
 //
 // Partial-sum slots: a body of chunk k gets one slot per partner chunk p:
 // as a sink of item (k, p) for p >= k, as a source of item (p, k) for p < k;
@@ -508,6 +511,118 @@ __device__ inline void sym_sweep_rows(uint32_t nS, const double2* __restrict__ t
     }
 }
 
+// One step of the diagonal-tile sweep for wave W: source (a, b) against row
+// slots 0..kN-1 (rows kRow[]) with reaction and, when kDiag, slot 2 on the
+// sink side only.  The arithmetic per pair is sym_sweep's.
+template <uint32_t W, int kN, bool kDiag>
+__device__ inline void self_step(const double2& a, const SrcZM& b, const double* xi, const double* yi,
+                                 const double* zi, const float* mi, float* gx, float* gy, float* gz, float& ax,
+                                 float& ay, float& az)
+{
+    constexpr uint32_t kRow[3] = {(W + 3u) & 3u, (W + 2u) & 3u, W};
+    constexpr int kR = kN + (kDiag ? 1 : 0);
+    static_assert(kN >= 1 && kR <= 3, "row slots");
+    float dx[kR], dy[kR], dz[kR], r2[kR], ri3[kR];
+#pragma unroll
+    for (int t = 0; t < kR; ++t) {
+        const uint32_t r = kRow[t < kN ? t : 2];
+        dx[t] = (float)(a.x - xi[r]);
+        dy[t] = (float)(a.y - yi[r]);
+        dz[t] = (float)(b.z - zi[r]);
+    }
+#pragma unroll
+    for (int t = 0; t < kR; ++t) r2[t] = __builtin_fmaf(dz[t], dz[t], kEps2);
+#pragma unroll
+    for (int t = 0; t < kR; ++t) r2[t] = __builtin_fmaf(dy[t], dy[t], r2[t]);
+#pragma unroll
+    for (int t = 0; t < kR; ++t) r2[t] = __builtin_fmaf(dx[t], dx[t], r2[t]);
+#pragma unroll
+    for (int t = 0; t < kR; ++t) ri3[t] = __builtin_amdgcn_rsqf(r2[t]);
+#pragma unroll
+    for (int t = 0; t < kR; ++t) ri3[t] = (ri3[t] * ri3[t]) * ri3[t];
+#pragma unroll
+    for (int t = 0; t < kR; ++t) {
+        const int s = t < kN ? t : 2;
+        const float scj = b.m * ri3[t];
+        gx[s] = __builtin_fmaf(scj, dx[t], gx[s]);
+        gy[s] = __builtin_fmaf(scj, dy[t], gy[s]);
+        gz[s] = __builtin_fmaf(scj, dz[t], gz[s]);
+    }
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+#pragma unroll
+    for (int t = 0; t < kN; ++t) {
+        const float sci = mi[kRow[t]] * ri3[t];
+        sx = __builtin_fmaf(sci, dx[t], sx);
+        sy = __builtin_fmaf(sci, dy[t], sy);
+        sz = __builtin_fmaf(sci, dz[t], sz);
+    }
+    ax = rot_sub(ax, sx);
+    ay = rot_sub(ay, sy);
+    az = rot_sub(az, sz);
+}
+
+// The diagonal tile (I = J) of a full block, with reactions, for wave W.  Its
+// 16 (sink row, source quarter) blocks hold every unordered pair twice and
+// every body once with itself, so wave W sweeps its quarter against three
+// rows only:
+//   slot 0, row W - 1 (mod 4), steps 0..63: the pairs of blocks W - 1 and W;
+//   slot 2, row W, steps 1..31 and, on the sink side only, step 32: the pairs
+//     inside block W (lanes l and l - 32 meet at step 32 from both ends);
+//   slot 1, row W + 2 (mod 4), steps 0..31 when W >= 2 and 1..32 when W < 2:
+//     of the pairs of blocks W and W + 2 (lanes p and q), wave W + 2 (W < 2)
+//     takes those with (p - q) mod 64 < 32 and wave W the others.
+// Each unordered pair is taken once, with its reaction, and each wave runs 127
+// row-steps with reaction and one without, where the one-sided sweep of the
+// four rows runs 256 without.  Sink sums go to f* by row (0 for row W + 1),
+// reactions to r* as in sym_sweep: the steps run in order 0..63, so the
+// rotation and the 16-step folds are sym_sweep's.
+template <uint32_t W>
+__device__ inline void sym_sweep_self(const double2* __restrict__ txy, const SrcZM* __restrict__ tzm,
+                                      uint32_t lane, const double* xi, const double* yi, const double* zi,
+                                      const float* mi, float* fx, float* fy, float* fz, double& rx, double& ry,
+                                      double& rz)
+{
+    constexpr uint32_t qb = W * 64u;
+    constexpr uint32_t kRow[3] = {(W + 3u) & 3u, (W + 2u) & 3u, W};
+    constexpr bool kHi = W >= 2u;
+    float gx[3] = {0.0f, 0.0f, 0.0f}, gy[3] = {0.0f, 0.0f, 0.0f}, gz[3] = {0.0f, 0.0f, 0.0f};
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    rx = ry = rz = 0.0;
+    const int from16 = (int)((lane - 16u) & 63u);
+    auto fold = [&]() {
+        rx = __shfl(rx, from16) + (double)ax;
+        ry = __shfl(ry, from16) + (double)ay;
+        rz = __shfl(rz, from16) + (double)az;
+        ax = ay = az = 0.0f;
+    };
+#define SELF_STEP(U, N, D)                                                                          \
+    {                                                                                               \
+        const uint32_t k = qb + ((lane - (U)) & 63u);                                               \
+        self_step<W, N, D>(txy[k], tzm[k], xi, yi, zi, mi, gx, gy, gz, ax, ay, az);                 \
+    }
+    SELF_STEP(0u, kHi ? 2 : 1, false)
+#pragma unroll 1
+    for (uint32_t u = 1; u < 16; ++u) SELF_STEP(u, 3, false)
+    fold();
+#pragma unroll 1
+    for (uint32_t u = 16; u < 32; ++u) SELF_STEP(u, 3, false)
+    fold();
+    SELF_STEP(32u, kHi ? 1 : 2, true)
+#pragma unroll 1
+    for (uint32_t u = 33; u < 48; ++u) SELF_STEP(u, 1, false)
+    fold();
+#pragma unroll 1
+    for (uint32_t u = 48; u < 64; ++u) SELF_STEP(u, 1, false)
+    fold();
+#undef SELF_STEP
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        fx[kRow[t]] = gx[t];
+        fy[kRow[t]] = gy[t];
+        fz[kRow[t]] = gz[t];
+    }
+}
+
 // Dynamic LDS: the source tile (xy, zm: 8 KB; between tiles it also carries
 // one wave's sink sums, 6 KB, for the combine) and the reaction accumulators
 // (R x 6 KB).
@@ -518,6 +633,13 @@ __host__ __device__ constexpr size_t sym_lds_bytes(uint32_t R)
 static_assert(kSymS * 3 * 64 * sizeof(double) <= kSymBlock * (sizeof(double2) + sizeof(SrcZM)),
               "the wave-combine buffer must fit in the tile");
 
+// kSelf: the diagonal tiles of full blocks of items flagged kItemSelf go
+// through sym_sweep_self; without it (and in unflagged items) they are
+// one-sided.  Which diagonal tiles take reactions is a property of the item
+// alone, so both instances give the same results; the engine launches this
+// one only when an item is flagged (the compiler schedules the main loop of
+// the instance with the extra sweep less well).
+template <bool kSelf>
 __global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict__ items, uint32_t n_items,
                                                       const double* __restrict__ x, const double* __restrict__ y,
                                                       const double* __restrict__ z, const float* __restrict__ m32,
@@ -575,9 +697,30 @@ __global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict
                 for (int s = 0; s < (int)kSymS; ++s) fx[s] = fy[s] = fz[s] = 0.0f;
                 double rx = 0.0, ry = 0.0, rz = 0.0;
                 const bool self = diag && bj == bi;
+                // the diagonal tile of a block with all four sink rows takes each
+                // pair once with its reaction; with fewer rows (a disk's last
+                // block) the one-sided sweep of the valid rows is the cheaper one
+                bool react = false;
                 // a wave whose quarter holds no source (a disk's last block) has
                 // nothing to add: its sources are padding with mass 0
-                if (j0 + wave * 64u < w.src_hi) {
+                if constexpr (kSelf) {
+                    react = !self || ((w.flags & kItemSelf) && nS == kSymS);
+                    if (j0 + wave * 64u < w.src_hi) {
+                        if (!self)
+                            sym_sweep_rows<true>(nS, txy, tzm, wave * 64u, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry,
+                                                 rz);
+                        else if (!react)
+                            sym_sweep_rows<false>(nS, txy, tzm, wave * 64u, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry,
+                                                  rz);
+                        else
+                            switch (wave) {
+                            case 0: sym_sweep_self<0>(txy, tzm, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
+                            case 1: sym_sweep_self<1>(txy, tzm, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
+                            case 2: sym_sweep_self<2>(txy, tzm, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
+                            default: sym_sweep_self<3>(txy, tzm, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
+                            }
+                    }
+                } else if (j0 + wave * 64u < w.src_hi) {
                     if (self)
                         sym_sweep_rows<false>(nS, txy, tzm, wave * 64u, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz);
                     else
@@ -589,7 +732,7 @@ __global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict
                     ay[s] += (double)fy[s];
                     az[s] += (double)fz[s];
                 }
-                if (!self) {
+                if (kSelf ? react : !self) {
                     const uint32_t k = wave * 64u + ((lane + 1u) & 63u);
                     double* ab = acc + (size_t)bj * 3 * kSymBlock;
                     ab[k] += rx;
@@ -654,14 +797,18 @@ __global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict
     }
 }
 
-hipError_t launch_force_sym(hipStream_t st, uint32_t R, const WorkItem* items, uint32_t n_items, uint32_t grid,
-                            const double* x, const double* y, const double* z, const float* m32,
+hipError_t launch_force_sym(hipStream_t st, uint32_t R, bool self, const WorkItem* items, uint32_t n_items,
+                            uint32_t grid, const double* x, const double* y, const double* z, const float* m32,
                             const DiskCtl* ctl, uint32_t need, float* part, uint32_t n_all)
 {
     if (n_items == 0) return hipSuccess;
     if (R < 1 || R > kSymMaxR) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_force_sym, dim3(grid), dim3(kBlock), sym_lds_bytes(R), st, items, n_items, x, y, z, m32,
-                       ctl, need, part, n_all);
+    if (self)
+        hipLaunchKernelGGL(k_force_sym<true>, dim3(grid), dim3(kBlock), sym_lds_bytes(R), st, items, n_items, x, y, z,
+                           m32, ctl, need, part, n_all);
+    else
+        hipLaunchKernelGGL(k_force_sym<false>, dim3(grid), dim3(kBlock), sym_lds_bytes(R), st, items, n_items, x, y,
+                           z, m32, ctl, need, part, n_all);
     return hipGetLastError();
 }
 

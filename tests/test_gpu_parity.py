@@ -299,15 +299,21 @@ def crowded(n, seed):
     return r, v, p, b, nbt
 
 
-def test_driver_loop_events_match_oracle():
-    r, v, p, b, nbt = crowded(96, 5)
+@pytest.mark.parametrize("method", [R.RD_RKF78, R.RD_RK4])
+def test_driver_loop_events_match_oracle(method):
+    """Driver loop with collisions, hit-centrum and a rebuild after 16 events:
+    adaptive RKF7(8), and fixed-step RK4, whose stages run inside the FP64
+    pair kernel on a disk this small."""
+    n, length, max_steps = (96, 40.0, 60) if method == R.RD_RKF78 else (128, 400.0, 120)
+    r, v, p, b, nbt = crowded(n, 5)
     thr = (0.2, 50.0, 1000.0)       # hit-centrum 0.2 AU, ejection 50 AU, radius factor 1000
     d, o = pair(r, v, p, b, nbt, thr=thr)
     h = 1.0 * KDAY
-    it = R.Integrator(d, R.RD_RKF78, h, adaptive=True, tol=1e-10)
-    oi = O.OInteg(o, 4, h, adaptive=True, tol=1e-10)
-    it.run(40.0, 2.0, 60)
-    rc, steps = oi.run(40.0, 2.0, 60)
+    adaptive = method == R.RD_RKF78
+    it = R.Integrator(d, method, h, adaptive=adaptive, tol=1e-10)
+    oi = O.OInteg(o, method, h, adaptive=adaptive, tol=1e-10)
+    it.run(length, 2.0, max_steps)
+    rc, steps = oi.run(length, 2.0, max_steps)
     assert rc == 0
     ev, oev = d.event_log(), o.event_log()
     assert len(ev) == len(oev) and len(ev) > 16          # enough to trigger a rebuild

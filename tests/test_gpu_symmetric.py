@@ -36,11 +36,15 @@ def check_bounds(got, ref, mut):
     assert np.all(err <= 2e-5 * m + 1e-18), (err / m).max()
 
 
-# bodies -> 256-body blocks -> chunks of R blocks: 301 -> 2 blocks (partial),
+# bodies -> 256-body blocks -> chunks of R blocks: 201 -> one block whose
+# fourth row is partial, 256 -> one full block (the diagonal tiles of disks of
+# at most 24 blocks take each pair once with its reaction), 151 -> one block of
+# three rows (one-sided diagonal tile), 301 -> 2 blocks (partial),
 # 5000 -> 20 (R = 1), 70000 -> 274 (R = 2, 137 chunks), 262200 -> 1025 (R = 8),
 # 1048576 -> 4096 (R at its cap of 8: 512 chunks, past the 255 the cap-free
 # rule allows, so 512 partial-sum slots per body)
-@pytest.mark.parametrize("n,seed", [(300, 31), (4999, 32), (69999, 33), (262199, 34), (1048575, 36)])
+@pytest.mark.parametrize("n,seed", [(200, 37), (255, 38), (150, 39), (300, 31), (4999, 32), (69999, 33),
+                                    (262199, 34), (1048575, 36)])
 def test_symmetric_force_bounds(n, seed):
     r, v, p, b, nbt = R.make_disk(n, seed)
     ref, mut = fp64_and_mutual(r, v, p, b, nbt)
@@ -80,8 +84,11 @@ def test_symmetric_agrees_with_one_sided():
 
 
 def test_symmetric_batch_matches_single_disks():
-    # uneven disks, each with its own chunking (R = 1 and R = 2), in one launch
-    sizes = [(69999, 41), (700, 42), (5000, 43), (129, 44)]
+    # uneven disks, each with its own chunking (R = 1 and R = 2), in one launch;
+    # disks of at most 24 blocks (6144 bodies) take their diagonal tiles with
+    # reactions, so the batch runs the kernel instance with that sweep while
+    # the 70000- and 6145-body disks alone run the one without it
+    sizes = [(69999, 41), (700, 42), (5000, 43), (129, 44), (6143, 45), (6144, 46)]
     parts = [R.make_disk(n, s) for n, s in sizes]
     cat = [np.concatenate([q[i] for q in parts]) for i in range(4)]
     bd = R.Disk(*cat, np.stack([q[4] for q in parts]), batch=True)
