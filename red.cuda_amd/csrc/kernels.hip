@@ -411,6 +411,11 @@ __global__ __launch_bounds__(kBlock) void k_force_mixed(const WorkItem* __restri
 // FP64, accumulated over the chunk's I blocks in FP64 (LDS).
 constexpr uint32_t kSymBlock = kSymBlockBodies;
 constexpr uint32_t kSymS = kSymBlock / 64;
+// LDS source tile: each 64-source quarter q twice in a row at entry 128 q, so
+// that the source of lane l at step u, (l - u) mod 64, is entry
+// 128 q + 64 + l - u: one address per lane, stepped down by one entry, and no
+// wrap to compute at every step
+constexpr uint32_t kSymTile = 2 * kSymBlock;
 
 struct alignas(16) SrcZM {
     double z;
@@ -424,13 +429,14 @@ __device__ inline float rot_sub(float acc, float v)
 }
 
 // One wave's sweep of its 64-source quarter over the first kS sink rows
-// (lane l, row s: sink l + 64 s): sink sums into f*, reactions (when kReact)
+// (lane l, row s: sink l + 64 s; kb: lane l's tile entry at step 0,
+// 128 q + 64 + l for quarter q): sink sums into f*, reactions (when kReact)
 // into r* (lane l: source (l + 1) mod 64 of the quarter).  The FP32 reaction
 // sums are folded into FP64 every 16 steps (64 terms, as on the sink side):
 // the FP64 running sum is rotated 16 lanes at each fold so that it stays with
 // the sources the FP32 sum in the same lane belongs to.
 template <bool kReact, int kS>
-__device__ inline void sym_sweep(const double2* __restrict__ txy, const SrcZM* __restrict__ tzm, uint32_t qb,
+__device__ inline void sym_sweep(const double2* __restrict__ txy, const SrcZM* __restrict__ tzm, uint32_t kb,
                                  uint32_t lane, const double* xi, const double* yi, const double* zi,
                                  const float* mi, float* fx, float* fy, float* fz, double& rx, double& ry,
                                  double& rz)
@@ -444,7 +450,7 @@ __device__ inline void sym_sweep(const double2* __restrict__ txy, const SrcZM* _
         // and the rsq each have latency the next phase would otherwise wait for).
 #pragma unroll 1
         for (uint32_t u = u0; u < u0 + 16; ++u) {
-            const uint32_t k = qb + ((lane - u) & 63u);
+            const uint32_t k = kb - u;
             const double2 a = txy[k];
             const SrcZM b = tzm[k];
             float dx[kS], dy[kS], dz[kS], r2[kS], ri3[kS];
@@ -499,15 +505,15 @@ __device__ inline void sym_sweep(const double2* __restrict__ txy, const SrcZM* _
 // (a disk's last block), and skipping them only drops terms that are exactly 0.
 template <bool kReact>
 __device__ inline void sym_sweep_rows(uint32_t nS, const double2* __restrict__ txy, const SrcZM* __restrict__ tzm,
-                                      uint32_t qb, uint32_t lane, const double* xi, const double* yi,
+                                      uint32_t kb, uint32_t lane, const double* xi, const double* yi,
                                       const double* zi, const float* mi, float* fx, float* fy, float* fz,
                                       double& rx, double& ry, double& rz)
 {
     switch (nS) {
-    case 1: sym_sweep<kReact, 1>(txy, tzm, qb, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
-    case 2: sym_sweep<kReact, 2>(txy, tzm, qb, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
-    case 3: sym_sweep<kReact, 3>(txy, tzm, qb, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
-    default: sym_sweep<kReact, kSymS>(txy, tzm, qb, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
+    case 1: sym_sweep<kReact, 1>(txy, tzm, kb, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
+    case 2: sym_sweep<kReact, 2>(txy, tzm, kb, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
+    case 3: sym_sweep<kReact, 3>(txy, tzm, kb, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
+    default: sym_sweep<kReact, kSymS>(txy, tzm, kb, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
     }
 }
 
@@ -582,7 +588,7 @@ __device__ inline void sym_sweep_self(const double2* __restrict__ txy, const Src
                                       const float* mi, float* fx, float* fy, float* fz, double& rx, double& ry,
                                       double& rz)
 {
-    constexpr uint32_t qb = W * 64u;
+    const uint32_t kb = W * 128u + 64u + lane;   // tile entry at step 0 (kSymTile)
     constexpr uint32_t kRow[3] = {(W + 3u) & 3u, (W + 2u) & 3u, W};
     constexpr bool kHi = W >= 2u;
     float gx[3] = {0.0f, 0.0f, 0.0f}, gy[3] = {0.0f, 0.0f, 0.0f}, gz[3] = {0.0f, 0.0f, 0.0f};
@@ -597,7 +603,7 @@ __device__ inline void sym_sweep_self(const double2* __restrict__ txy, const Src
     };
 #define SELF_STEP(U, N, D)                                                                          \
     {                                                                                               \
-        const uint32_t k = qb + ((lane - (U)) & 63u);                                               \
+        const uint32_t k = kb - (U);                                                                \
         self_step<W, N, D>(txy[k], tzm[k], xi, yi, zi, mi, gx, gy, gz, ax, ay, az);                 \
     }
     SELF_STEP(0u, kHi ? 2 : 1, false)
@@ -623,14 +629,14 @@ __device__ inline void sym_sweep_self(const double2* __restrict__ txy, const Src
     }
 }
 
-// Dynamic LDS: the source tile (xy, zm: 8 KB; between tiles it also carries
-// one wave's sink sums, 6 KB, for the combine) and the reaction accumulators
-// (R x 6 KB).
+// Dynamic LDS: the source tile (xy, zm, each quarter twice: 16 KB; between
+// tiles it also carries one wave's sink sums, 6 KB, for the combine) and the
+// reaction accumulators (R x 6 KB).
 __host__ __device__ constexpr size_t sym_lds_bytes(uint32_t R)
 {
-    return kSymBlock * (sizeof(double2) + sizeof(SrcZM)) + (size_t)R * 3 * kSymBlock * sizeof(double);
+    return kSymTile * (sizeof(double2) + sizeof(SrcZM)) + (size_t)R * 3 * kSymBlock * sizeof(double);
 }
-static_assert(kSymS * 3 * 64 * sizeof(double) <= kSymBlock * (sizeof(double2) + sizeof(SrcZM)),
+static_assert(kSymS * 3 * 64 * sizeof(double) <= kSymTile * (sizeof(double2) + sizeof(SrcZM)),
               "the wave-combine buffer must fit in the tile");
 
 // kSelf: the diagonal tiles of full blocks of items flagged kItemSelf go
@@ -648,11 +654,12 @@ __global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict
 {
     extern __shared__ __align__(16) unsigned char sym_lds[];
     double2* txy = reinterpret_cast<double2*>(sym_lds);
-    SrcZM* tzm = reinterpret_cast<SrcZM*>(txy + kSymBlock);
-    double* red = reinterpret_cast<double*>(sym_lds);           // [S][3][64], aliases the tile
-    double* acc = reinterpret_cast<double*>(tzm + kSymBlock);   // [R][3][256]
+    SrcZM* tzm = reinterpret_cast<SrcZM*>(txy + kSymTile);
+    double* red = reinterpret_cast<double*>(sym_lds);          // [S][3][64], aliases the tile
+    double* acc = reinterpret_cast<double*>(tzm + kSymTile);   // [R][3][256]
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t kb = wave * 128u + 64u + lane;   // this lane's tile entry at step 0 (kSymTile)
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         const WorkItem w = items[it];
         if (!(ctl[w.disk].flags & need)) continue;
@@ -688,8 +695,11 @@ __global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict
                         b.z = z[j];
                         b.m = m32[j];
                     }
-                    txy[tid] = a;
-                    tzm[tid] = b;
+                    const uint32_t e = (tid & ~63u) * 2u + (tid & 63u);   // quarter q: 128 q + l and + 64
+                    txy[e] = a;
+                    txy[e + 64u] = a;
+                    tzm[e] = b;
+                    tzm[e + 64u] = b;
                 }
                 __syncthreads();
                 float fx[kSymS], fy[kSymS], fz[kSymS];
@@ -707,11 +717,9 @@ __global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict
                     react = !self || ((w.flags & kItemSelf) && nS == kSymS);
                     if (j0 + wave * 64u < w.src_hi) {
                         if (!self)
-                            sym_sweep_rows<true>(nS, txy, tzm, wave * 64u, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry,
-                                                 rz);
+                            sym_sweep_rows<true>(nS, txy, tzm, kb, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz);
                         else if (!react)
-                            sym_sweep_rows<false>(nS, txy, tzm, wave * 64u, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry,
-                                                  rz);
+                            sym_sweep_rows<false>(nS, txy, tzm, kb, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz);
                         else
                             switch (wave) {
                             case 0: sym_sweep_self<0>(txy, tzm, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz); break;
@@ -722,9 +730,9 @@ __global__ __launch_bounds__(kBlock) void k_force_sym(const WorkItem* __restrict
                     }
                 } else if (j0 + wave * 64u < w.src_hi) {
                     if (self)
-                        sym_sweep_rows<false>(nS, txy, tzm, wave * 64u, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz);
+                        sym_sweep_rows<false>(nS, txy, tzm, kb, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz);
                     else
-                        sym_sweep_rows<true>(nS, txy, tzm, wave * 64u, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz);
+                        sym_sweep_rows<true>(nS, txy, tzm, kb, lane, xi, yi, zi, mi, fx, fy, fz, rx, ry, rz);
                 }
 #pragma unroll
                 for (int s = 0; s < (int)kSymS; ++s) {
