@@ -92,6 +92,40 @@ __host__ __device__ constexpr double rk4_c(int r) { return (r == 1 || r == 2) ? 
 // compute nothing.
 constexpr int kExactBlock = kBlock + 64;   // 4 term waves + 1 summing wave
 
+// m / (sqrt(r2) * r2) for r2 and m in the range exact_scale_in_range accepts,
+// bit for bit what the IEEE square root and division give: the instruction
+// sequences the compiler emits for them (v_rsq_f64 and two Goldschmidt steps
+// for the root; v_rcp_f64, two Newton steps and a residual step for the
+// quotient) without their range handling, which does nothing in that range (no
+// operand is scaled by v_ldexp or v_div_scale, and v_div_fixup and the
+// zero/infinity test of the root return the computed value).  About a fifth of
+// a pair term's instructions.
+__device__ __forceinline__ bool exact_scale_in_range(double r2, double m)
+{
+    return r2 >= 0x1p-200 && r2 <= 0x1p200 && m >= 0x1p-300 && m <= 0x1p300;
+}
+__device__ __forceinline__ double exact_scale_fast(double r2, double m)
+{
+    const double y = __builtin_amdgcn_rsq(r2);
+    double g = r2 * y, h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    double e = __builtin_fma(-g, g, r2);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(e, h, g);
+    e = __builtin_fma(-g, g, r2);
+    g = __builtin_fma(e, h, g);   // sqrt(r2)
+    const double den = g * r2;
+    double q = __builtin_amdgcn_rcp(den);
+    double t = __builtin_fma(-den, q, 1.0);
+    q = __builtin_fma(q, t, q);
+    t = __builtin_fma(-den, q, 1.0);
+    q = __builtin_fma(q, t, q);   // 1 / den
+    const double a = m * q;
+    const double rem = __builtin_fma(-den, a, m);
+    return __builtin_fma(rem, q, a);
+}
+
 // Single-disk classical RK4 stage R for body i, component q (defined with the
 // stage kernels below).
 template <int R, bool kFused>
@@ -161,15 +195,25 @@ __device__ __forceinline__ void force_exact_body(const WorkItem* __restrict__ it
                             const double dy = sj.y - yi;
                             const double dz = sj.z - zi;
                             const double r2 = SQR(dx) + SQR(dy) + SQR(dz);
-                            const double d = sqrt(r2);
-                            const double sc = sj.w / (d * r2);
-                            // a skipped pair (self, inactive or padding source) adds +0:
-                            // the running sum starts at +0 and round-to-nearest never
-                            // makes it -0, so acc + 0 == acc bit for bit
                             const bool skip = sj.w < 0.0 || t0 + k == i;
-                            term[b][0][g][k] = skip ? 0.0 : sc * dx;
-                            term[b][1][g][k] = skip ? 0.0 : sc * dy;
-                            term[b][2][g][k] = skip ? 0.0 : sc * dz;
+                            double sc;
+                            if constexpr (kS >= 16) {
+                                // throughput-bound (large disks): the short sequence, with
+                                // the library's for the rare pair outside its range
+                                sc = exact_scale_fast(r2, sj.w);
+                                if (!skip && !exact_scale_in_range(r2, sj.w)) sc = sj.w / (sqrt(r2) * r2);
+                            } else {
+                                // latency-bound (small disks): the branch would cost more
+                                // than the shorter sequence saves
+                                sc = sj.w / (sqrt(r2) * r2);
+                            }
+                            // a skipped pair (self, inactive or padding source) adds 0 * d,
+                            // a zero of either sign: the running sum starts at +0 and
+                            // round-to-nearest never makes it -0, so the sum is unchanged
+                            sc = skip ? 0.0 : sc;
+                            term[b][0][g][k] = sc * dx;
+                            term[b][1][g][k] = sc * dy;
+                            term[b][2][g][k] = sc * dz;
                         }
                     }
                     if (tid < kTile) {
@@ -251,8 +295,8 @@ __global__ __launch_bounds__(kBlock) void k_force_exact_lane(const WorkItem* __r
                 const double dy = sj.y - yi;
                 const double dz = sj.z - zi;
                 const double r2 = SQR(dx) + SQR(dy) + SQR(dz);
-                const double d = sqrt(r2);
-                const double sc = sj.w / (d * r2);
+                double sc = exact_scale_fast(r2, sj.w);
+                if (!exact_scale_in_range(r2, sj.w)) sc = sj.w / (sqrt(r2) * r2);
                 ax += sc * dx;
                 ay += sc * dy;
                 az += sc * dz;

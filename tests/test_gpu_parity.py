@@ -47,6 +47,21 @@ def test_force_fp64_bitexact(case):
     assert bits_equal(dv, v)
 
 
+def test_fp64_lane_kernel_bitexact():
+    """A batch of at least 1024 256-sink groups (4 per CU) runs the FP64 pair
+    kernel with one sink per lane; its accelerations are the oracle's bit for
+    bit, checked on three of its 260 disks of 1025 bodies."""
+    parts = [R.make_disk(1024, 900 + k) for k in range(260)]
+    cat = [np.concatenate([q[i] for q in parts]) for i in range(4)]
+    bd = R.Disk(*cat, np.stack([q[4] for q in parts]), batch=True)
+    for k in (0, 129, 259):
+        r, v, p, b, nbt = parts[k]
+        o = O.ODisk(r, v, p, b, nbt)
+        dy = bd.calc_dydx(1, 0, 0.0, r, v, k=k)
+        rc, ody = o.calc_dydx(1, 0, 0.0, r, v)
+        assert rc == 0 and bits_equal(dy, ody), k
+
+
 @pytest.mark.parametrize("mode,spl", [(R.RD_PAIRS_ONE_SIDED, 1), (R.RD_PAIRS_ONE_SIDED, 2),
                                       (R.RD_PAIRS_ONE_SIDED, 4), (R.RD_PAIRS_SYMMETRIC, 2)])
 def test_force_mixed_vs_fp64(mode, spl):
