@@ -962,14 +962,14 @@ void or_set_guard(or_integ* it, uint32_t max_attempts, double min_dt, int nonfin
     it->min_dt = min_dt;
     it->nonfinite = nonfinite;
 }
-/* the guard's verdict on the attempt just judged (iter counts it) */
+/* the guard's decision on the attempt just judged (iter counts it) */
 static int guard_trips(const or_integ* it, double max_err, int iter)
 {
     if (it->nonfinite && (it->last_bad || !isfinite(max_err))) return 1;
     if (max_err > it->tol && it->max_attempts && (uint32_t)iter >= it->max_attempts) return 1;
     return 0;
 }
-/* the guard's verdict on the attempt about to be made (iter made so far) */
+/* the guard's decision on the attempt about to be made (iter made so far) */
 static int guard_refuses_dt(or_integ* it, int iter)
 {
     if (!(it->adaptive && 0.0 < it->min_dt && fabs(it->dt_try) <= it->min_dt)) return 0;
