@@ -694,6 +694,7 @@ def main():
     # pair-kernel launch time (HIP events on the engine stream) for the roofline
     ms, pairs_launch = d.bench_force(20)
     achieved = FLOP_PER_PAIR * pairs_launch / (ms * 1e-3)
+    peak = PEAK_FP32 if args.precision == "mixed" else PEAK_FP64
     # the RK passes of one attempt (HIP events on the engine stream)
     stage_rl = roofline_stage(d, args.n + 1, traffic=profiled_stage_traffic(f"k_stage_rk8_disk{args.n}_rkf78.txt")
                               if (args.precision == "mixed" and args.seed == 2) else None)
@@ -760,6 +761,8 @@ def main():
                 "mixed_over_fp64_steps_per_s": steps_per_s / ((c1[1] - c0[1]) / el_),
                 "roofline": {"bound": "valu-fp64", "achieved": ach64 / 1e12, "peak": PEAK_FP64 / 1e12,
                              "unit": "TFLOP/s", "frac": ach64 / PEAK_FP64, "kernel": "k_force_exact",
+                             "traffic": profiled_traffic("k_force_exact", "k_force_exact_disk16384_rkf78.txt", None)
+                             if (args.n == 16384 and args.seed == 2) else None,
                              "ms_per_launch": ms64, "pairs_per_launch": pl64}}
         del it_, dt_
 
@@ -799,11 +802,12 @@ def main():
                         f"(timed steps alone: {rel_e:.2e})",
         "max_rel_force_err_vs_fp64": f_err,
         "final_state_vs_fp64": fs_diff,
-        "roofline": {"bound": "valu-fp32", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32,
+        "roofline": {"bound": "valu-fp32" if args.precision == "mixed" else "valu-fp64",
+                     "achieved": achieved / 1e12, "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": profiled_traffic(pair_kernel(args.precision, tuning),
                                                  f"{pair_kernel(args.precision, tuning)}_disk16384_rkf78.txt", tuning)
-                     if (args.precision == "mixed" and args.n == 16384 and args.seed == 2) else None,
+                     if (args.n == 16384 and args.seed == 2) else None,
                      "kernel": pair_kernel(args.precision, tuning),
                      "ms_per_launch": ms, "pairs_per_launch": pairs_launch},
         "roofline_stage": stage_rl,
