@@ -25,13 +25,23 @@ def pair(r, v, p, b, nbt, thr=(0.0, 0.0, 0.0), gas=None, t=0.0):
     return d, o
 
 
-# below ~4096 bodies the FP64 kernel takes 4-sink items, above 16-sink ones
-@pytest.mark.parametrize("case", ["disk256", "inactive1000", "classes700", "ragged1", "disk5000"])
+# below ~4096 bodies the FP64 kernel takes 4-sink items, above 16-sink ones;
+# "extremes5000": sources of mass 1e-95 and 0 and a pair 1e-31 AU apart, outside
+# the range of the 16-sink items' short root and division sequences (the
+# library path takes those pairs)
+@pytest.mark.parametrize("case", ["disk256", "inactive1000", "classes700", "ragged1", "disk5000", "extremes5000"])
 def test_force_fp64_bitexact(case):
     if case == "disk256":
         r, v, p, b, nbt = R.make_disk(255, 11)
     elif case == "disk5000":
         r, v, p, b, nbt = R.make_disk(4999, 15)
+    elif case == "extremes5000":
+        r, v, p, b, nbt = R.make_disk(4999, 16)
+        p["mass"][[10, 2000, 4500]] = 1e-95
+        p["mass"][3333] = 0.0
+        # coordinates this small resolve a 1e-31 AU offset (r2 = 1e-62 < 2^-200)
+        r["x"][3000], r["y"][3000], r["z"][3000] = 1e-20, 2e-20, 0.0
+        r["x"][3001], r["y"][3001], r["z"][3001] = 1e-20 + 1e-31, 2e-20, 0.0
     elif case == "inactive1000":
         r, v, p, b, nbt = small_system(1000, 12, inactive=(1, 2, 17, 256, 511, 999))
     elif case == "classes700":
