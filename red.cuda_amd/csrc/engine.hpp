@@ -38,11 +38,12 @@ constexpr uint32_t kSymMaxR = 8;
 constexpr uint32_t kSymChunks = 128;
 // the symmetric kernel takes the diagonal tiles of a disk of at most this many
 // 256-body blocks (2 / (blocks + 1) of its tiles, 8% or more) with reactions,
-// the others one-sided: the kernel instance that holds the reaction sweep runs
-// its main loop about 2% slower (measured on one MI355X: a batch of 512 disks
-// of 64..4096 bodies, 16% diagonal tiles, runs 2.7% faster with it, a single
-// 16384-body disk, 3%, 2% slower), and the sweep saves a third of a diagonal
-// tile's work
+// the others one-sided: the kernel instance that holds the reaction sweep
+// compiles its main loop in another order, and the sweep saves a third of a
+// diagonal tile's work (measured on one MI355X: a batch of 512 disks of
+// 64..4096 bodies, 16% diagonal tiles, runs 2.7% faster with it; a single
+// 16384-body disk, 3%, ran 2% slower before the source tile was doubled in LDS
+// and runs at the same speed since)
 constexpr uint32_t kSymSelfMaxBlocks = 24;
 
 // Per-disk control flags (DiskCtl::flags).
