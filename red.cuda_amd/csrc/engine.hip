@@ -673,6 +673,7 @@ StageArgs make_args(rd_disk* d, int r, int adaptive, int err_all)
     a.gas = d->gas;
     a.mixed = d->precision == RD_PREC_MIXED;
     a.adaptive = adaptive;
+    a.commit = -1;
     a.one_disk = d->n_disk == 1;
     if (a.one_disk && !d->h_info.empty()) a.info0 = d->h_info[0];
     a.trash = d->trash;
@@ -721,17 +722,16 @@ hipError_t enqueue_attempt(rd_integrator* it, int commit_advance)
     // kernel, one launch per stage instead of two
     if (d->precision == RD_PREC_FP64 && !d->exact_lane && d->exact_small && d->n_disk == 1 &&
         d->gas.on == kGasNone && it->method == RD_RK4 && !it->adaptive) {
+        // the last stage also commits (k_commit's copy and time advance): its
+        // summing lanes cover every body, and no pair term of that launch reads y_n
         for (int r = 0; r <= last_stage(it); ++r) {
             StageArgs a = make_args(d, r, it->adaptive, it->err_all);
             a.fuse_comb1 = r == 0;   // stage 0 forms stage 1 itself, as in the separate kernel
+            if (r == last_stage(it)) a.commit = commit_advance;
             if ((e = launch_force_exact_rk4(d->stream, r, d->items_force, d->n_items_force, d->grid_force,
                                             a.fpos[0], a.fpos[1], a.fpos[2], d->m_ex, d->ctl,
                                             r == 0 ? F_K0_FORCE : F_ACTIVE, a)) != hipSuccess)
                 return e;
-        }
-        if (commit_advance >= 0) {
-            StageArgs a = make_args(d, 0, it->adaptive, it->err_all);
-            if ((e = launch_commit(d->stream, a, commit_advance, d->ctl)) != hipSuccess) return e;
         }
         return hipSuccess;
     }

@@ -126,6 +126,8 @@ struct StageArgs {
     int mixed;
     int adaptive;
     int fuse_comb1;              // RK4 stage 0 also forms stage 1 (fixed step: k0 is never reused)
+    int commit;                  // fused fixed-step RK4 stage 3: -1 stores y_{n+1} in out; 0 or 1
+                                 // commits it in place (k_commit's copy) and, for 1, advances t
     int one_disk;                // every body belongs to disk 0
     DiskInfo info0;              // info[0] when one_disk (kernel arguments are scalar loads)
     double* trash;               // [3][kBodyBlock]: stores of a single-disk stage that does not apply

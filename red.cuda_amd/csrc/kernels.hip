@@ -1529,6 +1529,17 @@ __device__ __forceinline__ void stage_rk4_body(const StageArgs& a, uint32_t i, i
         const double b0 = (1.0 / 6.0) * h, b1 = (1.0 / 3.0) * h;
         const double yo = yy + b0 * (k0v + kk) + b1 * (k1v + k2v);
         const double vo = vv + b0 * (k0a + kRa) + b1 * (k1a + k2a);
+        if constexpr (kFused) {
+            // k_commit's work for a committed attempt: y_n <- y_{n+1} in place (this
+            // lane alone reads body i's y_n, and no pair term of the launch does)
+            // and t += h (no stage of a gas-free fused step reads t)
+            if (a.commit >= 0 && (c.flags & F_COMMIT)) {
+                const_cast<double*>(a.pos[q])[i] = yo;
+                const_cast<double*>(a.vel[q])[i] = vo;
+                if (a.commit == 1 && q == 0 && i == di.off) const_cast<DiskCtl*>(a.ctl)[0].t = c.t + c.h;
+                return;
+            }
+        }
         *at(a.out[q]) = yo;
         *at(a.out[3 + q]) = vo;
         if (a.adaptive) {
